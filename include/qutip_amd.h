@@ -1,0 +1,197 @@
+/*
+ * qutip_amd C ABI -- MI355X (gfx950) kernels for QuTiP's time-evolution hot
+ * path: the right-hand-side product and the Runge-Kutta stage updates inside
+ * sesolve / mesolve / mcsolve.
+ *
+ * Conventions
+ *   - every function returns an int status: QA_OK (0) or a negative QA_ERR_*;
+ *     qa_last_error() returns the message of the last failure on the calling
+ *     thread.  No C++ exception crosses the ABI.
+ *   - "device pointer" arguments are HBM addresses (from qa_malloc or any
+ *     hipMalloc); "host pointer" arguments are ordinary CPU memory.
+ *   - complex numbers are interleaved (re, im) doubles (complex128) or floats
+ *     (complex64 operator values when vals_fp32 != 0) -- numpy's layout.
+ *   - sparse operators: CSR with int32 row pointers and column indices
+ *     (QuTiP's idxint, qutip/core/data/base.pxd:3-6), or SELL-64: rows in
+ *     slices of 64, slice s holding width_s entries per row stored
+ *     column-major (entry j of row 64 s + l at slice_ptr[s] + 64 j + l),
+ *     padding entries with value 0.
+ *   - stream: a hipStream_t passed as void*; NULL is the default stream.
+ *   - dense matrices are column-major ("Fortran" Dense,
+ *     qutip/core/data/dense.pxd:9-33), so vec(rho) is the buffer itself
+ *     (column stacking, qutip/core/superoperator.py:291-302).
+ */
+#ifndef QUTIP_AMD_H
+#define QUTIP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QA_VERSION 1
+
+#define QA_OK 0
+#define QA_ERR_HIP (-1)   /* a HIP runtime call failed */
+#define QA_ERR_ARG (-2)   /* invalid argument (shape, null pointer, range) */
+#define QA_ERR_NOMEM (-3) /* device allocation failed */
+#define QA_ERR_STATE (-4) /* integrator failure: see the status out-argument */
+#define QA_ERR_LIMIT (-5) /* a documented size limit was exceeded */
+
+/* ---- runtime ----------------------------------------------------------- */
+const char *qa_last_error(void);
+int qa_version(void);
+int qa_device_count(int *n);
+int qa_set_device(int dev);
+int qa_device_info(int dev, char *name, int name_len, int *cu_count, size_t *hbm_bytes);
+int qa_mem_info(size_t *free_bytes, size_t *total_bytes);
+int qa_malloc(void **ptr, size_t bytes);
+int qa_free(void *ptr);
+int qa_host_malloc(void **ptr, size_t bytes); /* pinned host memory */
+int qa_host_free(void *ptr);
+/* h2d / d2h return after the copy completed; d2d is asynchronous */
+int qa_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int qa_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int qa_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+/* zero-fill by a kernel (not hipMemset), so it may be recorded into graphs */
+int qa_fill_zero(void *dst, size_t bytes, void *stream);
+int qa_stream_create(void **stream);
+int qa_stream_destroy(void *stream);
+int qa_stream_sync(void *stream);
+int qa_device_sync(void);
+
+/* ---- data-layer kernels --------------------------------------------------
+ * These back the dispatcher specialisations of the device data types.      */
+
+/* out = (accumulate ? out : 0) + scale * A @ x, A CSR nrows x ncols with nnz
+ * entries.  Replaces _matmul_csr_vector (qutip/core/data/src/
+ * matmul_csr_vector.cpp:107-149) as called by matmul_csr_dense_dense
+ * (qutip/core/data/matmul.pyx:226-272) and imatmul_data_dense
+ * (matmul.pyx:1207-1215).  FP64 accumulation; lanes per row chosen from
+ * nnz / nrows. */
+int qa_csr_matvec_z(int nrows, int ncols, const int *rowptr, const int *col, const void *vals,
+                    int vals_fp32, int nnz, const void *x, void *out, double scale_re,
+                    double scale_im, int accumulate, void *stream);
+
+/* CSR times nvec dense columns (column-major, leading dims ldx, ldo).
+ * Replaces the per-column loop of matmul_csr_dense_dense
+ * (matmul.pyx:255-265) and _matmul_csr_dense_c_order
+ * (qutip/core/data/src/matmul_csr_dense.cpp:20-70). */
+int qa_csr_matmat_z(int nrows, int ncols, const int *rowptr, const int *col, const void *vals,
+                    int vals_fp32, int nvec, const void *x, int64_t ldx, void *out, int64_t ldo,
+                    double scale_re, double scale_im, int accumulate, void *stream);
+
+/* SELL-64 version of qa_csr_matvec_z (nslices = ceil(nrows / 64)). */
+int qa_sell_matvec_z(int nrows, int nslices, const int *slice_ptr, const int *col,
+                     const void *vals, int vals_fp32, const void *x, void *out, double scale_re,
+                     double scale_im, int accumulate, void *stream);
+
+/* out = base + sum_j coefs[j] * vecs[j] (real coefficients, nterms <= 16,
+ * base may be NULL).  Replaces the copy + zaxpy chain of
+ * Explicit_RungeKutta._accumulate (qutip/solver/integrator/explicit_rk.pyx:
+ * 432-437 via iadd_dense, qutip/core/data/add.pyx:203-218). */
+int qa_lincomb_z(int64_t n, const void *base, int nterms, const void *const *vecs,
+                 const double *coefs, void *out, void *stream);
+
+/* out = alpha x + beta y (y may be NULL).  Backs add / iadd / mul / imul
+ * (qutip/core/data/add.pyx:324-374, qutip/core/data/mul.pyx:103-139). */
+int qa_axpby_z(int64_t n, double are, double aim, const void *x, double bre, double bim,
+               const void *y, void *out, void *stream);
+
+/* sqrt(1/n sum (|diff| / (atol + rtol |state|))^2) into *result (host).
+ * Replaces wrmn_error_dense (qutip/core/data/ode.pyx:27-52). */
+int qa_wrmn_error_z(int64_t n, const void *diff, const void *state, double atol, double rtol,
+                    double *result, void *stream);
+
+/* sum |x|^2 into *result (host).  norm.l2 / frobenius are its square root
+ * (qutip/core/data/norm.pyx:127-135, 195-216). */
+int qa_norm2_z(int64_t n, const void *x, double *result, void *stream);
+
+/* sum conj(x) y into result2[0..1] (host).  Replaces inner_dense
+ * (qutip/core/data/inner.pyx:139-164). */
+int qa_inner_z(int64_t n, const void *x, const void *y, double *result2, void *stream);
+
+/* trace of a column-major d x d matrix into result2 (host).  Replaces
+ * trace_dense / trace_oper_ket_dense (qutip/core/data/trace.pyx:78-86). */
+int qa_trace_z(int d, const void *rho, double *result2, void *stream);
+
+/* <psi|O|psi> for O CSR n x n.  Replaces _expect_csr_dense_ket
+ * (qutip/core/data/expect.pyx:133-144). */
+int qa_expect_csr_ket_z(int n, const int *rowptr, const int *col, const void *vals,
+                        int vals_fp32, const void *psi, double *result2, void *stream);
+
+/* tr(O rho) for O CSR d x d and rho column-major d x d.  Replaces
+ * _expect_csr_dense_dm (qutip/core/data/expect.pyx:146-158). */
+int qa_expect_csr_dm_z(int d, const int *rowptr, const int *col, const void *vals, int vals_fp32,
+                       const void *rho, double *result2, void *stream);
+
+/* ---- fused Runge-Kutta engine ---------------------------------------------
+ * Replaces Explicit_RungeKutta (qutip/solver/integrator/explicit_rk.pyx) with
+ * its derivative QobjEvo.matmul_data (qutip/core/cy/qobjevo.pyx:1103-1116)
+ * for a constant generator A in SELL-64 form: dy/dt = A y.  The whole step
+ * loop (stages, error norm, step control, dense output, output-time
+ * interpolation and expectation functionals) runs on the device.
+ *
+ * The engine keeps the operator pointers it is given (the caller keeps the
+ * operator alive) and owns its own stream and work vectors. */
+typedef struct qa_rk_solver qa_rk_solver;
+
+/* Tableau as in explicit_rk.pyx:117-151: a[rk_extra][rk_extra], b[rk_step],
+ * c[rk_extra], e[rk_step] (NULL: fixed step), bi[rk_extra][dorder] (NULL: no
+ * dense output).  Options as in IntegratorVern7.integrator_options
+ * (qutip/solver/integrator/qutip_integrator.py:52-60). */
+int qa_rk_create(qa_rk_solver **out, int n, int nslices, const int *slice_ptr, const int *col,
+                 const void *vals, int vals_fp32, int order, int rk_step, int rk_extra,
+                 int dorder, const double *a, const double *b, const double *c,
+                 const double *e, const double *bi, double rtol, double atol, int nsteps,
+                 double first_step, double min_step, double max_step, int interpolate);
+int qa_rk_destroy(qa_rk_solver *s);
+void *qa_rk_stream(qa_rk_solver *s);
+
+/* Output functionals evaluated at every output time: functional f has
+ * entries p in [fptr[f], fptr[f+1]) with value
+ *   kind 0 (linear):    sum_p w_p y[cols_p]            -- tr(O rho), tr(rho)
+ *   kind 1 (quadratic): sum_p w_p conj(y[rows_p]) y[cols_p] -- <psi|O|psi>
+ * (host arrays; w complex interleaved).  Replaces the per-output e_ops
+ * evaluation of Result.add (qutip/solver/result.py:303-340). */
+int qa_rk_set_functionals(qa_rk_solver *s, int nfun, const int *kinds, const int *fptr,
+                          const int *rows, const int *cols, const double *w);
+
+/* set_initial_value: state y0 (device, n complex128) at t0, first step
+ * estimated as in _estimate_first_step (explicit_rk.pyx:204-276). */
+int qa_rk_set_state(qa_rk_solver *s, double t0, const void *y0);
+
+/* integrate(t) for each of times[0..ntimes-1] (increasing, host array) --
+ * Integrator.run (qutip/solver/integrator/integrator.py:197-212).  With
+ * single_step != 0 and ntimes == 1 this is integrate(t, step=True)
+ * (mcstep).  fun_out (host, ntimes x nfun complex) receives the
+ * functionals; states_out (device, ntimes x ceil(n/64)*64 complex, or NULL)
+ * the states.  *status gets the integrator status (explicit_rk.pxd:5-12),
+ * *t_reached the time of the last output.  Returns QA_ERR_STATE when the
+ * integration failed (status < 0). */
+int qa_rk_run(qa_rk_solver *s, int ntimes, const double *times, int single_step,
+              double *fun_out, void *states_out, int *status, double *t_reached);
+
+/* get_state: time and state (device, n complex128) of the last output. */
+int qa_rk_get_state(qa_rk_solver *s, double *t, void *y_out);
+
+/* stats[0..7]: attempts, accepted steps, dense-output preparations, RHS
+ * evaluations, graph launches, status, output index, phase. */
+int qa_rk_stats(qa_rk_solver *s, int64_t *stats);
+
+/* Average duration (ms, HIP events on the engine stream, `reps` launches) of
+ * each stage kernel into ms_per_stage[0 .. rk_extra-1] (rk_step entries when
+ * there is no dense output).  Clobbers the integration state: call
+ * qa_rk_set_state afterwards. */
+int qa_rk_profile_stages(qa_rk_solver *s, int reps, double *ms_per_stage);
+
+/* Enqueue `count` loop iterations without the graph and wait (profiling). */
+int qa_rk_iterate(qa_rk_solver *s, int count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUTIP_AMD_H */

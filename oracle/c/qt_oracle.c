@@ -1,0 +1,350 @@
+/*
+ * C restatement of QuTiP's CPU time-evolution path -- TEST ORACLE and the
+ * timed CPU baseline.  Never linked into the product.
+ *
+ * What it restates (all paths relative to the reference checkout):
+ *   qto_csr_matvec    qutip/core/data/src/matmul_csr_vector.cpp:107-149
+ *                     (the portable path; the reference build sets no -m
+ *                     flags, meson.build:23-29): out[r] += scale * sum_p
+ *                     data[p] * vec[col[p]], real/imag split, row order.
+ *   qto_rk_solve      qutip/solver/integrator/explicit_rk.pyx (whole file):
+ *                     set_initial_value :204-230, _estimate_first_step
+ *                     :232-276, integrate :278-331, _step_in_err :333-356,
+ *                     _compute_step :358-389, _error :391-397,
+ *                     _prep_dense_out :399-410, _interpolate_step :412-430,
+ *                     _accumulate :432-437 with iadd_data's zero skip
+ *                     :34-44, _get_timestep :439-449, _recompute_safe_step
+ *                     :451-467; the RHS call per stage is the reference's
+ *                     zero-fill (imul_data) + accumulate spmv
+ *                     (qutip/core/data/matmul.pyx:1207-1215), the stage
+ *                     combines are copy + one zaxpy per non-zero coefficient
+ *                     (qutip/core/data/add.pyx:203-218).
+ *   wrmn_error        qutip/core/data/ode.pyx:27-52.
+ *   norms             frobenius (qutip/core/data/norm.pyx, dznrm2): plain
+ *                     sqrt(sum |y|^2) here.
+ *
+ * Output: at every time of tlist, a set of linear functionals
+ * f_k(y) = sum_i w_i y[idx_i] of the state (for a vectorised density
+ * matrix, tr(O rho) and tr(rho) are such functionals).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct { double re, im; } cplx;
+
+void qto_csr_matvec(int nrows, const int *row_index, const int *col_index,
+                    const double *data_, const double *vec_, double scale_re,
+                    double scale_im, double *out_)
+{
+    const cplx *data = (const cplx *)data_;
+    const cplx *vec = (const cplx *)vec_;
+    cplx *out = (cplx *)out_;
+    for (int row = 0; row < nrows; row++) {
+        double dot_re = 0.0, dot_im = 0.0;
+        for (int p = row_index[row]; p < row_index[row + 1]; p++) {
+            double dr = data[p].re, di = data[p].im;
+            double vr = vec[col_index[p]].re, vi = vec[col_index[p]].im;
+            dot_re += dr * vr - di * vi;
+            dot_im += dr * vi + di * vr;
+        }
+        out[row].re += scale_re * dot_re - scale_im * dot_im;
+        out[row].im += scale_re * dot_im + scale_im * dot_re;
+    }
+}
+
+enum { AT_FRONT = 2, INTERPOLATED = 1, NORMAL = 0, TOO_MUCH_WORK = -1,
+       DT_UNDERFLOW = -2, OUTSIDE_RANGE = -3, NOT_INITIATED = -4 };
+
+typedef struct {
+    int n;
+    const int *rowptr, *col;
+    const double *vals;
+    /* tableau */
+    int order, rk_step, rk_extra, dorder, interpolate, adaptive, fsal;
+    const double *a, *b, *c, *e, *bi;
+    double rtol, atol, first_step, min_step, max_step;
+    int max_numsteps;
+    /* state */
+    cplx **k, *y, *y_prev, *y_front, *y_temp, *k_fsal;
+    double t, t_prev, t_front, dt_safe, dt_int;
+    int status;
+    long n_rhs, n_attempts;
+} rk_t;
+
+static void zero(cplx *v, int n) { memset(v, 0, sizeof(cplx) * (size_t)n); }
+static void copy_to(const cplx *src, cplx *dst, int n)
+{
+    for (int i = 0; i < n; i++) dst[i] = src[i];
+}
+/* left += factor * right, skipped when factor == 0 (explicit_rk.pyx:34-44) */
+static void iadd(cplx *left, const cplx *right, double f, int n)
+{
+    if (f == 0.0) return;
+    for (int i = 0; i < n; i++) {
+        left[i].re += f * right[i].re;
+        left[i].im += f * right[i].im;
+    }
+}
+static double frob(const cplx *v, int n)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += v[i].re * v[i].re + v[i].im * v[i].im;
+    return sqrt(s);
+}
+static void rhs(rk_t *s, const cplx *y, cplx *out)
+{
+    s->n_rhs++;
+    zero(out, s->n);
+    qto_csr_matvec(s->n, s->rowptr, s->col, s->vals, (const double *)y, 1.0, 0.0,
+                   (double *)out);
+}
+static void accumulate(rk_t *s, cplx *target, const double *factors, double dt, int size)
+{
+    for (int i = 0; i < size; i++) iadd(target, s->k[i], dt * factors[i], s->n);
+}
+static double wrmn(const cplx *diff, const cplx *state, double atol, double rtol, int n)
+{
+    double sum = 0.0;
+    for (int i = 0; i < n; i++) {
+        double ad = hypot(diff[i].re, diff[i].im);
+        double as = hypot(state[i].re, state[i].im);
+        double q = ad / (atol + rtol * as);
+        sum += q * q;
+    }
+    return sqrt(sum / n);
+}
+
+static double estimate_first_step(rk_t *s, const cplx *y0)
+{
+    if (!s->adaptive) return 0.0;
+    int n = s->n;
+    double norm = frob(y0, n);
+    double tol = s->atol + norm * s->rtol;
+    if (s->fsal) copy_to(s->k_fsal, s->k[0], n);
+    else rhs(s, y0, s->k[0]);
+    if (norm <= s->atol) norm = 1;
+    double tmp_norm = frob(s->k[0], n);
+    double factorial = 1.0;
+    for (int i = 1; i <= s->order; i++) factorial *= i;
+    double dt1, dt2;
+    if (tmp_norm >= s->atol * 1e-6)
+        dt1 = pow(tol * factorial * pow(norm, s->order), 1.0 / (s->order + 1)) / tmp_norm;
+    else
+        dt1 = pow(tol * factorial, 1.0 / (s->order + 1)) * norm * 0.5;
+    copy_to(y0, s->y_temp, n);
+    iadd(s->y_temp, s->k[0], dt1 / 100, n);
+    rhs(s, s->y_temp, s->k[1]);
+    tmp_norm = frob(s->k[1], n);
+    if (tmp_norm >= s->atol * 1e-6)
+        dt2 = pow(tol * factorial * pow(norm, s->order), 1.0 / (s->order + 1)) / tmp_norm;
+    else
+        dt2 = dt1;
+    double dt = dt1 < dt2 ? dt1 : dt2;
+    if (s->max_step) dt = dt < s->max_step ? dt : s->max_step;
+    if (s->min_step) dt = dt > s->min_step ? dt : s->min_step;
+    return dt;
+}
+
+static double compute_step(rk_t *s, double dt)
+{
+    int n = s->n, ns = s->rk_extra;
+    s->n_attempts++;
+    if (s->fsal) copy_to(s->k_fsal, s->k[0], n);
+    else rhs(s, s->y_prev, s->k[0]);
+    for (int i = 1; i < s->rk_step; i++) {
+        copy_to(s->y_prev, s->y_temp, n);
+        accumulate(s, s->y_temp, s->a + (size_t)i * ns, dt, i);
+        rhs(s, s->y_temp, s->k[i]);
+    }
+    copy_to(s->y_prev, s->y_front, n);
+    accumulate(s, s->y_front, s->b, dt, s->rk_step);
+    s->t_front = s->t_prev + dt;
+    if (!s->adaptive) return 0.0;
+    zero(s->y_temp, n);
+    accumulate(s, s->y_temp, s->e, dt, s->rk_step);
+    return wrmn(s->y_temp, s->y_front, s->atol, s->rtol, n);
+}
+
+static void recompute_safe_step(rk_t *s, double err, double dt)
+{
+    double factor;
+    if (!s->adaptive) return;
+    if (err == 0) factor = 10;
+    else {
+        factor = 0.9 * pow(err, -1.0 / (s->order + 1));
+        factor = factor < 10 ? factor : 10;
+        factor = factor > 0.2 ? factor : 0.2;
+    }
+    s->dt_safe = dt * factor;
+    if (s->max_step) s->dt_safe = s->dt_safe < s->max_step ? s->dt_safe : s->max_step;
+    if (s->min_step) s->dt_safe = s->dt_safe > s->min_step ? s->dt_safe : s->min_step;
+}
+
+static double get_timestep(rk_t *s, double t)
+{
+    double dt_needed = t - s->t_prev;
+    if (!s->adaptive) return dt_needed;
+    if (s->interpolate) return s->dt_safe;
+    if (dt_needed <= s->dt_safe) return dt_needed;
+    return dt_needed / ((int)(dt_needed / s->dt_safe) + 1);
+}
+
+static int step_in_err(rk_t *s, double t, int max_step)
+{
+    double error = 1;
+    int nsteps = 0;
+    while (error >= 1) {
+        double dt = get_timestep(s, t);
+        error = compute_step(s, dt);
+        s->dt_int = dt;
+        recompute_safe_step(s, error, dt);
+        if (dt == s->min_step && error > 1) { s->status = DT_UNDERFLOW; break; }
+        nsteps++;
+        if (nsteps > max_step) { s->status = TOO_MUCH_WORK; break; }
+    }
+    if (s->fsal) copy_to(s->k[s->rk_step - 1], s->k_fsal, s->n);
+    return nsteps;
+}
+
+static void prep_dense_out(rk_t *s)
+{
+    int n = s->n, ns = s->rk_extra;
+    double dt = s->dt_int;
+    for (int i = s->rk_step; i < s->rk_extra; i++) {
+        copy_to(s->y_prev, s->y_temp, n);
+        accumulate(s, s->y_temp, s->a + (size_t)i * ns, dt, i);
+        rhs(s, s->y_temp, s->k[i]);
+    }
+}
+
+static void interpolate_step(rk_t *s, double t, cplx *out)
+{
+    double bf[64];
+    double tau = (t - s->t_prev) / s->dt_int;
+    for (int i = 0; i < s->rk_extra; i++) {
+        double v = 0.0;
+        for (int j = s->dorder - 1; j >= 0; j--) {
+            v += s->bi[(size_t)i * s->dorder + j];
+            v *= tau;
+        }
+        bf[i] = v;
+    }
+    copy_to(s->y_prev, out, s->n);
+    accumulate(s, out, bf, s->dt_int, s->rk_extra);
+}
+
+static void integrate(rk_t *s, double t, int step)
+{
+    int nsteps_left = s->max_numsteps;
+    if (t == s->t) return;
+    if (t < s->t_prev) { s->status = OUTSIDE_RANGE; return; }
+    if (s->interpolate && t < s->t_front) {
+        if (s->status != INTERPOLATED) {
+            prep_dense_out(s);
+            s->status = INTERPOLATED;
+        }
+        interpolate_step(s, t, s->y);
+        s->t = t;
+        return;
+    }
+    s->status = NORMAL;
+    if (step && s->t < s->t_front && t > s->t_front) t = s->t_front;
+    while (s->t_front < t && s->status >= 0) {
+        copy_to(s->y_front, s->y_prev, s->n);
+        s->t_prev = s->t_front;
+        nsteps_left -= step_in_err(s, t, nsteps_left);
+        if (step) break;
+    }
+    if (s->status < 0) return;
+    if (s->t_front > t) {
+        prep_dense_out(s);
+        s->status = INTERPOLATED;
+        s->t = t;
+        interpolate_step(s, t, s->y);
+    } else {
+        s->status = AT_FRONT;
+        s->t = s->t_front;
+        copy_to(s->y_front, s->y, s->n);
+    }
+}
+
+static void eval_functionals(const cplx *y, int nfun, const int *fptr, const int *fidx,
+                             const cplx *fw, cplx *out)
+{
+    for (int f = 0; f < nfun; f++) {
+        double re = 0, im = 0;
+        for (int p = fptr[f]; p < fptr[f + 1]; p++) {
+            cplx w = fw[p], v = y[fidx[p]];
+            re += w.re * v.re - w.im * v.im;
+            im += w.re * v.im + w.im * v.re;
+        }
+        out[f].re = re;
+        out[f].im = im;
+    }
+}
+
+/* Solve dy/dt = A y over tlist; writes nfun functionals per output time into
+ * fun_out[t * nfun + f] (complex interleaved) and the final state into
+ * y_final.  stats[0..2] = RHS calls, step attempts, final status.
+ * Returns the integrator status (>= 0 on success). */
+int qto_rk_solve(int n, const int *rowptr, const int *col, const double *vals,
+                 const double *y0, int ntimes, const double *tlist,
+                 int order, int rk_step, int rk_extra, int dorder,
+                 const double *a, const double *b, const double *c,
+                 const double *e, const double *bi,
+                 double rtol, double atol, int nsteps, double first_step,
+                 double min_step, double max_step, int interpolate,
+                 int nfun, const int *fun_ptr, const int *fun_idx, const double *fun_w,
+                 double *fun_out, double *y_final, long *stats)
+{
+    rk_t s;
+    memset(&s, 0, sizeof(s));
+    s.n = n; s.rowptr = rowptr; s.col = col; s.vals = vals;
+    s.order = order; s.rk_step = rk_step; s.rk_extra = rk_extra; s.dorder = dorder;
+    s.a = a; s.b = b; s.c = c; s.e = e; s.bi = bi;
+    s.adaptive = e != NULL;
+    s.interpolate = (bi != NULL) && interpolate;
+    s.rtol = rtol; s.atol = atol; s.first_step = first_step;
+    s.min_step = min_step ? min_step : 1e-15;
+    s.max_step = max_step; s.max_numsteps = nsteps;
+    s.dt_safe = atol;
+    /* first_same_as_last: np.allclose(a[rk_step-1, :rk_step], b, atol=1e-14) */
+    s.fsal = 1;
+    for (int j = 0; j < rk_step; j++) {
+        double x = a[(size_t)(rk_step - 1) * rk_extra + j], yv = b[j];
+        if (fabs(x - yv) > 1e-14 + 1e-5 * fabs(yv)) s.fsal = 0;
+    }
+    int nk = s.interpolate ? rk_extra : rk_step;
+    s.k = (cplx **)malloc(sizeof(cplx *) * (size_t)nk);
+    for (int i = 0; i < nk; i++) s.k[i] = (cplx *)calloc((size_t)n, sizeof(cplx));
+    s.y = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    s.y_prev = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    s.y_front = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    s.y_temp = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    s.k_fsal = (cplx *)calloc((size_t)n, sizeof(cplx));
+    /* set_initial_value */
+    copy_to((const cplx *)y0, s.y, n);
+    copy_to(s.y, s.y_prev, n);
+    copy_to(s.y, s.y_front, n);
+    s.t = s.t_prev = s.t_front = tlist[0];
+    s.dt_int = 0;
+    s.status = NORMAL;
+    if (s.fsal) rhs(&s, s.y, s.k_fsal);
+    s.dt_safe = s.first_step ? s.first_step : estimate_first_step(&s, s.y);
+
+    cplx *fo = (cplx *)fun_out;
+    eval_functionals(s.y, nfun, fun_ptr, fun_idx, (const cplx *)fun_w, fo);
+    for (int it = 1; it < ntimes; it++) {
+        integrate(&s, tlist[it], 0);
+        if (s.status < 0) break;
+        eval_functionals(s.y, nfun, fun_ptr, fun_idx, (const cplx *)fun_w,
+                         fo + (size_t)it * nfun);
+    }
+    if (y_final) copy_to(s.y, (cplx *)y_final, n);
+    if (stats) { stats[0] = s.n_rhs; stats[1] = s.n_attempts; stats[2] = s.status; }
+    int st = s.status;
+    for (int i = 0; i < nk; i++) free(s.k[i]);
+    free(s.k); free(s.y); free(s.y_prev); free(s.y_front); free(s.y_temp); free(s.k_fsal);
+    return st;
+}

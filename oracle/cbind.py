@@ -1,0 +1,128 @@
+"""ctypes bindings of the C oracle (oracle/c/qt_oracle.c) and of the
+reference's own spmv built into oracle/_ref (test infrastructure only)."""
+import ctypes as C
+import pathlib
+
+import numpy as np
+import scipy.sparse as sp
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_ORACLE = HERE / "c" / "libqt_oracle.so"
+LIB_REF = HERE / "_ref" / "libqtref_spmv.so"
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def _load(path):
+    if not path.exists():
+        raise FileNotFoundError(f"{path} is not built (run `make -C oracle`)")
+    return C.CDLL(str(path))
+
+
+_oracle = None
+_ref = None
+
+
+def oracle_lib():
+    global _oracle
+    if _oracle is None:
+        lib = _load(LIB_ORACLE)
+        lib.qto_csr_matvec.argtypes = [C.c_int, _ip, _ip, _dp, _dp, C.c_double,
+                                       C.c_double, _dp]
+        lib.qto_csr_matvec.restype = None
+        lib.qto_rk_solve.restype = C.c_int
+        _oracle = lib
+    return _oracle
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        lib = _load(LIB_REF)
+        lib.qtref_csr_matvec.argtypes = [_dp, _ip, _ip, _dp, C.c_double, C.c_double,
+                                         _dp, C.c_int]
+        lib.qtref_csr_matvec.restype = None
+        _ref = lib
+    return _ref
+
+
+def ref_available():
+    return LIB_REF.exists()
+
+
+def _csr_parts(A):
+    A = sp.csr_matrix(A, dtype=np.complex128)
+    return (np.ascontiguousarray(A.indptr, dtype=np.int32),
+            np.ascontiguousarray(A.indices, dtype=np.int32),
+            np.ascontiguousarray(A.data).view(np.float64))
+
+
+def csr_matvec(A, x, out=None, scale=1.0):
+    """out += scale * A @ x with the C restatement."""
+    rp, ci, dv = _csr_parts(A)
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros(A.shape[0], dtype=np.complex128) if out is None else out
+    oracle_lib().qto_csr_matvec(A.shape[0], rp, ci, dv, x.view(np.float64),
+                                float(np.real(scale)), float(np.imag(scale)),
+                                out.view(np.float64))
+    return out
+
+
+def ref_csr_matvec(A, x, out=None, scale=1.0):
+    """out += scale * A @ x with the reference's own C++ kernel."""
+    rp, ci, dv = _csr_parts(A)
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros(A.shape[0], dtype=np.complex128) if out is None else out
+    ref_lib().qtref_csr_matvec(dv, ci, rp, x.view(np.float64), float(np.real(scale)),
+                               float(np.imag(scale)), out.view(np.float64), A.shape[0])
+    return out
+
+
+def functionals_from_ops(e_ops, d, with_trace=True):
+    """Sparse weights w with tr(O rho) = sum_i w_i vec(rho)[i]:
+    vec index c*d + r holds rho[r, c], and tr(O rho) = sum O[r, c] rho[c, r],
+    so weight O[r, c] goes to index r*d + c."""
+    ptr, idx, w = [0], [], []
+    ops = list(e_ops)
+    if with_trace:
+        ops = ops + [sp.identity(d, dtype=np.complex128, format="csr")]
+    for O in ops:
+        O = sp.coo_matrix(O)
+        idx.extend((O.row * d + O.col).tolist())
+        w.extend(O.data.tolist())
+        ptr.append(len(idx))
+    return (np.asarray(ptr, dtype=np.int32), np.asarray(idx, dtype=np.int32),
+            np.ascontiguousarray(np.asarray(w, dtype=np.complex128)).view(np.float64))
+
+
+def rk_solve(A, y0, tlist, tableau, fun, rtol=1e-6, atol=1e-8, nsteps=1000,
+             first_step=0.0, min_step=0.0, max_step=0.0, interpolate=True):
+    """Run the C restatement of explicit_rk.pyx on dy/dt = A y.
+
+    ``fun`` = (ptr, idx, w) functionals; returns (values[ntimes, nfun],
+    y_final, stats dict)."""
+    lib = oracle_lib()
+    rp, ci, dv = _csr_parts(A)
+    n = A.shape[0]
+    y0 = np.ascontiguousarray(y0, dtype=np.complex128)
+    tl = np.ascontiguousarray(tlist, dtype=np.float64)
+    a = np.ascontiguousarray(tableau["a"], dtype=np.float64)
+    b = np.ascontiguousarray(tableau["b"], dtype=np.float64)
+    c = np.ascontiguousarray(tableau["c"], dtype=np.float64)
+    e = np.ascontiguousarray(tableau["e"], dtype=np.float64)
+    bi = np.ascontiguousarray(tableau["bi"], dtype=np.float64)
+    fptr, fidx, fw = fun
+    nfun = len(fptr) - 1
+    out = np.zeros((len(tl), nfun), dtype=np.complex128)
+    yf = np.zeros(n, dtype=np.complex128)
+    stats = np.zeros(3, dtype=np.int64)
+    P = lambda arr: arr.ctypes.data_as(C.c_void_p)
+    st = lib.qto_rk_solve(
+        C.c_int(n), P(rp), P(ci), P(dv), P(y0), C.c_int(len(tl)), P(tl),
+        C.c_int(tableau["order"]), C.c_int(len(b)), C.c_int(len(c)), C.c_int(bi.shape[1]),
+        P(a), P(b), P(c), P(e), P(bi),
+        C.c_double(rtol), C.c_double(atol), C.c_int(nsteps), C.c_double(first_step),
+        C.c_double(min_step), C.c_double(max_step), C.c_int(int(interpolate)),
+        C.c_int(nfun), P(fptr), P(fidx), P(fw), P(out), P(yf), P(stats))
+    return out, yf, {"n_rhs": int(stats[0]), "n_attempts": int(stats[1]), "status": int(st)}

@@ -1,0 +1,404 @@
+"""Host-side quantum objects: the part of QuTiP's object model the solvers
+need, restated so the solvers run where qutip itself is not installed.
+
+* ``Qobj``: matrix (scipy CSR or numpy dense, host) with ``dims``;
+  arithmetic and the flags used by the solvers (qutip/core/qobj.py).
+* ``QobjEvo``: constant operator wrapper with ``matmul_data`` /
+  ``expect_data`` (qutip/core/cy/qobjevo.pyx:1018-1116).  Its device copy
+  is made once and reused by every derivative call.
+* operators and states with QuTiP's conventions (qutip/core/operators.py:
+  353-505, 776; qutip/core/states.py basis): ``sigmaz = diag(1, -1)``,
+  ``sigmam = [[0, 0], [1, 0]]``, ``basis(2, 0)`` spin up / excited.
+* superoperators in the column-stacking convention
+  (qutip/core/superoperator.py:51-229, 357-415): ``spre(A) = I (x) A``,
+  ``spost(A) = A^T (x) I``, ``liouvillian``, ``lindblad_dissipator``.
+"""
+import numbers
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import data as _data
+
+__all__ = ["Qobj", "QobjEvo", "basis", "ket2dm", "destroy", "create", "num", "qeye",
+           "sigmax", "sigmay", "sigmaz", "sigmam", "sigmap", "tensor", "spre", "spost",
+           "liouvillian", "lindblad_dissipator", "expect", "rand_herm", "rand_dm"]
+
+HERM_ATOL = 1e-12  # settings.core['atol'] (qutip/core/options.py:183-203)
+
+
+def _as_csr(m):
+    return sp.csr_matrix(m, dtype=np.complex128)
+
+
+class Qobj:
+    __array_priority__ = 100
+
+    def __init__(self, arr, dims=None, superrep=None, isherm=None, copy=True):
+        if isinstance(arr, Qobj):
+            dims = arr.dims if dims is None else dims
+            arr = arr.data
+        if sp.issparse(arr):
+            self.data = _as_csr(arr).copy() if copy else _as_csr(arr)
+        else:
+            a = np.array(arr, dtype=np.complex128, copy=copy)
+            if a.ndim == 1:
+                a = a.reshape(-1, 1)
+            self.data = a
+        self.shape = self.data.shape
+        if dims is None:
+            dims = [[self.shape[0]], [self.shape[1]]]
+        self.dims = [list(dims[0]), list(dims[1])]
+        self._isherm = isherm
+        self.superrep = superrep
+
+    # -- type flags (qutip/core/dimensions.py types) ------------------------------
+    @property
+    def isket(self):
+        return self.shape[1] == 1 and not self._is_super_dims()
+
+    @property
+    def isbra(self):
+        return self.shape[0] == 1 and self.shape[1] != 1
+
+    @property
+    def isoper(self):
+        return not self.isket and not self.isbra and not self.issuper
+
+    def _is_super_dims(self):
+        return isinstance(self.dims[0][0], list)
+
+    @property
+    def issuper(self):
+        return self._is_super_dims() and self.shape[0] == self.shape[1]
+
+    @property
+    def isoperket(self):
+        return self._is_super_dims() and self.shape[1] == 1
+
+    @property
+    def isherm(self):
+        if self._isherm is None:
+            if self.shape[0] != self.shape[1]:
+                self._isherm = False
+            else:
+                d = self.full()
+                self._isherm = bool(np.allclose(d, d.conj().T, atol=HERM_ATOL, rtol=0))
+        return self._isherm
+
+    # -- conversions ------------------------------------------------------------
+    def full(self):
+        return self.data.toarray() if sp.issparse(self.data) else np.asarray(self.data)
+
+    def to_csr(self):
+        return _as_csr(self.data)
+
+    def dag(self):
+        d = self.data.conj().T
+        return Qobj(d, dims=[self.dims[1], self.dims[0]], isherm=self._isherm)
+
+    def conj(self):
+        return Qobj(self.data.conj(), dims=self.dims)
+
+    def trans(self):
+        return Qobj(self.data.T, dims=[self.dims[1], self.dims[0]])
+
+    def tr(self):
+        return complex(self.data.diagonal().sum()) if sp.issparse(self.data) else \
+            complex(np.trace(self.data))
+
+    def norm(self):
+        if self.isket or self.isbra:
+            return float(np.linalg.norm(self.full()))
+        # trace norm for operators
+        return float(np.sum(np.linalg.svd(self.full(), compute_uv=False)))
+
+    def unit(self):
+        return self * (1 / (self.norm() if (self.isket or self.isbra) else self.tr()))
+
+    # -- arithmetic ---------------------------------------------------------------
+    def __add__(self, other):
+        if isinstance(other, Qobj):
+            if other.shape != self.shape:
+                raise ValueError("incompatible shapes")
+            return Qobj(self.data + other.data, dims=self.dims)
+        if isinstance(other, numbers.Number):
+            if other == 0:
+                return self.copy()
+            return self + other * qeye_like(self)
+        return NotImplemented
+
+    __radd__ = __add__
+
+    def __sub__(self, other):
+        return self + (-1) * other
+
+    def __rsub__(self, other):
+        return (-1) * self + other
+
+    def __neg__(self):
+        return (-1) * self
+
+    def __mul__(self, other):
+        if isinstance(other, numbers.Number):
+            return Qobj(self.data * other, dims=self.dims)
+        if isinstance(other, Qobj):
+            return self @ other
+        return NotImplemented
+
+    def __rmul__(self, other):
+        if isinstance(other, numbers.Number):
+            return Qobj(self.data * other, dims=self.dims)
+        return NotImplemented
+
+    def __truediv__(self, other):
+        return self * (1 / other)
+
+    def __matmul__(self, other):
+        if not isinstance(other, Qobj):
+            return NotImplemented
+        if self.shape[1] != other.shape[0]:
+            raise ValueError("incompatible shapes for matmul")
+        d = self.data @ other.data
+        if sp.issparse(d) and (not sp.issparse(self.data) or not sp.issparse(other.data)):
+            d = d.toarray()
+        return Qobj(d, dims=[self.dims[0], other.dims[1]])
+
+    def copy(self):
+        return Qobj(self.data, dims=self.dims)
+
+    def __repr__(self):
+        return f"Qobj(dims={self.dims}, shape={self.shape})"
+
+
+def qeye_like(q):
+    return Qobj(sp.identity(q.shape[0], dtype=np.complex128, format="csr"), dims=q.dims)
+
+
+# ---------------------------------------------------------------------------
+# states and operators
+# ---------------------------------------------------------------------------
+def basis(dims, n=0):
+    d = int(np.prod(dims)) if isinstance(dims, (list, tuple)) else int(dims)
+    v = np.zeros((d, 1), dtype=np.complex128)
+    v[n, 0] = 1.0
+    return Qobj(v, dims=[[d], [1]], copy=False)
+
+
+def ket2dm(psi):
+    return psi @ psi.dag()
+
+
+def destroy(N, offset=0):
+    d = sp.diags(np.sqrt(np.arange(offset + 1, N + offset, dtype=np.float64)), 1,
+                 shape=(N, N), dtype=np.complex128, format="csr")
+    return Qobj(d, dims=[[N], [N]])
+
+
+def create(N, offset=0):
+    return destroy(N, offset).dag()
+
+
+def num(N):
+    return Qobj(sp.diags(np.arange(N, dtype=np.float64), 0, format="csr", dtype=np.complex128),
+                dims=[[N], [N]], isherm=True)
+
+
+def qeye(N):
+    return Qobj(sp.identity(N, dtype=np.complex128, format="csr"), dims=[[N], [N]], isherm=True)
+
+
+def sigmax():
+    return Qobj(sp.csr_matrix(np.array([[0, 1], [1, 0]], dtype=np.complex128)), isherm=True)
+
+
+def sigmay():
+    return Qobj(sp.csr_matrix(np.array([[0, -1j], [1j, 0]], dtype=np.complex128)), isherm=True)
+
+
+def sigmaz():
+    return Qobj(sp.csr_matrix(np.array([[1, 0], [0, -1]], dtype=np.complex128)), isherm=True)
+
+
+def sigmam():
+    return Qobj(sp.csr_matrix(np.array([[0, 0], [1, 0]], dtype=np.complex128)))
+
+
+def sigmap():
+    return Qobj(sp.csr_matrix(np.array([[0, 1], [0, 0]], dtype=np.complex128)))
+
+
+def tensor(*args):
+    if len(args) == 1 and isinstance(args[0], (list, tuple)):
+        args = args[0]
+    out = args[0]
+    for q in args[1:]:
+        if sp.issparse(out.data) or sp.issparse(q.data):
+            d = sp.kron(_as_csr(out.data), _as_csr(q.data), format="csr")
+        else:
+            d = np.kron(out.data, q.data)
+        out = Qobj(d, dims=[out.dims[0] + q.dims[0], out.dims[1] + q.dims[1]])
+    return out
+
+
+# ---------------------------------------------------------------------------
+# superoperators (column stacking)
+# ---------------------------------------------------------------------------
+def _super_dims(A):
+    return [[A.dims[0], A.dims[1]], [A.dims[0], A.dims[1]]]
+
+
+def spre(A):
+    d = A.shape[0]
+    return Qobj(sp.kron(sp.identity(d, dtype=np.complex128, format="csr"), _as_csr(A.data),
+                        format="csr"), dims=_super_dims(A), superrep="super")
+
+
+def spost(A):
+    d = A.shape[0]
+    return Qobj(sp.kron(_as_csr(A.data).T, sp.identity(d, dtype=np.complex128, format="csr"),
+                        format="csr"), dims=_super_dims(A), superrep="super")
+
+
+def lindblad_dissipator(a, b=None):
+    """D[a] rho = a rho b^dag - 1/2 {b^dag a, rho} (superoperator.py:161-229)."""
+    b = a if b is None else b
+    ad_b = a.dag() @ b
+    return Qobj(_as_csr(spre(a).data @ spost(b.dag()).data)
+                - 0.5 * _as_csr(spre(ad_b).data) - 0.5 * _as_csr(spost(ad_b).data),
+                dims=_super_dims(a), superrep="super")
+
+
+def liouvillian(H=None, c_ops=()):
+    """-i (spre(H) - spost(H)) + sum_c D[c] (superoperator.py:51-142)."""
+    c_ops = list(c_ops or [])
+    if H is not None and H.issuper:
+        L = H.data.copy()
+        ref = H
+    elif H is not None:
+        L = -1j * (_as_csr(spre(H).data) - _as_csr(spost(H).data))
+        ref = H
+    else:
+        ref = c_ops[0]
+        L = sp.csr_matrix((ref.shape[0] ** 2,) * 2, dtype=np.complex128)
+    for c in c_ops:
+        L = L + _as_csr(lindblad_dissipator(c).data)
+    L = _as_csr(L)
+    L.sum_duplicates()
+    L.eliminate_zeros()
+    return Qobj(L, dims=ref.dims if ref.issuper else _super_dims(ref), superrep="super")
+
+
+def expect(oper, state):
+    """qutip.expect for Qobj inputs (qutip/core/expect.py:79-101), on host
+    data; real part when oper is Hermitian and the state is a ket or
+    Hermitian."""
+    O = _as_csr(oper.data)
+    if state.isket:
+        psi = state.full().ravel()
+        val = complex(np.vdot(psi, O @ psi))
+    else:
+        rho = state.full()
+        val = complex(O.multiply(rho.T).sum())
+    if oper.isherm and (state.isket or state.isherm):
+        return val.real
+    return val
+
+
+# ---------------------------------------------------------------------------
+# random objects used by the serving workload (qutip/core/random_objects.py)
+# ---------------------------------------------------------------------------
+def rand_herm(N, density=0.3, seed=None):
+    """Random sparse Hermitian matrix with about density*N^2 entries (eigenvalues
+    not fixed).  Same distributional family as random_objects.py:198-283
+    (complex Gaussian entries, symmetrised); the exact draws of the reference
+    are not reproduced."""
+    rng = np.random.default_rng(seed)
+    nvals = max(1, int(N * N * density / 2))
+    r = rng.integers(0, N, nvals)
+    c = rng.integers(0, N, nvals)
+    v = (rng.standard_normal(nvals) + 1j * rng.standard_normal(nvals)) / 2
+    M = sp.coo_matrix((v, (r, c)), shape=(N, N)).tocsr()
+    M = M + M.conj().T
+    M.sum_duplicates()
+    return Qobj(M, dims=[[N], [N]], isherm=True)
+
+
+def rand_dm(N, seed=None):
+    """Random density matrix from a Ginibre matrix, rho = G G^dag / tr
+    (random_objects.py:508-604, 'ginibre' distribution, full rank)."""
+    rng = np.random.default_rng(seed)
+    G = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    rho = G @ G.conj().T
+    rho /= np.trace(rho)
+    return Qobj(rho, dims=[[N], [N]], isherm=True)
+
+
+# ---------------------------------------------------------------------------
+# QobjEvo (constant systems)
+# ---------------------------------------------------------------------------
+class QobjEvo:
+    """Constant operator with the data-layer entry points the solvers call
+    (qutip/core/cy/qobjevo.pyx:1018-1131).  ``matmul_data`` and
+    ``expect_data`` take device data (HipDense) and run on the GPU; the
+    device copy of the operator is made on first use."""
+
+    def __init__(self, Q):
+        if isinstance(Q, QobjEvo):
+            Q = Q.qobj
+        if not isinstance(Q, Qobj):
+            raise TypeError("QobjEvo needs a Qobj")
+        self.qobj = Q
+        self.dims = Q.dims
+        self.shape = Q.shape
+        self._dev = None
+        self._dev_fp32 = None
+
+    isconstant = True
+
+    @property
+    def issuper(self):
+        return self.qobj.issuper
+
+    @property
+    def isoper(self):
+        return self.qobj.isoper
+
+    def __call__(self, t=0.0, args=None):
+        return self.qobj
+
+    def device_data(self, fp32=False):
+        if fp32:
+            if self._dev_fp32 is None:
+                self._dev_fp32 = _data.HipCSR(self.qobj.to_csr(), fp32=True)
+            return self._dev_fp32
+        if self._dev is None:
+            self._dev = _data.HipCSR(self.qobj.to_csr())
+        return self._dev
+
+    def matmul_data(self, t, state, out=None):
+        """out += self(t) @ state  (qobjevo.pyx:1103-1116)."""
+        return _data.matmul(self.device_data(), state, 1.0, out=out)
+
+    def expect_data(self, t, state):
+        return _data.expect(self.device_data(), state)
+
+    def __add__(self, other):
+        other = other.qobj if isinstance(other, QobjEvo) else other
+        return QobjEvo(self.qobj + other)
+
+    def __sub__(self, other):
+        other = other.qobj if isinstance(other, QobjEvo) else other
+        return QobjEvo(self.qobj - other)
+
+    def __mul__(self, other):
+        return QobjEvo(self.qobj * other)
+
+    __rmul__ = __mul__
+
+    def dag(self):
+        return QobjEvo(self.qobj.dag())
+
+    def __matmul__(self, other):
+        other = other.qobj if isinstance(other, QobjEvo) else other
+        return QobjEvo(self.qobj @ other)

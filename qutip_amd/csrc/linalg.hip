@@ -1,0 +1,451 @@
+// Data-layer kernels behind the dispatcher specialisations of the device
+// types (matmul / add / mul / expect / inner / norm / wrmn_error / trace).
+//
+// Each entry point cites the reference kernel or dispatcher it replaces.
+// Operators: CSR (int32 row pointers and columns, complex128 or complex64
+// values) and SELL-64 (rows in slices of one wavefront, entries stored
+// column-major inside a slice so the 64 lanes of a wave read consecutive
+// addresses; padding entries have value 0).
+#include <mutex>
+
+#include "qa_api_internal.h"
+#include "qa_common.h"
+
+using namespace qa;
+
+// ---------------------------------------------------------------------------
+// CSR matvec: G lanes per row (G = 1..64, power of two), FP64 accumulation.
+// out[r] = (accumulate ? out[r] : 0) + scale * sum_p A[r, col[p]] x[col[p]]
+// ---------------------------------------------------------------------------
+template <typename V, int G>
+__global__ __launch_bounds__(256) void csr_matvec_kernel(int nrows, const int *__restrict__ rowptr,
+                                                         const int *__restrict__ col,
+                                                         const V *__restrict__ vals,
+                                                         const double2 *__restrict__ x,
+                                                         double2 *__restrict__ out, double2 scale,
+                                                         int accumulate) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = gid / G;
+    const int sub = (int)(gid % G);
+    double2 acc = make_double2(0.0, 0.0);
+    if (row < nrows) {
+        const int beg = rowptr[row], end = rowptr[row + 1];
+        for (int p = beg + sub; p < end; p += G) cfma(acc, ld(vals, p), x[col[p]]);
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        acc.x += __shfl_xor(acc.x, off, QA_WAVE);
+        acc.y += __shfl_xor(acc.y, off, QA_WAVE);
+    }
+    if (row < nrows && sub == 0) {
+        double2 r = cmul(scale, acc);
+        if (accumulate) r = cadd(r, out[row]);
+        out[row] = r;
+    }
+}
+
+// CSR times a block of nvec dense columns (column-major, leading dimensions
+// ldx / ldo): one thread per (row, column).  Replaces the per-column loop of
+// matmul_csr_dense_dense (qutip/core/data/matmul.pyx:255-265).
+template <typename V>
+__global__ __launch_bounds__(256) void csr_matmat_kernel(int nrows, const int *__restrict__ rowptr,
+                                                         const int *__restrict__ col,
+                                                         const V *__restrict__ vals, int nvec,
+                                                         const double2 *__restrict__ x, int64_t ldx,
+                                                         double2 *__restrict__ out, int64_t ldo,
+                                                         double2 scale, int accumulate) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = blockIdx.y;
+    if (row >= nrows || v >= nvec) return;
+    const double2 *xv = x + (int64_t)v * ldx;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int p = rowptr[row]; p < rowptr[row + 1]; ++p) cfma(acc, ld(vals, p), xv[col[p]]);
+    double2 r = cmul(scale, acc);
+    double2 *o = out + (int64_t)v * ldo + row;
+    if (accumulate) r = cadd(r, *o);
+    *o = r;
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void sell_matvec_kernel(int nrows, int nslices,
+                                                          const int *__restrict__ slice_ptr,
+                                                          const int *__restrict__ col,
+                                                          const V *__restrict__ vals,
+                                                          const double2 *__restrict__ x,
+                                                          double2 *__restrict__ out, double2 scale,
+                                                          int accumulate) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = (int)(r / QA_WAVE), lane = (int)(r % QA_WAVE);
+    if (slice >= nslices) return;
+    double2 acc = sell_row_dot(slice_ptr, col, vals, x, slice, lane);
+    if (r < nrows) {
+        double2 o = cmul(scale, acc);
+        if (accumulate) o = cadd(o, out[r]);
+        out[r] = o;
+    }
+}
+
+// out = base + sum_j coef[j] * vecs[j]   (real coefficients, up to 16 terms)
+struct LinComb {
+    const double2 *v[16];
+    double c[16];
+};
+
+__global__ __launch_bounds__(256) void lincomb_kernel(int64_t n, const double2 *__restrict__ base,
+                                                      int nterms, LinComb lc,
+                                                      double2 *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double2 o = base ? base[i] : make_double2(0.0, 0.0);
+        for (int j = 0; j < nterms; ++j) raxpy(o, lc.c[j], lc.v[j][i]);
+        out[i] = o;
+    }
+}
+
+// out = alpha * x + beta * y  (complex scalars; y may be null)
+__global__ __launch_bounds__(256) void axpby_kernel(int64_t n, double2 alpha,
+                                                    const double2 *__restrict__ x, double2 beta,
+                                                    const double2 *__restrict__ y,
+                                                    double2 *__restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double2 o = cmul(alpha, x[i]);
+        if (y) o = cadd(o, cmul(beta, y[i]));
+        out[i] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions: per-workgroup partial sums, the last workgroup adds them in a
+// fixed order (deterministic) and writes res[0..1].
+// kind 0: wrmn sum (|d|/(atol + rtol |s|))^2 -> sqrt(sum / n)
+// kind 1: sum |x|^2
+// kind 2: sum conj(x) y (complex)
+// kind 3: sum over i of x[i * (d + 1)] (trace of a column-major d x d)
+// ---------------------------------------------------------------------------
+struct RedScratch {
+    double *partials;  // 2 * maxblocks
+    double *res;       // 2
+    unsigned int *counter;
+};
+
+__global__ __launch_bounds__(256) void reduce_kernel(int kind, int64_t n, const double2 *__restrict__ x,
+                                                     const double2 *__restrict__ y, double atol,
+                                                     double rtol, int64_t stride, RedScratch S) {
+    __shared__ double smem[8];
+    double sre = 0.0, sim = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (kind == 0) {
+            const double ad = sqrt(cabs2(x[i])), as = sqrt(cabs2(y[i]));
+            const double q = ad / (atol + rtol * as);
+            sre += q * q;
+        } else if (kind == 1) {
+            sre += cabs2(x[i]);
+        } else if (kind == 2) {
+            const double2 a = x[i], b = y[i];
+            sre += a.x * b.x + a.y * b.y;
+            sim += a.x * b.y - a.y * b.x;
+        } else {
+            const double2 a = x[i * stride];
+            sre += a.x;
+            sim += a.y;
+        }
+    }
+    sre = block_sum(sre, smem);
+    sim = block_sum(sim, smem);
+    if (threadIdx.x == 0) {
+        S.partials[2 * blockIdx.x] = sre;
+        S.partials[2 * blockIdx.x + 1] = sim;
+    }
+    if (last_block_done(S.counter) && threadIdx.x == 0) {
+        double tre = 0.0, tim = 0.0;
+        for (unsigned b = 0; b < gridDim.x; ++b) {
+            tre += S.partials[2 * b];
+            tim += S.partials[2 * b + 1];
+        }
+        if (kind == 0) tre = sqrt(tre / (double)n);
+        S.res[0] = tre;
+        S.res[1] = tim;
+    }
+}
+
+// <psi|O|psi> for a ket (kind 0) and tr(O rho) for a column-major d x d rho
+// (kind 1): one thread per row of O.
+template <typename V>
+__global__ __launch_bounds__(256) void expect_csr_kernel(int kind, int nrows,
+                                                         const int *__restrict__ rowptr,
+                                                         const int *__restrict__ col,
+                                                         const V *__restrict__ vals,
+                                                         const double2 *__restrict__ st,
+                                                         RedScratch S) {
+    __shared__ double smem[8];
+    double sre = 0.0, sim = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        double2 acc = make_double2(0.0, 0.0);
+        if (kind == 0) {
+            for (int p = rowptr[r]; p < rowptr[r + 1]; ++p) cfma(acc, ld(vals, p), st[col[p]]);
+            const double2 c = st[r];
+            sre += c.x * acc.x + c.y * acc.y;
+            sim += c.x * acc.y - c.y * acc.x;
+        } else {
+            // sum_c O[r, c] rho[c, r]; rho[c, r] lives at r * d + c
+            for (int p = rowptr[r]; p < rowptr[r + 1]; ++p)
+                cfma(acc, ld(vals, p), st[r * (int64_t)nrows + col[p]]);
+            sre += acc.x;
+            sim += acc.y;
+        }
+    }
+    sre = block_sum(sre, smem);
+    sim = block_sum(sim, smem);
+    if (threadIdx.x == 0) {
+        S.partials[2 * blockIdx.x] = sre;
+        S.partials[2 * blockIdx.x + 1] = sim;
+    }
+    if (last_block_done(S.counter) && threadIdx.x == 0) {
+        double tre = 0.0, tim = 0.0;
+        for (unsigned b = 0; b < gridDim.x; ++b) {
+            tre += S.partials[2 * b];
+            tim += S.partials[2 * b + 1];
+        }
+        S.res[0] = tre;
+        S.res[1] = tim;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kMaxRedBlocks = 512;
+std::mutex g_red_mutex;
+struct RedState {
+    int device = -1;
+    void *buf = nullptr;
+} g_red[16];
+
+int red_scratch(RedScratch *S) {
+    int dev = 0;
+    QA_HIP(hipGetDevice(&dev));
+    QA_REQUIRE(dev >= 0 && dev < 16, "device index %d out of range", dev);
+    RedState &st = g_red[dev];
+    if (!st.buf) {
+        QA_HIP(hipMalloc(&st.buf, (2 * kMaxRedBlocks + 2) * sizeof(double) + 256));
+        QA_HIP(hipMemset(st.buf, 0, (2 * kMaxRedBlocks + 2) * sizeof(double) + 256));
+        st.device = dev;
+    }
+    char *b = (char *)st.buf;
+    S->partials = (double *)b;
+    S->res = (double *)(b + 2 * kMaxRedBlocks * sizeof(double));
+    S->counter = (unsigned int *)(b + (2 * kMaxRedBlocks + 2) * sizeof(double));
+    return QA_OK;
+}
+
+unsigned red_blocks(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > kMaxRedBlocks) b = kMaxRedBlocks;
+    return (unsigned)b;
+}
+
+int finish_reduction(RedScratch &S, double *result, int ncomp, hipStream_t s) {
+    QA_CHECK_LAUNCH();
+    QA_HIP(hipMemcpyAsync(result, S.res, ncomp * sizeof(double), hipMemcpyDeviceToHost, s));
+    QA_HIP(hipStreamSynchronize(s));
+    return QA_OK;
+}
+
+template <typename V>
+int launch_csr_matvec(int nrows, const int *rowptr, const int *col, const V *vals,
+                      const double2 *x, double2 *out, double2 scale, int accumulate, int G,
+                      hipStream_t s) {
+    const int64_t threads = (int64_t)nrows * G;
+    const unsigned blocks = (unsigned)((threads + 255) / 256);
+    if (blocks == 0) return QA_OK;
+#define QA_CSR_CASE(g)                                                                   \
+    case g:                                                                              \
+        hipLaunchKernelGGL((csr_matvec_kernel<V, g>), dim3(blocks), dim3(256), 0, s, nrows, \
+                           rowptr, col, vals, x, out, scale, accumulate);                \
+        break;
+    switch (G) {
+        QA_CSR_CASE(1)
+        QA_CSR_CASE(2)
+        QA_CSR_CASE(4)
+        QA_CSR_CASE(8)
+        QA_CSR_CASE(16)
+        QA_CSR_CASE(32)
+        QA_CSR_CASE(64)
+        default:
+            qa::set_error("csr_matvec: lanes per row must be a power of two <= 64, got %d", G);
+            return QA_ERR_ARG;
+    }
+#undef QA_CSR_CASE
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
+int lanes_for(int nrows, int nnz) {
+    const double avg = nrows > 0 ? (double)nnz / nrows : 0.0;
+    int G = 1;
+    while (G < 64 && G < avg) G <<= 1;
+    return G;
+}
+}  // namespace
+
+extern "C" {
+
+int qa_csr_matvec_z(int nrows, int ncols, const int *rowptr, const int *col, const void *vals,
+                    int vals_fp32, int nnz, const void *x, void *out, double scale_re,
+                    double scale_im, int accumulate, void *stream) {
+    QA_REQUIRE(nrows >= 0 && ncols >= 0, "qa_csr_matvec_z: negative shape");
+    QA_REQUIRE(nrows == 0 || (rowptr && out), "qa_csr_matvec_z: null pointer");
+    if (nrows == 0) return QA_OK;
+    const int G = lanes_for(nrows, nnz);
+    const double2 sc = make_double2(scale_re, scale_im);
+    hipStream_t s = (hipStream_t)stream;
+    if (vals_fp32)
+        return launch_csr_matvec(nrows, rowptr, col, (const float2 *)vals, (const double2 *)x,
+                                 (double2 *)out, sc, accumulate, G, s);
+    return launch_csr_matvec(nrows, rowptr, col, (const double2 *)vals, (const double2 *)x,
+                             (double2 *)out, sc, accumulate, G, s);
+}
+
+int qa_csr_matmat_z(int nrows, int ncols, const int *rowptr, const int *col, const void *vals,
+                    int vals_fp32, int nvec, const void *x, int64_t ldx, void *out, int64_t ldo,
+                    double scale_re, double scale_im, int accumulate, void *stream) {
+    QA_REQUIRE(nrows >= 0 && ncols >= 0 && nvec >= 0, "qa_csr_matmat_z: negative shape");
+    QA_REQUIRE(ldx >= ncols && ldo >= nrows, "qa_csr_matmat_z: leading dimension too small");
+    QA_REQUIRE(nvec <= 65535, "qa_csr_matmat_z: at most 65535 columns per call");
+    if (nrows == 0 || nvec == 0) return QA_OK;
+    dim3 grid((unsigned)((nrows + 255) / 256), (unsigned)nvec);
+    const double2 sc = make_double2(scale_re, scale_im);
+    hipStream_t s = (hipStream_t)stream;
+    if (vals_fp32)
+        hipLaunchKernelGGL(csr_matmat_kernel<float2>, grid, dim3(256), 0, s, nrows, rowptr, col,
+                           (const float2 *)vals, nvec, (const double2 *)x, ldx, (double2 *)out,
+                           ldo, sc, accumulate);
+    else
+        hipLaunchKernelGGL(csr_matmat_kernel<double2>, grid, dim3(256), 0, s, nrows, rowptr, col,
+                           (const double2 *)vals, nvec, (const double2 *)x, ldx, (double2 *)out,
+                           ldo, sc, accumulate);
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
+int qa_sell_matvec_z(int nrows, int nslices, const int *slice_ptr, const int *col,
+                     const void *vals, int vals_fp32, const void *x, void *out, double scale_re,
+                     double scale_im, int accumulate, void *stream) {
+    QA_REQUIRE(nrows >= 0 && nslices == (nrows + QA_WAVE - 1) / QA_WAVE,
+               "qa_sell_matvec_z: nslices must be ceil(nrows / 64)");
+    if (nrows == 0) return QA_OK;
+    const unsigned blocks = (unsigned)(((int64_t)nslices * QA_WAVE + 255) / 256);
+    const double2 sc = make_double2(scale_re, scale_im);
+    hipStream_t s = (hipStream_t)stream;
+    if (vals_fp32)
+        hipLaunchKernelGGL(sell_matvec_kernel<float2>, dim3(blocks), dim3(256), 0, s, nrows,
+                           nslices, slice_ptr, col, (const float2 *)vals, (const double2 *)x,
+                           (double2 *)out, sc, accumulate);
+    else
+        hipLaunchKernelGGL(sell_matvec_kernel<double2>, dim3(blocks), dim3(256), 0, s, nrows,
+                           nslices, slice_ptr, col, (const double2 *)vals, (const double2 *)x,
+                           (double2 *)out, sc, accumulate);
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
+int qa_lincomb_z(int64_t n, const void *base, int nterms, const void *const *vecs,
+                 const double *coefs, void *out, void *stream) {
+    QA_REQUIRE(n >= 0 && nterms >= 0 && nterms <= 16, "qa_lincomb_z: 0 <= nterms <= 16");
+    if (n == 0) return QA_OK;
+    LinComb lc;
+    for (int j = 0; j < 16; ++j) {
+        lc.v[j] = j < nterms ? (const double2 *)vecs[j] : nullptr;
+        lc.c[j] = j < nterms ? coefs[j] : 0.0;
+    }
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(lincomb_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, n,
+                       (const double2 *)base, nterms, lc, (double2 *)out);
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
+int qa_axpby_z(int64_t n, double are, double aim, const void *x, double bre, double bim,
+               const void *y, void *out, void *stream) {
+    QA_REQUIRE(n >= 0 && (n == 0 || (x && out)), "qa_axpby_z: null pointer");
+    if (n == 0) return QA_OK;
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(axpby_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, n,
+                       make_double2(are, aim), (const double2 *)x, make_double2(bre, bim),
+                       (const double2 *)y, (double2 *)out);
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
+static int run_reduce(int kind, int64_t n, const void *x, const void *y, double atol, double rtol,
+                      int64_t stride, double *result, int ncomp, void *stream) {
+    std::lock_guard<std::mutex> lock(g_red_mutex);
+    RedScratch S;
+    int rc = red_scratch(&S);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(reduce_kernel, dim3(red_blocks(n)), dim3(256), 0, s, kind, n,
+                       (const double2 *)x, (const double2 *)y, atol, rtol, stride, S);
+    return finish_reduction(S, result, ncomp, s);
+}
+
+int qa_wrmn_error_z(int64_t n, const void *diff, const void *state, double atol, double rtol,
+                    double *result, void *stream) {
+    QA_REQUIRE(n > 0 && diff && state && result, "qa_wrmn_error_z: bad arguments");
+    return run_reduce(0, n, diff, state, atol, rtol, 1, result, 1, stream);
+}
+
+int qa_norm2_z(int64_t n, const void *x, double *result, void *stream) {
+    QA_REQUIRE(n >= 0 && result, "qa_norm2_z: bad arguments");
+    if (n == 0) { *result = 0.0; return QA_OK; }
+    return run_reduce(1, n, x, nullptr, 0, 0, 1, result, 1, stream);
+}
+
+int qa_inner_z(int64_t n, const void *x, const void *y, double *result2, void *stream) {
+    QA_REQUIRE(n >= 0 && result2, "qa_inner_z: bad arguments");
+    if (n == 0) { result2[0] = result2[1] = 0.0; return QA_OK; }
+    return run_reduce(2, n, x, y, 0, 0, 1, result2, 2, stream);
+}
+
+int qa_trace_z(int d, const void *rho, double *result2, void *stream) {
+    QA_REQUIRE(d >= 0 && result2, "qa_trace_z: bad arguments");
+    if (d == 0) { result2[0] = result2[1] = 0.0; return QA_OK; }
+    return run_reduce(3, d, rho, nullptr, 0, 0, (int64_t)d + 1, result2, 2, stream);
+}
+
+static int run_expect(int kind, int nrows, const int *rowptr, const int *col, const void *vals,
+                      int vals_fp32, const void *st, double *result2, void *stream) {
+    std::lock_guard<std::mutex> lock(g_red_mutex);
+    RedScratch S;
+    int rc = red_scratch(&S);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (vals_fp32)
+        hipLaunchKernelGGL(expect_csr_kernel<float2>, dim3(red_blocks(nrows)), dim3(256), 0, s,
+                           kind, nrows, rowptr, col, (const float2 *)vals, (const double2 *)st, S);
+    else
+        hipLaunchKernelGGL(expect_csr_kernel<double2>, dim3(red_blocks(nrows)), dim3(256), 0, s,
+                           kind, nrows, rowptr, col, (const double2 *)vals, (const double2 *)st, S);
+    return finish_reduction(S, result2, 2, s);
+}
+
+int qa_expect_csr_ket_z(int n, const int *rowptr, const int *col, const void *vals,
+                        int vals_fp32, const void *psi, double *result2, void *stream) {
+    QA_REQUIRE(n > 0 && rowptr && psi && result2, "qa_expect_csr_ket_z: bad arguments");
+    return run_expect(0, n, rowptr, col, vals, vals_fp32, psi, result2, stream);
+}
+
+int qa_expect_csr_dm_z(int d, const int *rowptr, const int *col, const void *vals, int vals_fp32,
+                       const void *rho, double *result2, void *stream) {
+    QA_REQUIRE(d > 0 && rowptr && rho && result2, "qa_expect_csr_dm_z: bad arguments");
+    return run_expect(1, d, rowptr, col, vals, vals_fp32, rho, result2, stream);
+}
+
+}  // extern "C"

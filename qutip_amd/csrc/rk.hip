@@ -1,0 +1,1072 @@
+// Fused explicit Runge-Kutta engine for dy/dt = A y with a constant sparse
+// generator A (the superoperator of mesolve, -iH of sesolve).
+//
+// Restates the algorithm of qutip/solver/integrator/explicit_rk.pyx (step
+// control :333-356 and :439-467, stages :358-389, error :391-397, dense output
+// :399-430) with the work laid out for gfx950:
+//
+// * one kernel per RK stage: the SELL-64 spmv k_i = A x_i with an epilogue
+//   that forms the next stage input x_{i+1} = y_prev + dt sum_j a_{i+1,j} k_j
+//   row by row (the reference does a copy plus one zaxpy per coefficient per
+//   stage, explicit_rk.pyx:432-437);
+// * the last stage's epilogue forms y_front, the embedded error vector and
+//   the weighted RMS norm partials (ode.pyx:27-52); the last workgroup to
+//   finish reduces them in a fixed order and runs the step controller on the
+//   device, so the host never waits on a step;
+// * dense-output stages, the output-time interpolation and the expectation
+//   values ("functionals") are further kernels gated by the device phase;
+// * one loop iteration (all of the above) is captured once into a hipGraph
+//   and replayed; kernels whose phase does not match return immediately.
+//
+// Device state lives in an RKCtl block in HBM.  Vectors are complex128,
+// padded to a multiple of 64 rows.  Operator values are complex128 or
+// complex64 (widened to FP64 in registers; accumulation is always FP64).
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "qa_api_internal.h"
+#include "qa_common.h"
+
+using namespace qa;
+
+namespace {
+
+enum Phase { PH_IDLE = 0, PH_STEP = 1, PH_DENSE = 2, PH_OUTPUT = 3, PH_DONE = 4, PH_ERROR = 5 };
+// integrator status codes: qutip/solver/integrator/explicit_rk.pxd:5-12
+enum Status {
+    ST_AT_FRONT = 2,
+    ST_INTERPOLATED = 1,
+    ST_NORMAL = 0,
+    ST_TOO_MUCH_WORK = -1,
+    ST_DT_UNDERFLOW = -2,
+    ST_OUTSIDE_RANGE = -3,
+    ST_NOT_INITIATED = -4
+};
+
+constexpr int kMaxStages = 16;
+constexpr int kMaxPending = 4;     // output times handled per output launch
+constexpr int kOutChunk = 1024;    // functional entries per output workgroup
+constexpr int kStepBlocks = 256;   // threads per stage workgroup
+
+struct RKCtl {
+    double t_prev, t_front, dt_safe, dt_int, dt_try, err, t_last;
+    double rtol, atol, min_step, max_step;
+    double *times;  // mutable copy of the requested output times
+    double2 *fun_out;
+    double2 *states_out;
+    int ntimes, phase, cur, out_idx, nsteps_left, nsteps_call, dense_ready, status;
+    int order, max_numsteps, interpolate, adaptive, single_step, nfun;
+    long long n_attempts, n_accepted, n_dense, n_outputs;
+    unsigned int counter;
+};
+
+// ---- step control (executed by one thread) ---------------------------------
+__device__ double get_timestep(const RKCtl *c, double t) {
+    // explicit_rk.pyx:439-449
+    const double dt_needed = t - c->t_prev;
+    if (!c->adaptive) return dt_needed;
+    if (c->interpolate) return c->dt_safe;
+    if (dt_needed <= c->dt_safe) return dt_needed;
+    return dt_needed / (double)((long long)(dt_needed / c->dt_safe) + 1);
+}
+
+__device__ void recompute_safe_step(RKCtl *c, double err, double dt) {
+    // explicit_rk.pyx:451-467
+    if (!c->adaptive) return;
+    double factor;
+    if (err == 0.0) {
+        factor = 10.0;
+    } else {
+        factor = 0.9 * pow(err, -1.0 / (c->order + 1));
+        factor = fmin(10.0, factor);
+        factor = fmax(0.2, factor);
+    }
+    c->dt_safe = dt * factor;
+    if (c->max_step != 0.0) c->dt_safe = fmin(c->max_step, c->dt_safe);
+    if (c->min_step != 0.0) c->dt_safe = fmax(c->min_step, c->dt_safe);
+}
+
+// y_prev <- y_front, t_prev <- t_front, then a fresh _step_in_err
+// (explicit_rk.pyx:312-316).
+__device__ void start_step(RKCtl *c, double target) {
+    c->cur ^= 1;
+    c->t_prev = c->t_front;
+    c->dense_ready = 0;
+    c->nsteps_call = 0;
+    c->dt_try = get_timestep(c, target);
+    c->phase = PH_STEP;
+}
+
+// Decide what the next output time needs (explicit_rk.pyx:288-331).
+__device__ void route(RKCtl *c) {
+    if (c->out_idx >= c->ntimes) {
+        c->phase = PH_DONE;
+        return;
+    }
+    double target = c->times[c->out_idx];
+    if (target < c->t_prev) {
+        c->status = ST_OUTSIDE_RANGE;
+        c->phase = PH_ERROR;
+        return;
+    }
+    if (target < c->t_front) {
+        if (!c->interpolate) {
+            c->status = ST_OUTSIDE_RANGE;
+            c->phase = PH_ERROR;
+            return;
+        }
+        c->phase = c->dense_ready ? PH_OUTPUT : PH_DENSE;
+        return;
+    }
+    if (c->single_step && c->t_last < c->t_front && target > c->t_front) {
+        // integrate(t, step=True) clamps to the front (explicit_rk.pyx:309-311)
+        c->times[c->out_idx] = c->t_front;
+        target = c->t_front;
+    }
+    if (target == c->t_front) {
+        c->phase = PH_OUTPUT;
+        return;
+    }
+    c->nsteps_left = c->max_numsteps;
+    c->status = ST_NORMAL;
+    start_step(c, target);
+}
+
+// After one attempt of _compute_step (explicit_rk.pyx:333-356).
+__device__ void after_attempt(RKCtl *c, double err) {
+    const double dt = c->dt_try;
+    c->err = err;
+    c->n_attempts++;
+    c->dt_int = dt;
+    recompute_safe_step(c, err, dt);
+    if (dt == c->min_step && err > 1.0) {
+        c->status = ST_DT_UNDERFLOW;
+        c->phase = PH_ERROR;
+        return;
+    }
+    c->nsteps_call++;
+    if (c->nsteps_call > c->nsteps_left) {
+        c->status = ST_TOO_MUCH_WORK;
+        c->phase = PH_ERROR;
+        return;
+    }
+    double target = c->times[c->out_idx];
+    if (err >= 1.0) {
+        c->dt_try = get_timestep(c, target);
+        c->phase = PH_STEP;
+        return;
+    }
+    c->t_front = c->t_prev + dt;
+    c->nsteps_left -= c->nsteps_call;
+    c->nsteps_call = 0;
+    c->n_accepted++;
+    c->dense_ready = 0;
+    if (c->single_step && c->t_front < target) {
+        // integrate(t, step=True) stops after one step at the front
+        c->times[c->out_idx] = c->t_front;
+        target = c->t_front;
+    }
+    if (c->t_front < target) {
+        start_step(c, target);
+    } else if (c->t_front > target) {
+        if (c->interpolate) {
+            c->phase = PH_DENSE;
+        } else {
+            c->status = ST_OUTSIDE_RANGE;
+            c->phase = PH_ERROR;
+        }
+    } else {
+        c->phase = PH_OUTPUT;
+    }
+}
+
+// ---- kernels -----------------------------------------------------------------
+enum Input { IN_YPREV = 0, IN_YFRONT = 1, IN_X = 2 };
+enum Epi { EPI_NONE = 0, EPI_NEXT = 1, EPI_FINAL = 2 };
+
+template <typename V>
+struct StageArgs {
+    const int *slice_ptr;
+    const int *col;
+    const V *vals;
+    int nrows, nslices;
+    int64_t npad;
+    double2 *y0, *y1, *x0, *x1, *K;
+    double *partials;
+    RKCtl *ctl;
+    int stage, run_phase, input, epi, nterms;
+    double coef[kMaxStages];
+    double ecoef[kMaxStages];
+};
+
+template <typename V>
+__global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
+    RKCtl *ctl = P.ctl;
+    if (ctl->phase != P.run_phase) return;  // uniform over the whole grid
+    const int cur = ctl->cur;
+    double2 *yprev = cur ? P.y1 : P.y0;
+    double2 *yfront = cur ? P.y0 : P.y1;
+    const double dt = (P.run_phase == PH_STEP) ? ctl->dt_try : ctl->dt_int;
+    const double2 *xin = P.input == IN_YPREV    ? yprev
+                         : P.input == IN_YFRONT ? yfront
+                         : ((P.stage & 1) ? P.x1 : P.x0);
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = (int)(r / QA_WAVE), lane = (int)(r % QA_WAVE);
+    double part = 0.0;
+    if (slice < P.nslices) {
+        const double2 acc = sell_row_dot(P.slice_ptr, P.col, P.vals, xin, slice, lane);
+        P.K[(int64_t)P.stage * P.npad + r] = acc;
+        if (P.epi == EPI_NEXT) {
+            double2 o = yprev[r];
+            for (int j = 0; j < P.nterms; ++j) {
+                const double f = dt * P.coef[j];
+                if (f != 0.0) raxpy(o, f, j == P.stage ? acc : P.K[(int64_t)j * P.npad + r]);
+            }
+            double2 *xout = ((P.stage + 1) & 1) ? P.x1 : P.x0;
+            xout[r] = o;
+        } else if (P.epi == EPI_FINAL) {
+            double2 yf = yprev[r];
+            double2 er = make_double2(0.0, 0.0);
+            for (int j = 0; j < P.nterms; ++j) {
+                const double fb = dt * P.coef[j], fe = dt * P.ecoef[j];
+                if (fb != 0.0 || fe != 0.0) {
+                    const double2 kj = j == P.stage ? acc : P.K[(int64_t)j * P.npad + r];
+                    if (fb != 0.0) raxpy(yf, fb, kj);
+                    if (fe != 0.0) raxpy(er, fe, kj);
+                }
+            }
+            yfront[r] = yf;
+            if (r < P.nrows) {
+                const double q = sqrt(cabs2(er)) / (ctl->atol + ctl->rtol * sqrt(cabs2(yf)));
+                part = q * q;
+            }
+        }
+    }
+    if (P.epi == EPI_FINAL) {
+        __shared__ double smem[kStepBlocks / QA_WAVE];
+        const double s = block_sum(part, smem);
+        if (threadIdx.x == 0) P.partials[blockIdx.x] = s;
+        if (last_block_done(&ctl->counter) && threadIdx.x == 0) {
+            double tot = 0.0;
+            for (unsigned b = 0; b < gridDim.x; ++b) tot += P.partials[b];
+            const double err = ctl->adaptive ? sqrt(tot / (double)P.nrows) : 0.0;
+            after_attempt(ctl, err);
+        }
+    }
+}
+
+struct LinCoef {
+    double c[kMaxStages];
+};
+
+// x = y_prev + dt_int * sum_j coef_j k_j, for a first dense stage whose input
+// is not y_front (explicit_rk.pyx:404-407).
+__global__ __launch_bounds__(256) void rk_combine_kernel(RKCtl *ctl, int64_t npad, double2 *y0,
+                                                         double2 *y1, const double2 *K,
+                                                         double2 *xout, int nterms, LinCoef lc) {
+    if (ctl->phase != PH_DENSE) return;
+    const double2 *yprev = ctl->cur ? y1 : y0;
+    const double dt = ctl->dt_int;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < npad;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        double2 o = yprev[r];
+        for (int j = 0; j < nterms; ++j) {
+            const double f = dt * lc.c[j];
+            if (f != 0.0) raxpy(o, f, K[(int64_t)j * npad + r]);
+        }
+        xout[r] = o;
+    }
+}
+
+struct OutArgs {
+    RKCtl *ctl;
+    double2 *y0, *y1;
+    const double2 *K;
+    int64_t npad;
+    int nk, dorder;
+    const double *bi;  // [nk][dorder]
+    int nfun;
+    const int *fptr, *fkind, *frow, *fcol;
+    const double2 *fw;
+    double2 *partials;  // [kMaxPending][nfun][nchunks]
+    int nchunks;
+    double2 *y_last;
+};
+
+// Interpolation weights of the pending outputs: w[q][i] = dt_int * b_factor_i
+// (explicit_rk.pyx:412-430); front[q] = 1 when the output time equals t_front.
+__device__ int pending_outputs(const RKCtl *c, const OutArgs &A, double (*w)[kMaxStages],
+                               int *front) {
+    int nq = 0;
+    while (nq < kMaxPending && c->out_idx + nq < c->ntimes &&
+           c->times[c->out_idx + nq] <= c->t_front)
+        ++nq;
+    for (int q = 0; q < nq; ++q) {
+        const double t = c->times[c->out_idx + q];
+        front[q] = (t == c->t_front);
+        for (int i = 0; i < kMaxStages; ++i) w[q][i] = 0.0;
+        if (front[q] || A.bi == nullptr) continue;
+        const double tau = (t - c->t_prev) / c->dt_int;
+        for (int i = 0; i < A.nk; ++i) {
+            double v = 0.0;
+            for (int j = A.dorder - 1; j >= 0; --j) {
+                v += A.bi[i * A.dorder + j];
+                v *= tau;
+            }
+            w[q][i] = c->dt_int * v;
+        }
+    }
+    return nq;
+}
+
+__device__ __forceinline__ double2 state_at(const OutArgs &A, const double2 *yprev,
+                                            const double2 *yfront, const double *w, int front,
+                                            int64_t idx) {
+    if (front) return yfront[idx];
+    double2 o = yprev[idx];
+    for (int i = 0; i < A.nk; ++i)
+        if (w[i] != 0.0) raxpy(o, w[i], A.K[(int64_t)i * A.npad + idx]);
+    return o;
+}
+
+// Writes the states of the pending outputs (for store_states) and the state
+// at the last requested time (for get_state).
+__global__ __launch_bounds__(256) void rk_store_kernel(OutArgs A) {
+    RKCtl *c = A.ctl;
+    const int ph = c->phase;
+    if (ph != PH_DENSE && ph != PH_OUTPUT) return;
+    __shared__ double w[kMaxPending][kMaxStages];
+    __shared__ int front[kMaxPending];
+    __shared__ int nq_s;
+    if (threadIdx.x == 0) nq_s = pending_outputs(c, A, w, front);
+    __syncthreads();
+    const int nq = nq_s;
+    const bool has_last = c->out_idx + nq == c->ntimes;
+    if (!c->states_out && !has_last) return;
+    const double2 *yprev = c->cur ? A.y1 : A.y0;
+    const double2 *yfront = c->cur ? A.y0 : A.y1;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < A.npad;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        for (int q = 0; q < nq; ++q) {
+            const bool is_last = (c->out_idx + q == c->ntimes - 1);
+            if (!c->states_out && !is_last) continue;
+            const double2 v = state_at(A, yprev, yfront, w[q], front[q], r);
+            if (c->states_out) c->states_out[(int64_t)(c->out_idx + q) * A.npad + r] = v;
+            if (is_last) A.y_last[r] = v;
+        }
+    }
+}
+
+// Functionals of the pending output states: linear f = sum w y[col] (a
+// vectorised density matrix: tr(O rho), tr(rho)) or quadratic
+// f = sum w conj(y[row]) y[col] (a ket: <psi|O|psi>, ||psi||^2).  Grid
+// (nchunks, nfun); the last workgroup reduces in a fixed order, stores the
+// values and advances the output index.
+__global__ __launch_bounds__(256) void rk_output_kernel(OutArgs A) {
+    RKCtl *c = A.ctl;
+    const int ph = c->phase;
+    if (ph != PH_DENSE && ph != PH_OUTPUT) return;
+    __shared__ double w[kMaxPending][kMaxStages];
+    __shared__ int front[kMaxPending];
+    __shared__ int nq_s;
+    __shared__ double smem[256 / QA_WAVE];
+    if (threadIdx.x == 0) nq_s = pending_outputs(c, A, w, front);
+    __syncthreads();
+    const int nq = nq_s;
+    const double2 *yprev = c->cur ? A.y1 : A.y0;
+    const double2 *yfront = c->cur ? A.y0 : A.y1;
+    const int f = blockIdx.y;
+    const int beg = A.fptr[f] + blockIdx.x * kOutChunk;
+    const int end = min(A.fptr[f + 1], beg + kOutChunk);
+    const int quad = A.fkind[f];
+    for (int q = 0; q < nq; ++q) {
+        double2 acc = make_double2(0.0, 0.0);
+        for (int e = beg + threadIdx.x; e < end; e += blockDim.x) {
+            double2 v = cmul(A.fw[e], state_at(A, yprev, yfront, w[q], front[q], A.fcol[e]));
+            if (quad) {
+                const double2 l = state_at(A, yprev, yfront, w[q], front[q], A.frow[e]);
+                v = cmul(make_double2(l.x, -l.y), v);
+            }
+            acc = cadd(acc, v);
+        }
+        const double2 s = block_sum2(acc, smem);
+        if (threadIdx.x == 0)
+            A.partials[((int64_t)q * A.nfun + f) * A.nchunks + blockIdx.x] = s;
+    }
+    // last-block handshake over the 2-D grid
+    __shared__ int is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned total = gridDim.x * gridDim.y;
+        is_last = (atomicAdd(&c->counter, 1u) == total - 1);
+    }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    if (threadIdx.x != 0) return;
+    atomicExch(&c->counter, 0u);
+    if (nq == 0) {
+        // a non-finite step left no output time inside [t_prev, t_front]
+        c->status = ST_OUTSIDE_RANGE;
+        c->phase = PH_ERROR;
+        return;
+    }
+    for (int q = 0; q < nq; ++q) {
+        for (int ff = 0; ff < A.nfun; ++ff) {
+            double2 t = make_double2(0.0, 0.0);
+            for (int k = 0; k < A.nchunks; ++k)
+                t = cadd(t, A.partials[((int64_t)q * A.nfun + ff) * A.nchunks + k]);
+            c->fun_out[(int64_t)(c->out_idx + q) * A.nfun + ff] = t;
+        }
+    }
+    if (ph == PH_DENSE) {
+        c->dense_ready = 1;
+        c->n_dense++;
+    }
+    c->t_last = c->times[c->out_idx + nq - 1];
+    c->status = front[nq - 1] ? ST_AT_FRONT : ST_INTERPOLATED;
+    c->n_outputs += nq;
+    c->out_idx += nq;
+    route(c);
+}
+
+// Start of a run over new output times.
+__global__ void rk_begin_kernel(RKCtl *c, int ntimes, double2 *states_out) {
+    c->ntimes = ntimes;
+    c->states_out = states_out;
+    c->out_idx = 0;
+    route(c);
+}
+
+// Initial state (explicit_rk.pyx:204-230): t_prev = t_front = t_last = t0,
+// y_front = y_last = y0, step size from the host estimate.
+__global__ void rk_init_kernel(RKCtl *c, double t0, double dt_safe) {
+    c->t_prev = c->t_front = c->t_last = t0;
+    c->dt_int = 0.0;
+    c->dt_safe = dt_safe;
+    c->dt_try = dt_safe;
+    c->cur = 0;  // y_prev = y0 buffer, y_front = y1 buffer
+    c->dense_ready = 0;
+    c->status = ST_NORMAL;
+    c->phase = PH_IDLE;
+}
+
+// Profiling helper: force the phase a stage kernel waits for.
+__global__ void rk_set_phase_kernel(RKCtl *c, int phase) { c->phase = phase; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host-side engine object
+// ---------------------------------------------------------------------------
+struct qa_rk_solver {
+    int n = 0, nslices = 0, vals_fp32 = 0;
+    int64_t npad = 0;
+    const int *slice_ptr = nullptr;
+    const int *col = nullptr;
+    const void *vals = nullptr;
+    // tableau
+    int order = 0, rk_step = 0, rk_extra = 0, dorder = 0, interpolate = 1, adaptive = 1;
+    int first_dense_from_front = 0;
+    std::vector<double> a, b, c, e, bi;
+    double rtol = 1e-6, atol = 1e-8, first_step = 0, min_step = 0, max_step = 0;
+    int nsteps = 1000;
+    hipStream_t stream = nullptr;
+    // device buffers
+    void *block = nullptr;  // one allocation for vectors
+    double2 *y0 = nullptr, *y1 = nullptr, *x0 = nullptr, *x1 = nullptr, *K = nullptr;
+    double2 *y_last = nullptr, *tmp = nullptr;
+    double *partials = nullptr;
+    RKCtl *ctl = nullptr;
+    double *bi_dev = nullptr;
+    double *times_dev = nullptr;
+    int times_cap = 0;
+    double2 *fun_out_dev = nullptr;
+    int64_t fun_out_cap = 0;
+    // functionals
+    int nfun = 0, nchunks = 1;
+    int *fptr_dev = nullptr, *fkind_dev = nullptr, *frow_dev = nullptr, *fcol_dev = nullptr;
+    double2 *fw_dev = nullptr, *fpart_dev = nullptr;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graph_dirty = true;
+    RKCtl *ctl_host = nullptr;  // pinned mirror
+    long long graph_launches = 0;
+    int state_set = 0;
+};
+
+namespace {
+
+template <typename V>
+StageArgs<V> stage_args(qa_rk_solver *S, int i) {
+    StageArgs<V> P;
+    P.slice_ptr = S->slice_ptr;
+    P.col = S->col;
+    P.vals = (const V *)S->vals;
+    P.nrows = S->n;
+    P.nslices = S->nslices;
+    P.npad = S->npad;
+    P.y0 = S->y0;
+    P.y1 = S->y1;
+    P.x0 = S->x0;
+    P.x1 = S->x1;
+    P.K = S->K;
+    P.partials = S->partials;
+    P.ctl = S->ctl;
+    P.stage = i;
+    for (int j = 0; j < kMaxStages; ++j) P.coef[j] = P.ecoef[j] = 0.0;
+    const int ne = S->rk_extra;
+    if (i < S->rk_step) {
+        P.run_phase = PH_STEP;
+        P.input = i == 0 ? IN_YPREV : IN_X;
+        if (i < S->rk_step - 1) {
+            P.epi = EPI_NEXT;
+            P.nterms = i + 1;
+            for (int j = 0; j <= i; ++j) P.coef[j] = S->a[(size_t)(i + 1) * ne + j];
+        } else {
+            P.epi = EPI_FINAL;
+            P.nterms = S->rk_step;
+            for (int j = 0; j < S->rk_step; ++j) {
+                P.coef[j] = S->b[j];
+                P.ecoef[j] = S->adaptive ? S->e[j] : 0.0;
+            }
+        }
+    } else {
+        P.run_phase = PH_DENSE;
+        P.input = (i == S->rk_step && S->first_dense_from_front) ? IN_YFRONT : IN_X;
+        if (i < S->rk_extra - 1) {
+            P.epi = EPI_NEXT;
+            P.nterms = i + 1;
+            for (int j = 0; j <= i; ++j) P.coef[j] = S->a[(size_t)(i + 1) * ne + j];
+        } else {
+            P.epi = EPI_NONE;
+            P.nterms = 0;
+        }
+    }
+    return P;
+}
+
+template <typename V>
+int launch_stage(qa_rk_solver *S, int i) {
+    StageArgs<V> P = stage_args<V>(S, i);
+    const unsigned blocks = (unsigned)(S->npad / kStepBlocks + (S->npad % kStepBlocks ? 1 : 0));
+    hipLaunchKernelGGL(rk_stage_kernel<V>, dim3(blocks), dim3(kStepBlocks), 0, S->stream, P);
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
+OutArgs out_args(qa_rk_solver *S) {
+    OutArgs A;
+    A.ctl = S->ctl;
+    A.y0 = S->y0;
+    A.y1 = S->y1;
+    A.K = S->K;
+    A.npad = S->npad;
+    A.nk = S->interpolate ? S->rk_extra : S->rk_step;
+    A.dorder = S->dorder;
+    A.bi = S->bi_dev;
+    A.nfun = S->nfun;
+    A.fptr = S->fptr_dev;
+    A.fkind = S->fkind_dev;
+    A.frow = S->frow_dev;
+    A.fcol = S->fcol_dev;
+    A.fw = S->fw_dev;
+    A.partials = S->fpart_dev;
+    A.nchunks = S->nchunks;
+    A.y_last = S->y_last;
+    return A;
+}
+
+// Enqueue one loop iteration (used both for graph capture and direct launch).
+int enqueue_iteration(qa_rk_solver *S) {
+    int rc;
+    for (int i = 0; i < S->rk_step; ++i) {
+        rc = S->vals_fp32 ? launch_stage<float2>(S, i) : launch_stage<double2>(S, i);
+        if (rc) return rc;
+    }
+    if (S->interpolate && S->rk_extra > S->rk_step) {
+        if (!S->first_dense_from_front) {
+            LinCoef lc;
+            const int i = S->rk_step;
+            for (int j = 0; j < kMaxStages; ++j)
+                lc.c[j] = j < i ? S->a[(size_t)i * S->rk_extra + j] : 0.0;
+            double2 *xout = (i & 1) ? S->x1 : S->x0;
+            unsigned blocks = (unsigned)std::min<int64_t>((S->npad + 255) / 256, 4096);
+            hipLaunchKernelGGL(rk_combine_kernel, dim3(blocks), dim3(256), 0, S->stream, S->ctl,
+                               S->npad, S->y0, S->y1, S->K, xout, i, lc);
+            QA_CHECK_LAUNCH();
+        }
+        for (int i = S->rk_step; i < S->rk_extra; ++i) {
+            rc = S->vals_fp32 ? launch_stage<float2>(S, i) : launch_stage<double2>(S, i);
+            if (rc) return rc;
+        }
+    }
+    OutArgs A = out_args(S);
+    unsigned sblocks = (unsigned)std::min<int64_t>((S->npad + 255) / 256, 2048);
+    hipLaunchKernelGGL(rk_store_kernel, dim3(sblocks), dim3(256), 0, S->stream, A);
+    QA_CHECK_LAUNCH();
+    hipLaunchKernelGGL(rk_output_kernel, dim3(S->nchunks, std::max(S->nfun, 1)), dim3(256), 0,
+                       S->stream, A);
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
+int rebuild_graph(qa_rk_solver *S) {
+    if (S->exec) {
+        QA_HIP(hipGraphExecDestroy(S->exec));
+        S->exec = nullptr;
+    }
+    if (S->graph) {
+        QA_HIP(hipGraphDestroy(S->graph));
+        S->graph = nullptr;
+    }
+    QA_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_iteration(S);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(S->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    QA_HIP(e2);
+    S->graph = g;
+    QA_HIP(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
+    S->graph_dirty = false;
+    return QA_OK;
+}
+
+void free_all(qa_rk_solver *S) {
+    if (S->exec) (void)hipGraphExecDestroy(S->exec);
+    if (S->graph) (void)hipGraphDestroy(S->graph);
+    void *ptrs[] = {S->block, S->ctl, S->bi_dev, S->times_dev, S->fun_out_dev, S->fptr_dev,
+                    S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (S->ctl_host) (void)hipHostFree(S->ctl_host);
+    if (S->stream) (void)hipStreamDestroy(S->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int qa_rk_create(qa_rk_solver **out, int n, int nslices, const int *slice_ptr, const int *col,
+                 const void *vals, int vals_fp32, int order, int rk_step, int rk_extra,
+                 int dorder, const double *a, const double *b, const double *c,
+                 const double *e, const double *bi, double rtol, double atol, int nsteps,
+                 double first_step, double min_step, double max_step, int interpolate) {
+    QA_REQUIRE(out != nullptr, "qa_rk_create: null output");
+    *out = nullptr;
+    QA_REQUIRE(n > 0 && nslices == (n + QA_WAVE - 1) / QA_WAVE,
+               "qa_rk_create: need n > 0 and nslices = ceil(n / 64)");
+    QA_REQUIRE(slice_ptr && col && vals, "qa_rk_create: null operator pointer");
+    QA_REQUIRE(rk_step >= 1 && rk_step <= kMaxStages && rk_extra >= rk_step &&
+                   rk_extra <= kMaxStages,
+               "qa_rk_create: 1 <= rk_step <= rk_extra <= %d", kMaxStages);
+    QA_REQUIRE(a && b && c, "qa_rk_create: tableau a, b, c required");
+    QA_REQUIRE(!bi || (dorder >= 1 && dorder <= 16), "qa_rk_create: bad dense-output order");
+    QA_REQUIRE(nsteps >= 0, "qa_rk_create: nsteps must be >= 0");
+    qa_rk_solver *S = new (std::nothrow) qa_rk_solver();
+    QA_REQUIRE(S != nullptr, "qa_rk_create: out of host memory");
+    S->n = n;
+    S->nslices = nslices;
+    S->npad = (int64_t)nslices * QA_WAVE;
+    S->slice_ptr = slice_ptr;
+    S->col = col;
+    S->vals = vals;
+    S->vals_fp32 = vals_fp32;
+    S->order = order;
+    S->rk_step = rk_step;
+    S->rk_extra = rk_extra;
+    S->dorder = bi ? dorder : 0;
+    S->a.assign(a, a + (size_t)rk_extra * rk_extra);
+    S->b.assign(b, b + rk_step);
+    S->c.assign(c, c + rk_extra);
+    S->adaptive = e != nullptr;
+    if (e) S->e.assign(e, e + rk_step);
+    S->interpolate = (bi != nullptr) && interpolate;
+    if (bi) S->bi.assign(bi, bi + (size_t)rk_extra * dorder);
+    S->rtol = rtol;
+    S->atol = atol;
+    S->nsteps = nsteps;
+    S->first_step = first_step;
+    S->min_step = min_step != 0.0 ? min_step : 1e-15;
+    S->max_step = max_step;
+    // First dense stage input equals y_front when its a-row is b (vern7).
+    S->first_dense_from_front = 0;
+    if (S->interpolate && rk_extra > rk_step) {
+        bool same = true;
+        for (int j = 0; j < rk_extra; ++j) {
+            const double aj = S->a[(size_t)rk_step * rk_extra + j];
+            const double bj = j < rk_step ? S->b[j] : 0.0;
+            if (aj != bj) same = false;
+        }
+        S->first_dense_from_front = same;
+    }
+    int rc = QA_OK;
+    auto fail = [&](int code) {
+        free_all(S);
+        delete S;
+        return code;
+    };
+    hipError_t he = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) return fail(qa::hip_fail(he, "hipStreamCreate", __FILE__, __LINE__));
+    const int nk = S->interpolate ? rk_extra : rk_step;
+    const int64_t nvec = 6 + nk;  // y0 y1 x0 x1 y_last tmp + K
+    const size_t vbytes = (size_t)S->npad * sizeof(double2);
+    const size_t nblocks_step = (size_t)(S->npad / kStepBlocks + 1);
+    size_t bytes = vbytes * nvec + nblocks_step * sizeof(double) + 256;
+    he = hipMalloc(&S->block, bytes);
+    if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMalloc(rk vectors)", __FILE__, __LINE__));
+    char *p = (char *)S->block;
+    S->y0 = (double2 *)p; p += vbytes;
+    S->y1 = (double2 *)p; p += vbytes;
+    S->x0 = (double2 *)p; p += vbytes;
+    S->x1 = (double2 *)p; p += vbytes;
+    S->y_last = (double2 *)p; p += vbytes;
+    S->tmp = (double2 *)p; p += vbytes;
+    S->K = (double2 *)p; p += vbytes * nk;
+    S->partials = (double *)p;
+    rc = qa_fill_zero(S->block, bytes - 256 - nblocks_step * sizeof(double), S->stream);
+    if (rc) return fail(rc);
+    he = hipMalloc((void **)&S->ctl, sizeof(RKCtl));
+    if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMalloc(ctl)", __FILE__, __LINE__));
+    he = hipHostMalloc((void **)&S->ctl_host, sizeof(RKCtl), hipHostMallocDefault);
+    if (he != hipSuccess) return fail(qa::hip_fail(he, "hipHostMalloc", __FILE__, __LINE__));
+    RKCtl init;
+    memset(&init, 0, sizeof(init));
+    init.rtol = rtol;
+    init.atol = atol;
+    init.min_step = S->min_step;
+    init.max_step = max_step;
+    init.order = order;
+    init.max_numsteps = nsteps;
+    init.interpolate = S->interpolate;
+    init.adaptive = S->adaptive;
+    init.status = ST_NOT_INITIATED;
+    init.phase = PH_IDLE;
+    he = hipMemcpyAsync(S->ctl, &init, sizeof(init), hipMemcpyHostToDevice, S->stream);
+    if (he != hipSuccess) return fail(qa::hip_fail(he, "ctl upload", __FILE__, __LINE__));
+    if (S->interpolate) {
+        he = hipMalloc((void **)&S->bi_dev, S->bi.size() * sizeof(double));
+        if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMalloc(bi)", __FILE__, __LINE__));
+        he = hipMemcpyAsync(S->bi_dev, S->bi.data(), S->bi.size() * sizeof(double),
+                            hipMemcpyHostToDevice, S->stream);
+        if (he != hipSuccess) return fail(qa::hip_fail(he, "bi upload", __FILE__, __LINE__));
+    }
+    he = hipStreamSynchronize(S->stream);
+    if (he != hipSuccess) return fail(qa::hip_fail(he, "sync", __FILE__, __LINE__));
+    // no functionals until set
+    int zero_ptr[2] = {0, 0};
+    rc = qa_rk_set_functionals(S, 0, nullptr, zero_ptr, nullptr, nullptr, nullptr);
+    if (rc) return fail(rc);
+    *out = S;
+    return QA_OK;
+}
+
+int qa_rk_destroy(qa_rk_solver *S) {
+    if (!S) return QA_OK;
+    (void)hipStreamSynchronize(S->stream);
+    free_all(S);
+    delete S;
+    return QA_OK;
+}
+
+void *qa_rk_stream(qa_rk_solver *S) { return S ? (void *)S->stream : nullptr; }
+
+int qa_rk_set_functionals(qa_rk_solver *S, int nfun, const int *kinds, const int *fptr,
+                          const int *rows, const int *cols, const double *w) {
+    QA_REQUIRE(S != nullptr, "qa_rk_set_functionals: null solver");
+    QA_REQUIRE(nfun >= 0 && fptr && fptr[0] == 0, "qa_rk_set_functionals: bad fptr");
+    const int nnz = fptr[nfun];
+    int maxlen = 0;
+    for (int f = 0; f < nfun; ++f) {
+        QA_REQUIRE(fptr[f + 1] >= fptr[f], "qa_rk_set_functionals: fptr must be non-decreasing");
+        maxlen = std::max(maxlen, fptr[f + 1] - fptr[f]);
+    }
+    for (int p = 0; p < nnz; ++p)
+        QA_REQUIRE(cols[p] >= 0 && cols[p] < S->n, "functional entry %d: column %d out of range",
+                   p, cols[p]);
+    for (int f = 0; f < nfun; ++f) {
+        if (kinds[f]) {
+            for (int p = fptr[f]; p < fptr[f + 1]; ++p)
+                QA_REQUIRE(rows[p] >= 0 && rows[p] < S->n,
+                           "functional entry %d: row %d out of range", p, rows[p]);
+        }
+    }
+    QA_HIP(hipStreamSynchronize(S->stream));
+    void *olds[] = {S->fptr_dev, S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev};
+    for (void *p : olds)
+        if (p) QA_HIP(hipFree(p));
+    S->fptr_dev = S->fkind_dev = S->frow_dev = S->fcol_dev = nullptr;
+    S->fw_dev = S->fpart_dev = nullptr;
+    S->nfun = nfun;
+    S->nchunks = std::max(1, (maxlen + kOutChunk - 1) / kOutChunk);
+    const int nf1 = std::max(nfun, 1);
+    const int nz1 = std::max(nnz, 1);
+    QA_HIP(hipMalloc((void **)&S->fptr_dev, sizeof(int) * (nf1 + 1)));
+    QA_HIP(hipMalloc((void **)&S->fkind_dev, sizeof(int) * nf1));
+    QA_HIP(hipMalloc((void **)&S->frow_dev, sizeof(int) * nz1));
+    QA_HIP(hipMalloc((void **)&S->fcol_dev, sizeof(int) * nz1));
+    QA_HIP(hipMalloc((void **)&S->fw_dev, sizeof(double2) * nz1));
+    QA_HIP(hipMalloc((void **)&S->fpart_dev,
+                     sizeof(double2) * (size_t)kMaxPending * nf1 * S->nchunks));
+    std::vector<int> zeros_i(nz1, 0);
+    std::vector<int> fp(nf1 + 1, 0), fk(nf1, 0);
+    for (int f = 0; f <= nfun; ++f) fp[f] = fptr[f];
+    for (int f = 0; f < nfun; ++f) fk[f] = kinds[f] ? 1 : 0;
+    QA_HIP(hipMemcpy(S->fptr_dev, fp.data(), sizeof(int) * (nf1 + 1), hipMemcpyHostToDevice));
+    QA_HIP(hipMemcpy(S->fkind_dev, fk.data(), sizeof(int) * nf1, hipMemcpyHostToDevice));
+    if (nnz > 0) {
+        QA_HIP(hipMemcpy(S->frow_dev, rows ? rows : zeros_i.data(), sizeof(int) * nnz,
+                         hipMemcpyHostToDevice));
+        QA_HIP(hipMemcpy(S->fcol_dev, cols, sizeof(int) * nnz, hipMemcpyHostToDevice));
+        QA_HIP(hipMemcpy(S->fw_dev, w, sizeof(double2) * nnz, hipMemcpyHostToDevice));
+    }
+    S->graph_dirty = true;
+    return QA_OK;
+}
+
+// set_initial_value + _estimate_first_step (explicit_rk.pyx:204-276).
+int qa_rk_set_state(qa_rk_solver *S, double t0, const void *y0_dev) {
+    QA_REQUIRE(S != nullptr && y0_dev != nullptr, "qa_rk_set_state: null argument");
+    hipStream_t s = S->stream;
+    const size_t nb = (size_t)S->n * sizeof(double2);
+    int rc = qa_fill_zero(S->y1, (size_t)S->npad * sizeof(double2), s);
+    if (rc) return rc;
+    QA_HIP(hipMemcpyAsync(S->y1, y0_dev, nb, hipMemcpyDeviceToDevice, s));
+    QA_HIP(hipMemcpyAsync(S->y_last, S->y1, (size_t)S->npad * sizeof(double2),
+                          hipMemcpyDeviceToDevice, s));
+    double dt = S->first_step;
+    if (dt == 0.0 && S->adaptive) {
+        double norm2 = 0.0, kn2 = 0.0;
+        rc = qa_norm2_z(S->n, S->y1, &norm2, s);
+        if (rc) return rc;
+        double norm = sqrt(norm2);
+        const double tol = S->atol + norm * S->rtol;
+        double2 *k0 = S->K, *k1 = S->K + S->npad;
+        rc = qa_sell_matvec_z(S->n, S->nslices, S->slice_ptr, S->col, S->vals, S->vals_fp32,
+                              S->y1, k0, 1.0, 0.0, 0, s);
+        if (rc) return rc;
+        if (norm <= S->atol) norm = 1.0;
+        rc = qa_norm2_z(S->n, k0, &kn2, s);
+        if (rc) return rc;
+        double tmp_norm = sqrt(kn2);
+        double factorial = 1.0;
+        for (int i = 1; i <= S->order; ++i) factorial *= i;
+        double dt1, dt2;
+        if (tmp_norm >= S->atol * 1e-6)
+            dt1 = pow(tol * factorial * pow(norm, S->order), 1.0 / (S->order + 1)) / tmp_norm;
+        else
+            dt1 = pow(tol * factorial, 1.0 / (S->order + 1)) * norm * 0.5;
+        const void *vecs[1] = {k0};
+        const double cf[1] = {dt1 / 100};
+        rc = qa_lincomb_z(S->npad, S->y1, 1, vecs, cf, S->tmp, s);
+        if (rc) return rc;
+        rc = qa_sell_matvec_z(S->n, S->nslices, S->slice_ptr, S->col, S->vals, S->vals_fp32,
+                              S->tmp, k1, 1.0, 0.0, 0, s);
+        if (rc) return rc;
+        rc = qa_norm2_z(S->n, k1, &kn2, s);
+        if (rc) return rc;
+        tmp_norm = sqrt(kn2);
+        if (tmp_norm >= S->atol * 1e-6)
+            dt2 = pow(tol * factorial * pow(norm, S->order), 1.0 / (S->order + 1)) / tmp_norm;
+        else
+            dt2 = dt1;
+        dt = std::min(dt1, dt2);
+        if (S->max_step != 0.0) dt = std::min(S->max_step, dt);
+        if (S->min_step != 0.0) dt = std::max(S->min_step, dt);
+    } else if (!S->adaptive) {
+        dt = 0.0;
+    }
+    hipLaunchKernelGGL(rk_init_kernel, dim3(1), dim3(1), 0, s, S->ctl, t0, dt);
+    QA_CHECK_LAUNCH();
+    QA_HIP(hipStreamSynchronize(s));
+    S->state_set = 1;
+    return QA_OK;
+}
+
+int qa_rk_run(qa_rk_solver *S, int ntimes, const double *times, int single_step,
+              double *fun_out, void *states_out, int *status, double *t_reached) {
+    QA_REQUIRE(S != nullptr, "qa_rk_run: null solver");
+    QA_REQUIRE(S->state_set, "qa_rk_run: set_state was not called");
+    QA_REQUIRE(ntimes >= 1 && times != nullptr, "qa_rk_run: need at least one time");
+    for (int i = 1; i < ntimes; ++i)
+        QA_REQUIRE(times[i] > times[i - 1], "qa_rk_run: times must be increasing");
+    hipStream_t s = S->stream;
+    if (ntimes > S->times_cap) {
+        if (S->times_dev) QA_HIP(hipFree(S->times_dev));
+        S->times_dev = nullptr;
+        S->times_cap = std::max(ntimes, 2 * S->times_cap);
+        QA_HIP(hipMalloc((void **)&S->times_dev, sizeof(double) * S->times_cap));
+    }
+    const int64_t need = (int64_t)ntimes * std::max(S->nfun, 1);
+    if (need > S->fun_out_cap) {
+        if (S->fun_out_dev) QA_HIP(hipFree(S->fun_out_dev));
+        S->fun_out_dev = nullptr;
+        S->fun_out_cap = std::max(need, 2 * S->fun_out_cap);
+        QA_HIP(hipMalloc((void **)&S->fun_out_dev, sizeof(double2) * S->fun_out_cap));
+    }
+    QA_HIP(hipMemcpyAsync(S->times_dev, times, sizeof(double) * ntimes, hipMemcpyHostToDevice, s));
+    // pointers and flags that can change between runs go through the ctl block
+    struct {
+        double *times;
+        double2 *fun_out;
+    } ptrs = {S->times_dev, S->fun_out_dev};
+    QA_HIP(hipMemcpyAsync((char *)S->ctl + offsetof(RKCtl, times), &ptrs, sizeof(ptrs),
+                          hipMemcpyHostToDevice, s));
+    const int nf = S->nfun;
+    QA_HIP(hipMemcpyAsync((char *)S->ctl + offsetof(RKCtl, nfun), &nf, sizeof(int),
+                          hipMemcpyHostToDevice, s));
+    QA_HIP(hipMemcpyAsync((char *)S->ctl + offsetof(RKCtl, single_step), &single_step,
+                          sizeof(int), hipMemcpyHostToDevice, s));
+    // the sources above are pageable host memory: complete the copies here
+    QA_HIP(hipStreamSynchronize(s));
+    if (S->graph_dirty) {
+        int rc = rebuild_graph(S);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(rk_begin_kernel, dim3(1), dim3(1), 0, s, S->ctl, ntimes,
+                       (double2 *)states_out);
+    QA_CHECK_LAUNCH();
+    // Replay the captured iteration in chunks; read the phase back after each.
+    int chunk = 4;
+    long long iters = 0;
+    const long long max_iters = (long long)(S->nsteps + 2) * (ntimes + 1) * 2 + 64;
+    while (true) {
+        for (int k = 0; k < chunk; ++k) QA_HIP(hipGraphLaunch(S->exec, s));
+        iters += chunk;
+        S->graph_launches += chunk;
+        QA_HIP(hipMemcpyAsync(S->ctl_host, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToHost, s));
+        QA_HIP(hipStreamSynchronize(s));
+        const RKCtl &h = *S->ctl_host;
+        if (h.phase == PH_DONE || h.phase == PH_ERROR || h.phase == PH_IDLE) break;
+        if (iters > max_iters) {
+            qa::set_error("qa_rk_run: no progress after %lld iterations", iters);
+            return QA_ERR_STATE;
+        }
+        // next chunk: estimate from the progress so far
+        const int done = std::max(h.out_idx, 1);
+        const double per_out = (double)iters / done;
+        long long est = (long long)(per_out * (ntimes - h.out_idx)) + 1;
+        chunk = (int)std::max(1LL, std::min(est, 64LL));
+    }
+    const RKCtl &h = *S->ctl_host;
+    if (status) *status = h.status;
+    if (t_reached) *t_reached = h.t_last;
+    if (fun_out && S->nfun > 0) {
+        const int ndone = h.phase == PH_DONE ? ntimes : h.out_idx;
+        QA_HIP(hipMemcpy(fun_out, S->fun_out_dev, sizeof(double2) * (size_t)ndone * S->nfun,
+                         hipMemcpyDeviceToHost));
+    }
+    if (h.phase == PH_ERROR) {
+        qa::set_error("integration failed with status %d", h.status);
+        return QA_ERR_STATE;
+    }
+    return QA_OK;
+}
+
+int qa_rk_get_state(qa_rk_solver *S, double *t, void *y_out) {
+    QA_REQUIRE(S != nullptr, "qa_rk_get_state: null solver");
+    QA_HIP(hipMemcpyAsync(S->ctl_host, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToHost, S->stream));
+    if (y_out)
+        QA_HIP(hipMemcpyAsync(y_out, S->y_last, (size_t)S->n * sizeof(double2),
+                              hipMemcpyDeviceToDevice, S->stream));
+    QA_HIP(hipStreamSynchronize(S->stream));
+    if (t) *t = S->ctl_host->t_last;
+    return QA_OK;
+}
+
+int qa_rk_stats(qa_rk_solver *S, int64_t *stats) {
+    QA_REQUIRE(S != nullptr && stats != nullptr, "qa_rk_stats: null argument");
+    QA_HIP(hipMemcpyAsync(S->ctl_host, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToHost, S->stream));
+    QA_HIP(hipStreamSynchronize(S->stream));
+    const RKCtl &h = *S->ctl_host;
+    stats[0] = h.n_attempts;
+    stats[1] = h.n_accepted;
+    stats[2] = h.n_dense;
+    stats[3] = (int64_t)h.n_attempts * S->rk_step +
+               (int64_t)h.n_dense * (S->interpolate ? S->rk_extra - S->rk_step : 0);
+    stats[4] = S->graph_launches;
+    stats[5] = h.status;
+    stats[6] = h.out_idx;
+    stats[7] = h.phase;
+    return QA_OK;
+}
+
+// Average duration (ms) of each stage kernel, measured with HIP events on
+// the engine's stream, `reps` launches per stage.  The stage kernels run on
+// the engine's current vectors, so the integration state is clobbered: call
+// qa_rk_set_state afterwards.  The control block is saved and restored; while
+// profiling, it points at a private far-future output time so the step
+// controller of the last stage only touches the control block.
+int qa_rk_profile_stages(qa_rk_solver *S, int reps, double *ms_per_stage) {
+    QA_REQUIRE(S != nullptr && reps > 0 && ms_per_stage != nullptr,
+               "qa_rk_profile_stages: bad arguments");
+    hipStream_t s = S->stream;
+    QA_HIP(hipStreamSynchronize(s));
+    RKCtl *backup = nullptr;
+    double *far_time = nullptr;
+    QA_HIP(hipMalloc((void **)&backup, sizeof(RKCtl)));
+    QA_HIP(hipMalloc((void **)&far_time, sizeof(double)));
+    QA_HIP(hipMemcpyAsync(backup, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToDevice, s));
+    QA_HIP(hipMemcpyAsync(S->ctl_host, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToHost, s));
+    QA_HIP(hipStreamSynchronize(s));
+    const double big = 1e300;
+    QA_HIP(hipMemcpy(far_time, &big, sizeof(double), hipMemcpyHostToDevice));
+    RKCtl prof = *S->ctl_host;
+    prof.times = far_time;
+    prof.ntimes = 1;
+    prof.out_idx = 0;
+    prof.nsteps_left = 1 << 30;
+    prof.nsteps_call = 0;
+    prof.counter = 0;
+    QA_HIP(hipMemcpy(S->ctl, &prof, sizeof(RKCtl), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    QA_HIP(hipEventCreate(&e0));
+    QA_HIP(hipEventCreate(&e1));
+    const int nk = S->interpolate ? S->rk_extra : S->rk_step;
+    int rc = QA_OK;
+    for (int i = 0; i < nk && rc == QA_OK; ++i) {
+        const int phase = i < S->rk_step ? PH_STEP : PH_DENSE;
+        double tot = 0.0;
+        for (int r = 0; r < reps && rc == QA_OK; ++r) {
+            hipLaunchKernelGGL(rk_set_phase_kernel, dim3(1), dim3(1), 0, s, S->ctl, phase);
+            QA_HIP(hipEventRecord(e0, s));
+            rc = S->vals_fp32 ? launch_stage<float2>(S, i) : launch_stage<double2>(S, i);
+            QA_HIP(hipEventRecord(e1, s));
+            QA_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            QA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            tot += ms;
+        }
+        ms_per_stage[i] = tot / reps;
+    }
+    QA_HIP(hipMemcpyAsync(S->ctl, backup, sizeof(RKCtl), hipMemcpyDeviceToDevice, s));
+    QA_HIP(hipStreamSynchronize(s));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(backup);
+    (void)hipFree(far_time);
+    S->state_set = 0;
+    return rc;
+}
+
+// Launch one loop iteration directly (no graph) -- used to time kernels
+// without graph replay and by tests of the gating logic.
+int qa_rk_iterate(qa_rk_solver *S, int count) {
+    QA_REQUIRE(S != nullptr && count >= 0, "qa_rk_iterate: bad arguments");
+    for (int k = 0; k < count; ++k) {
+        int rc = enqueue_iteration(S);
+        if (rc) return rc;
+    }
+    QA_HIP(hipStreamSynchronize(S->stream));
+    return QA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,455 @@
+"""Device data types and their data-layer operations.
+
+``HipDense`` and ``HipCSR`` mirror QuTiP's ``Dense`` and ``CSR`` data types
+(qutip/core/data/dense.pxd:9-33, csr.pxd:17-30) with their storage in HBM:
+
+* HipDense: complex128, column-major ("Fortran" order), so a density matrix
+  buffer *is* its column-stacked vector (qutip/core/superoperator.py:291-302)
+  and stack/unstack are free reshapes;
+* HipCSR: int32 row pointers / columns (QuTiP's ``idxint``), complex128 or
+  complex64 values; a SELL-64 copy is built on first use by the fused RK
+  engine.
+
+The module-level functions carry the signatures of the reference
+dispatchers they specialise (``matmul(left, right, scale=1)``,
+``expect(op, state)``, ``inner(left, right, scalar_is_ket=False)``, ...);
+qutip_amd.qutip_plugin registers them into qutip.core.data when qutip is
+importable.  All arithmetic runs in the gfx950 kernels; the host only moves
+data in and out (``from_array`` / ``to_array``).
+"""
+import ctypes as C
+import numbers
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+
+__all__ = ["HipDense", "HipCSR", "DeviceBuffer", "matmul", "add", "iadd", "mul", "imul",
+           "expect", "inner", "norm_l2", "frobenius", "wrmn_error", "zeros", "trace",
+           "column_stack", "column_unstack", "to_hip", "to_numpy"]
+
+
+class DeviceBuffer:
+    """A block of HBM owned by this object (freed on garbage collection)."""
+    __slots__ = ("ptr", "nbytes", "__weakref__")
+
+    def __init__(self, nbytes):
+        L = _lib.lib()
+        p = C.c_void_p()
+        _lib.check(L.qa_malloc(C.byref(p), C.c_size_t(max(int(nbytes), 1))), "qa_malloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def __del__(self):
+        p = getattr(self, "ptr", None)
+        if p and _lib._lib is not None:
+            _lib._lib.qa_free(C.c_void_p(p))
+            self.ptr = None
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("upload larger than buffer")
+        _lib.call("qa_memcpy_h2d", C.c_void_p(self.ptr), arr.ctypes.data_as(C.c_void_p),
+                  C.c_size_t(arr.nbytes), None)
+
+    def download(self, arr):
+        if arr.nbytes > self.nbytes:
+            raise ValueError("download larger than buffer")
+        _lib.call("qa_memcpy_d2h", arr.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr),
+                  C.c_size_t(arr.nbytes), None)
+        return arr
+
+    @classmethod
+    def from_array(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes)
+        b.upload(arr)
+        return b
+
+
+def _vp(x):
+    return C.c_void_p(x)
+
+
+class HipDense:
+    """Dense complex128 matrix in HBM, column-major."""
+
+    def __init__(self, shape, buf=None, offset=0):
+        self.shape = (int(shape[0]), int(shape[1]))
+        n = self.shape[0] * self.shape[1]
+        self._buf = buf if buf is not None else DeviceBuffer(16 * n)
+        self._offset = offset  # bytes
+        self.fortran = True
+
+    # -- qutip Data protocol ---------------------------------------------------
+    @classmethod
+    def sparcity(cls):
+        return "dense"
+
+    @property
+    def ptr(self):
+        return self._buf.ptr + self._offset
+
+    @property
+    def size(self):
+        return self.shape[0] * self.shape[1]
+
+    @classmethod
+    def from_array(cls, arr, shape=None):
+        a = np.asarray(arr, dtype=np.complex128)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1)
+        shape = a.shape if shape is None else shape
+        flat = np.ascontiguousarray(a.ravel(order="F"))
+        out = cls(shape)
+        out._buf.upload(flat)
+        return out
+
+    def to_array(self):
+        flat = np.empty(self.size, dtype=np.complex128)
+        if self.size:
+            _lib.call("qa_memcpy_d2h", flat.ctypes.data_as(C.c_void_p), _vp(self.ptr),
+                      C.c_size_t(flat.nbytes), None)
+        return flat.reshape(self.shape, order="F")
+
+    def copy(self):
+        out = HipDense(self.shape)
+        _lib.call("qa_memcpy_d2d", _vp(out.ptr), _vp(self.ptr), C.c_size_t(16 * self.size), None)
+        return out
+
+    def reshape(self, shape):
+        """Same buffer, new column-major shape (stack/unstack columns)."""
+        if shape[0] * shape[1] != self.size:
+            raise ValueError("reshape changes the number of elements")
+        return HipDense(shape, buf=self._buf, offset=self._offset)
+
+    def trace(self):
+        if self.shape[0] != self.shape[1]:
+            raise ValueError("trace of a non-square matrix")
+        res = (C.c_double * 2)()
+        _lib.call("qa_trace_z", self.shape[0], _vp(self.ptr), res, None)
+        return complex(res[0], res[1])
+
+    def adjoint(self):
+        return HipDense.from_array(self.to_array().conj().T)
+
+    def conj(self):
+        return HipDense.from_array(self.to_array().conj())
+
+    def transpose(self):
+        return HipDense.from_array(self.to_array().T)
+
+    def __reduce__(self):
+        return (HipDense.from_array, (self.to_array(),))
+
+    def __repr__(self):
+        return f"HipDense(shape={self.shape})"
+
+    # operators route to the dispatchers like qutip's Data (base.pyx:49-93)
+    def __add__(self, other):
+        return add(self, other)
+
+    def __sub__(self, other):
+        return add(self, other, -1)
+
+    def __mul__(self, other):
+        return mul(self, other)
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, other):
+        return mul(self, 1 / other)
+
+    def __neg__(self):
+        return mul(self, -1)
+
+    def __matmul__(self, other):
+        return matmul(self, other)
+
+
+class HipCSR:
+    """CSR matrix in HBM (int32 indices, complex128 or complex64 values)."""
+
+    def __init__(self, A, fp32=False):
+        A = sp.csr_matrix(A, dtype=np.complex128)
+        A.sort_indices()
+        self.shape = A.shape
+        self.nnz = int(A.nnz)
+        self.fp32 = bool(fp32)
+        self._host = A
+        self._rowptr = DeviceBuffer.from_array(A.indptr.astype(np.int32))
+        self._col = DeviceBuffer.from_array(A.indices.astype(np.int32) if self.nnz else
+                                            np.zeros(1, np.int32))
+        vals = A.data.astype(np.complex64 if fp32 else np.complex128)
+        self._vals = DeviceBuffer.from_array(vals if self.nnz else np.zeros(1, vals.dtype))
+        self._sell = None
+
+    @classmethod
+    def sparcity(cls):
+        return "sparse"
+
+    @classmethod
+    def from_scipy(cls, A, fp32=False):
+        return cls(A, fp32=fp32)
+
+    def to_scipy(self):
+        return self._host.copy()
+
+    def to_array(self):
+        return self._host.toarray()
+
+    def copy(self):
+        return HipCSR(self._host, fp32=self.fp32)
+
+    def adjoint(self):
+        return HipCSR(self._host.conj().T, fp32=self.fp32)
+
+    def conj(self):
+        return HipCSR(self._host.conj(), fp32=self.fp32)
+
+    def transpose(self):
+        return HipCSR(self._host.T, fp32=self.fp32)
+
+    def trace(self):
+        return complex(self._host.diagonal().sum())
+
+    def __reduce__(self):
+        return (HipCSR, (self._host, self.fp32))
+
+    def __repr__(self):
+        return f"HipCSR(shape={self.shape}, nnz={self.nnz}, fp32={self.fp32})"
+
+    @property
+    def sell(self):
+        if self._sell is None:
+            self._sell = SellMatrix(self._host, fp32=self.fp32)
+        return self._sell
+
+    def __matmul__(self, other):
+        return matmul(self, other)
+
+
+def sell_arrays(A, slice_height=64):
+    """SELL-64 layout of a CSR matrix: per slice of 64 rows, width = longest
+    row; entry j of row 64 s + l at slice_ptr[s] + 64 j + l (column-major in
+    the slice).  Padding: value 0, column = the row's last column (or 0)."""
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    n = A.shape[0]
+    ns = (n + slice_height - 1) // slice_height
+    npad = ns * slice_height
+    rn = np.zeros(npad, dtype=np.int64)
+    rn[:n] = np.diff(A.indptr)
+    width = rn.reshape(ns, slice_height).max(axis=1)
+    slice_ptr = np.zeros(ns + 1, dtype=np.int64)
+    slice_ptr[1:] = np.cumsum(width * slice_height)
+    total = int(slice_ptr[-1])
+    if total >= 2 ** 31:
+        raise ValueError("SELL storage exceeds int32 offsets")
+    cols = np.zeros(max(total, 1), dtype=np.int32)
+    vals = np.zeros(max(total, 1), dtype=np.complex128)
+    rows = np.repeat(np.arange(n), rn[:n])
+    j = np.arange(A.nnz) - np.repeat(A.indptr[:-1], rn[:n])
+    dest = slice_ptr[rows // slice_height] + j * slice_height + rows % slice_height
+    cols[dest] = A.indices
+    vals[dest] = A.data
+    # padding columns: keep loads in range and near the row
+    pad_mask = np.ones(total, dtype=bool)
+    pad_mask[dest] = False
+    if pad_mask.any():
+        pidx = np.nonzero(pad_mask)[0]
+        s_of = np.searchsorted(slice_ptr, pidx, side="right") - 1
+        lane = (pidx - slice_ptr[s_of]) % slice_height
+        r = s_of * slice_height + lane
+        cols[pidx] = np.minimum(r, A.shape[1] - 1).astype(np.int32)
+    return slice_ptr.astype(np.int32), cols, vals, npad
+
+
+class SellMatrix:
+    def __init__(self, A, fp32=False):
+        sptr, cols, vals, npad = sell_arrays(A)
+        self.shape = A.shape
+        self.nslices = len(sptr) - 1
+        self.npad = npad
+        self.stored = int(sptr[-1])
+        self.fp32 = fp32
+        self._sptr = DeviceBuffer.from_array(sptr)
+        self._cols = DeviceBuffer.from_array(cols)
+        self._vals = DeviceBuffer.from_array(vals.astype(np.complex64 if fp32 else np.complex128))
+
+    @property
+    def ptrs(self):
+        return self._sptr.ptr, self._cols.ptr, self._vals.ptr
+
+
+# ---------------------------------------------------------------------------
+# conversions (the `to` graph edges Dense <-> HipDense, CSR <-> HipCSR)
+# ---------------------------------------------------------------------------
+def to_hip(x, fp32=False):
+    if isinstance(x, (HipDense, HipCSR)):
+        return x
+    if sp.issparse(x):
+        return HipCSR(x, fp32=fp32)
+    return HipDense.from_array(np.asarray(x))
+
+
+def to_numpy(x):
+    if isinstance(x, (HipDense, HipCSR)):
+        return x.to_array()
+    return np.asarray(x)
+
+
+# ---------------------------------------------------------------------------
+# dispatcher specialisations
+# ---------------------------------------------------------------------------
+def _scalar(s):
+    s = complex(s)
+    return s.real, s.imag
+
+
+def zeros(rows, cols):
+    """zeros specialisation (qutip/core/data/constant.py:18-45)."""
+    out = HipDense((rows, cols))
+    _lib.call("qa_fill_zero", _vp(out.ptr), C.c_size_t(16 * out.size), None)
+    return out
+
+
+def matmul(left, right, scale=1, out=None):
+    """matmul(left, right, scale=1) -> HipDense (qutip/core/data/matmul.pyx:1018-1029);
+    with ``out`` it is the in-place ``out += scale * left @ right`` of
+    imatmul_data_dense (matmul.pyx:1207-1215)."""
+    if not isinstance(right, HipDense):
+        raise TypeError(f"matmul: right operand must be HipDense, got {type(right).__name__}")
+    if left.shape[1] != right.shape[0]:
+        raise ValueError(f"incompatible matrix shapes {left.shape} and {right.shape}")
+    acc = out is not None
+    if out is None:
+        out = HipDense((left.shape[0], right.shape[1]))
+    elif out.shape != (left.shape[0], right.shape[1]):
+        raise ValueError("out has the wrong shape")
+    sre, sim = _scalar(scale)
+    if isinstance(left, HipCSR):
+        if right.shape[1] == 1:
+            _lib.call("qa_csr_matvec_z", left.shape[0], left.shape[1], _vp(left._rowptr.ptr),
+                      _vp(left._col.ptr), _vp(left._vals.ptr), int(left.fp32), left.nnz,
+                      _vp(right.ptr), _vp(out.ptr), sre, sim, int(acc), None)
+        else:
+            _lib.call("qa_csr_matmat_z", left.shape[0], left.shape[1], _vp(left._rowptr.ptr),
+                      _vp(left._col.ptr), _vp(left._vals.ptr), int(left.fp32), right.shape[1],
+                      _vp(right.ptr), right.shape[0], _vp(out.ptr), out.shape[0], sre, sim,
+                      int(acc), None)
+        return out
+    raise TypeError(f"matmul: no device specialisation for {type(left).__name__} @ HipDense")
+
+
+def add(left, right, scale=1):
+    """left + scale * right (qutip/core/data/add.pyx:324-374)."""
+    if left.shape != right.shape:
+        raise ValueError(f"incompatible matrix shapes {left.shape} and {right.shape}")
+    out = HipDense(left.shape)
+    sre, sim = _scalar(scale)
+    _lib.call("qa_axpby_z", left.size, 1.0, 0.0, _vp(left.ptr), sre, sim, _vp(right.ptr),
+              _vp(out.ptr), None)
+    return out
+
+
+def iadd(left, right, scale=1):
+    """In place: left += scale * right (returns left)."""
+    if left.shape != right.shape:
+        raise ValueError(f"incompatible matrix shapes {left.shape} and {right.shape}")
+    sre, sim = _scalar(scale)
+    _lib.call("qa_axpby_z", left.size, 1.0, 0.0, _vp(left.ptr), sre, sim, _vp(right.ptr),
+              _vp(left.ptr), None)
+    return left
+
+
+def mul(matrix, value):
+    """value * matrix (qutip/core/data/mul.pyx:103-139)."""
+    if not isinstance(value, numbers.Number):
+        raise TypeError("mul: value must be a scalar")
+    out = HipDense(matrix.shape)
+    vre, vim = _scalar(value)
+    _lib.call("qa_axpby_z", matrix.size, vre, vim, _vp(matrix.ptr), 0.0, 0.0, None,
+              _vp(out.ptr), None)
+    return out
+
+
+def imul(matrix, value):
+    vre, vim = _scalar(value)
+    _lib.call("qa_axpby_z", matrix.size, vre, vim, _vp(matrix.ptr), 0.0, 0.0, None,
+              _vp(matrix.ptr), None)
+    return matrix
+
+
+def norm_l2(x):
+    """||x||_2 of a ket / bra (qutip/core/data/norm.pyx:127-135)."""
+    if x.shape[1] != 1 and x.shape[0] != 1:
+        raise ValueError("l2 norm is only defined for vectors")
+    return frobenius(x)
+
+
+def frobenius(x):
+    r = C.c_double()
+    _lib.call("qa_norm2_z", x.size, _vp(x.ptr), C.byref(r), None)
+    return float(np.sqrt(r.value))
+
+
+def wrmn_error(diff, state, atol, rtol):
+    """Weighted RMS error norm (qutip/core/data/ode.pyx:167-207)."""
+    if diff.shape != state.shape:
+        raise ValueError(f"incompatible matrix shapes {diff.shape} and {state.shape}")
+    r = C.c_double()
+    _lib.call("qa_wrmn_error_z", diff.size, _vp(diff.ptr), _vp(state.ptr), float(atol),
+              float(rtol), C.byref(r), None)
+    return r.value
+
+
+def inner(left, right, scalar_is_ket=False):
+    """<left|right> for kets (qutip/core/data/inner.pyx:323-334): left is
+    treated as a ket (conjugated) when its shape is a column."""
+    if left.size != right.size:
+        raise ValueError(f"incompatible matrix shapes {left.shape} and {right.shape}")
+    res = (C.c_double * 2)()
+    if left.shape[0] == 1 and left.shape[1] != 1:
+        # a bra: <bra|ket> = sum bra_i ket_i  (no conjugation)
+        conj = HipDense.from_array(left.to_array().conj())
+        _lib.call("qa_inner_z", left.size, _vp(conj.ptr), _vp(right.ptr), res, None)
+    else:
+        _lib.call("qa_inner_z", left.size, _vp(left.ptr), _vp(right.ptr), res, None)
+    return complex(res[0], res[1])
+
+
+def expect(op, state):
+    """expect(op, state) -> complex (qutip/core/data/expect.pyx:391-400):
+    <psi|op|psi> for a ket, tr(op rho) for a square state."""
+    if not isinstance(op, HipCSR):
+        op = HipCSR(sp.csr_matrix(to_numpy(op)))
+    res = (C.c_double * 2)()
+    if state.shape[1] == 1:
+        if op.shape[1] != state.shape[0]:
+            raise ValueError("incompatible shapes for expect")
+        _lib.call("qa_expect_csr_ket_z", op.shape[0], _vp(op._rowptr.ptr), _vp(op._col.ptr),
+                  _vp(op._vals.ptr), int(op.fp32), _vp(state.ptr), res, None)
+    else:
+        if op.shape[1] != state.shape[0] or state.shape[0] != state.shape[1]:
+            raise ValueError("incompatible shapes for expect")
+        _lib.call("qa_expect_csr_dm_z", op.shape[0], _vp(op._rowptr.ptr), _vp(op._col.ptr),
+                  _vp(op._vals.ptr), int(op.fp32), _vp(state.ptr), res, None)
+    return complex(res[0], res[1])
+
+
+def trace(x):
+    return x.trace()
+
+
+def column_stack(x):
+    """vec(rho): free for column-major storage (reshape.pyx:208-286)."""
+    return x.reshape((x.size, 1))
+
+
+def column_unstack(x, rows):
+    if x.size % rows:
+        raise ValueError("rows does not divide the vector length")
+    return x.reshape((rows, x.size // rows))

@@ -1,0 +1,10 @@
+"""Solvers of the hot path: sesolve / mesolve (fused or plain hip_rk) and
+the batched mcsolve."""
+from .integrator import Integrator, IntegratorException
+from .hip_rk import IntegratorHipRK, RKEngine
+from .result import Result
+from .solver_base import Solver
+from .sesolve import sesolve, SESolver
+from .mesolve import mesolve, MESolver
+
+Solver.add_integrator(IntegratorHipRK, "hip_rk")

@@ -1,0 +1,465 @@
+"""``hip_rk``: QuTiP's explicit Runge-Kutta integrators on the GPU.
+
+Two execution forms of the same algorithm (qutip/solver/integrator/
+explicit_rk.pyx, tableaus vern7 / tsit5):
+
+* fused (default when the right-hand side is a constant operator): the
+  ``qa_rk_*`` engine of libqutip_amd runs the entire step loop, including
+  step control, dense output and the output-time expectation values, on the
+  device (one hipGraph replay per loop iteration);
+* plain: the reference's structure -- a host loop that calls
+  ``derivative(t, y)`` once per stage on device data and combines stages with
+  device kernels; it accepts any derivative that works on HipDense (for
+  example a QobjEvo's ``matmul_data``).
+
+Options are those of IntegratorVern7 (qutip/solver/integrator/
+qutip_integrator.py:52-60) plus ``tableau`` ('vern7' | 'tsit5'), ``fused``
+(bool) and ``operator_precision`` ('fp64' | 'fp32': complex64 operator
+values, FP64 state, accumulation and error norm).
+"""
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+
+from .. import _lib
+from .. import data as _data
+from .integrator import Integrator, IntegratorException
+from .tableaus import TABLEAUS
+
+__all__ = ["IntegratorHipRK", "RKEngine", "STATUS_MESSAGES"]
+
+AT_FRONT, INTERPOLATED, NORMAL = 2, 1, 0
+TOO_MUCH_WORK, DT_UNDERFLOW, OUTSIDE_RANGE, NOT_INITIATED = -1, -2, -3, -4
+STATUS_MESSAGES = {  # explicit_rk.pyx:480-496
+    AT_FRONT: "Internal state at the desired time.",
+    INTERPOLATED: ("Internal state past the desired time and "
+                   "interpolation to step done."),
+    NORMAL: "No work done.",
+    TOO_MUCH_WORK: ("Too much work done in one call. Try to increase "
+                    "the nsteps parameter or increasing the tolerance."),
+    DT_UNDERFLOW: "Step size becomes too small. Try increasing tolerance.",
+    OUTSIDE_RANGE: "Step outside available range.",
+    NOT_INITIATED: "Not initialized.",
+}
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+class RKEngine:
+    """Owner of one ``qa_rk_solver`` (fused device RK for dy/dt = A y)."""
+
+    def __init__(self, op, tableau="vern7", rtol=1e-6, atol=1e-8, nsteps=1000, first_step=0.0,
+                 min_step=0.0, max_step=0.0, interpolate=True, fp32=False):
+        if isinstance(op, _data.HipCSR):
+            A = op.to_scipy()
+        else:
+            A = sp.csr_matrix(op)
+        if A.shape[0] != A.shape[1]:
+            raise ValueError("the generator must be square")
+        tab = TABLEAUS[tableau] if isinstance(tableau, str) else tableau
+        self.tableau = tab
+        self.n = A.shape[0]
+        self.sell = _data.SellMatrix(A, fp32=fp32)
+        self._a = np.ascontiguousarray(tab["a"], dtype=np.float64)
+        self._b = np.ascontiguousarray(tab["b"], dtype=np.float64)
+        self._c = np.ascontiguousarray(tab["c"], dtype=np.float64)
+        self._e = np.ascontiguousarray(tab["e"], dtype=np.float64) if tab.get("e") is not None else None
+        self._bi = np.ascontiguousarray(tab["bi"], dtype=np.float64) if tab.get("bi") is not None else None
+        L = _lib.lib()
+        h = C.c_void_p()
+        sptr, cols, vals = self.sell.ptrs
+        rc = L.qa_rk_create(
+            C.byref(h), self.n, self.sell.nslices, C.c_void_p(sptr), C.c_void_p(cols),
+            C.c_void_p(vals), int(fp32), int(tab["order"]), len(self._b), len(self._c),
+            self._bi.shape[1] if self._bi is not None else 0,
+            _dp(self._a), _dp(self._b), _dp(self._c),
+            _dp(self._e) if self._e is not None else None,
+            _dp(self._bi) if self._bi is not None else None,
+            float(rtol), float(atol), int(nsteps), float(first_step), float(min_step),
+            float(max_step), int(bool(interpolate)))
+        _lib.check(rc, "qa_rk_create")
+        self.handle = h
+        self.nfun = 0
+        self._y_dev = _data.HipDense((self.n, 1))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.qa_rk_destroy(h)
+            self.handle = None
+
+    def set_functionals(self, kinds, fptr, rows, cols, w):
+        kinds = np.ascontiguousarray(kinds, dtype=np.int32)
+        fptr = np.ascontiguousarray(fptr, dtype=np.int32)
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        w = np.ascontiguousarray(w, dtype=np.complex128).view(np.float64)
+        _lib.call("qa_rk_set_functionals", self.handle, len(kinds), _ip(kinds), _ip(fptr),
+                  _ip(rows), _ip(cols), _dp(w))
+        self.nfun = len(kinds)
+
+    def set_state(self, t0, y0):
+        y0 = y0 if isinstance(y0, _data.HipDense) else _data.HipDense.from_array(y0)
+        if y0.size != self.n:
+            raise ValueError(f"state has {y0.size} entries, the generator {self.n}")
+        _lib.call("qa_rk_set_state", self.handle, float(t0), C.c_void_p(y0.ptr))
+
+    def run(self, times, single_step=False, states_out=None):
+        """Returns (functionals [ntimes, nfun] complex, status, t_reached);
+        raises IntegratorException when the integration fails."""
+        times = np.ascontiguousarray(np.atleast_1d(times), dtype=np.float64)
+        out = np.zeros((len(times), max(self.nfun, 1)), dtype=np.complex128)
+        st = C.c_int(0)
+        tr = C.c_double(0.0)
+        rc = _lib.lib().qa_rk_run(self.handle, len(times), _dp(times), int(single_step),
+                                  _dp(out.view(np.float64)),
+                                  C.c_void_p(states_out) if states_out else None,
+                                  C.byref(st), C.byref(tr))
+        if rc == _lib.QA_ERR_STATE and st.value < 0:
+            raise IntegratorException(STATUS_MESSAGES.get(st.value, f"status {st.value}"))
+        _lib.check(rc, "qa_rk_run")
+        return out[:, :self.nfun], st.value, tr.value
+
+    def get_state(self, copy=True):
+        t = C.c_double(0.0)
+        _lib.call("qa_rk_get_state", self.handle, C.byref(t), C.c_void_p(self._y_dev.ptr))
+        return t.value, (self._y_dev.copy() if copy else self._y_dev)
+
+    def stats(self):
+        s = (C.c_int64 * 8)()
+        _lib.call("qa_rk_stats", self.handle, s)
+        keys = ["attempts", "accepted", "dense_preps", "rhs_evals", "graph_launches",
+                "status", "out_idx", "phase"]
+        return dict(zip(keys, [int(x) for x in s]))
+
+    def iterate(self, count):
+        _lib.call("qa_rk_iterate", self.handle, int(count))
+
+
+class PlainRK:
+    """Host-driven restatement of Explicit_RungeKutta on device data: one
+    derivative call per stage, stage combines as single lincomb kernels."""
+
+    def __init__(self, deriv, tableau, rtol, atol, nsteps, first_step, min_step, max_step,
+                 interpolate):
+        self.deriv = deriv
+        tab = TABLEAUS[tableau] if isinstance(tableau, str) else tableau
+        self.order = tab["order"]
+        self.a, self.b, self.c = tab["a"], tab["b"], tab["c"]
+        self.e, self.bi = tab.get("e"), tab.get("bi")
+        self.rk_step, self.rk_extra = len(self.b), len(self.c)
+        self.adaptive = self.e is not None
+        self.interpolate = self.bi is not None and interpolate
+        self.dorder = self.bi.shape[1] if self.bi is not None else 0
+        self.fsal = bool(np.allclose(self.a[self.rk_step - 1, :self.rk_step], self.b, atol=1e-14))
+        self.rtol, self.atol = rtol, atol
+        self.max_numsteps = nsteps
+        self.first_step = first_step
+        self.min_step = min_step or 1e-15
+        self.max_step = max_step
+        self._y_prev = None
+        self._status = NORMAL
+
+    def _f(self, t, y):
+        out = _data.zeros(*y.shape)
+        r = self.deriv(t, y, out) if _accepts_out(self.deriv) else self.deriv(t, y)
+        return out if r is None else r
+
+    def _comb(self, base, coefs, ks, dt):
+        terms = [(dt * c, k) for c, k in zip(coefs, ks) if dt * c != 0]
+        out = _data.HipDense(base.shape if base is not None else ks[0].shape)
+        vecs = (C.c_void_p * 16)(*[C.c_void_p(k.ptr) for _, k in terms])
+        cf = (C.c_double * 16)(*[c for c, _ in terms])
+        _lib.call("qa_lincomb_z", out.size, C.c_void_p(base.ptr) if base is not None else None,
+                  len(terms), vecs, cf, C.c_void_p(out.ptr), None)
+        return out
+
+    def set_initial_value(self, y0, t):
+        self._t = self._t_prev = self._t_front = t
+        self._dt_int = 0.0
+        self._y = y0
+        self._y_front = y0
+        self._y_prev = y0
+        self.k = [None] * (self.rk_extra if self.interpolate else self.rk_step)
+        if self.fsal:
+            self._k_fsal = self._f(t, y0)
+        self._dt_safe = self.first_step or self._estimate_first_step(t, y0)
+        self._status = NORMAL
+
+    def _estimate_first_step(self, t, y0):
+        if not self.adaptive:
+            return 0.0
+        norm = _data.frobenius(y0)
+        tol = self.atol + norm * self.rtol
+        self.k[0] = self._k_fsal if self.fsal else self._f(t, y0)
+        if norm <= self.atol:
+            norm = 1
+        tmp = _data.frobenius(self.k[0])
+        fact = float(np.prod(np.arange(1, self.order + 1)))
+        if tmp >= self.atol * 1e-6:
+            dt1 = (tol * fact * norm ** self.order) ** (1 / (self.order + 1)) / tmp
+        else:
+            dt1 = (tol * fact) ** (1 / (self.order + 1)) * norm * 0.5
+        yt = self._comb(y0, [1.0], [self.k[0]], dt1 / 100)
+        self.k[1] = self._f(t + dt1 / 100, yt)
+        tmp = _data.frobenius(self.k[1])
+        dt2 = (tol * fact * norm ** self.order) ** (1 / (self.order + 1)) / tmp \
+            if tmp >= self.atol * 1e-6 else dt1
+        dt = min(dt1, dt2)
+        if self.max_step:
+            dt = min(self.max_step, dt)
+        if self.min_step:
+            dt = max(self.min_step, dt)
+        return dt
+
+    def integrate(self, t, step=False):
+        nsteps_left = self.max_numsteps
+        if self._y_prev is None:
+            self._status = NOT_INITIATED
+            return
+        if t == self._t:
+            return
+        if t < self._t_prev:
+            self._status = OUTSIDE_RANGE
+            return
+        if self.interpolate and t < self._t_front:
+            if self._status != INTERPOLATED:
+                self._prep_dense_out()
+                self._status = INTERPOLATED
+            self._y = self._interp(t)
+            self._t = t
+            return
+        self._status = NORMAL
+        if step and self._t < self._t_front and t > self._t_front:
+            t = self._t_front
+        while self._t_front < t and self._status >= 0:
+            self._y_prev = self._y_front
+            self._t_prev = self._t_front
+            nsteps_left -= self._step_in_err(t, nsteps_left)
+            if step:
+                break
+        if self._status < 0:
+            return
+        if self._t_front > t:
+            self._prep_dense_out()
+            self._status = INTERPOLATED
+            self._t = t
+            self._y = self._interp(t)
+        else:
+            self._status = AT_FRONT
+            self._t = self._t_front
+            self._y = self._y_front
+
+    def _step_in_err(self, t, max_step):
+        error, nsteps = 1.0, 0
+        while error >= 1:
+            dt = self._get_timestep(t)
+            error = self._compute_step(dt)
+            self._dt_int = dt
+            self._recompute_safe_step(error, dt)
+            if dt == self.min_step and error > 1:
+                self._status = DT_UNDERFLOW
+                break
+            nsteps += 1
+            if nsteps > max_step:
+                self._status = TOO_MUCH_WORK
+                break
+        if self.fsal:
+            self._k_fsal = self.k[self.rk_step - 1]
+        return nsteps
+
+    def _compute_step(self, dt):
+        self.k[0] = self._k_fsal if self.fsal else self._f(self._t_prev, self._y_prev)
+        for i in range(1, self.rk_step):
+            yt = self._comb(self._y_prev, self.a[i, :i], self.k[:i], dt)
+            self.k[i] = self._f(self._t_prev + self.c[i] * dt, yt)
+        self._y_front = self._comb(self._y_prev, self.b, self.k[:self.rk_step], dt)
+        self._t_front = self._t_prev + dt
+        if not self.adaptive:
+            return 0.0
+        err = self._comb(None, self.e, self.k[:self.rk_step], dt)
+        return _data.wrmn_error(err, self._y_front, self.atol, self.rtol)
+
+    def _prep_dense_out(self):
+        dt = self._dt_int
+        for i in range(self.rk_step, self.rk_extra):
+            yt = self._comb(self._y_prev, self.a[i, :i], self.k[:i], dt)
+            self.k[i] = self._f(self._t_prev + self.c[i] * dt, yt)
+
+    def _interp(self, t):
+        tau = (t - self._t_prev) / self._dt_int
+        bf = np.zeros(self.rk_extra)
+        for i in range(self.rk_extra):
+            v = 0.0
+            for j in range(self.dorder - 1, -1, -1):
+                v += self.bi[i, j]
+                v *= tau
+            bf[i] = v
+        return self._comb(self._y_prev, bf, self.k[:self.rk_extra], self._dt_int)
+
+    def _get_timestep(self, t):
+        dt_needed = t - self._t_prev
+        if not self.adaptive:
+            return dt_needed
+        if self.interpolate:
+            return self._dt_safe
+        if dt_needed <= self._dt_safe:
+            return dt_needed
+        return dt_needed / (int(dt_needed / self._dt_safe) + 1)
+
+    def _recompute_safe_step(self, err, dt):
+        if not self.adaptive:
+            return
+        if err == 0:
+            factor = 10.0
+        else:
+            factor = min(10.0, max(0.2, 0.9 * err ** (-1 / (self.order + 1))))
+        self._dt_safe = dt * factor
+        if self.max_step:
+            self._dt_safe = min(self.max_step, self._dt_safe)
+        if self.min_step:
+            self._dt_safe = max(self.min_step, self._dt_safe)
+
+
+def _accepts_out(fn):
+    return getattr(fn, "__name__", "") in ("matmul_data",)
+
+
+def constant_operator(derivative):
+    """The constant generator behind ``derivative`` when it is the bound
+    ``matmul_data`` of a constant QobjEvo (qutip/solver/integrator/_rhs.pyx:
+    52-57 finds the QobjEvo the same way); None otherwise."""
+    owner = getattr(derivative, "__self__", None)
+    if owner is None or not getattr(owner, "isconstant", False):
+        return None
+    q = owner(0.0)
+    d = getattr(q, "data", None)
+    if d is None:
+        return None
+    if isinstance(d, _data.HipCSR):
+        return d.to_scipy()
+    if sp.issparse(d):
+        return sp.csr_matrix(d)
+    return sp.csr_matrix(np.asarray(d))
+
+
+class IntegratorHipRK(Integrator):
+    """Explicit RK (vern7 / tsit5) on the GPU.  Usable with
+    ``method="hip_rk"``."""
+    integrator_options = {
+        "atol": 1e-8,
+        "rtol": 1e-6,
+        "nsteps": 1000,
+        "first_step": 0,
+        "max_step": 0,
+        "min_step": 0,
+        "interpolate": True,
+        "tableau": "vern7",
+        "fused": True,
+        "operator_precision": "fp64",
+    }
+    method = "hip_rk"
+    name = "hip_rk"
+
+    def _prepare(self):
+        o = self.options
+        self._engine = None
+        self._plain = None
+        op = constant_operator(self.derivative) if o["fused"] else None
+        if op is not None:
+            self._engine = RKEngine(op, o["tableau"], o["rtol"], o["atol"], o["nsteps"],
+                                    o["first_step"], o["min_step"], o["max_step"],
+                                    o["interpolate"], fp32=(o["operator_precision"] == "fp32"))
+            self.name = "hip_rk (fused)"
+        else:
+            self._plain = PlainRK(self.derivative, o["tableau"], o["rtol"], o["atol"],
+                                  o["nsteps"], o["first_step"], o["min_step"], o["max_step"],
+                                  o["interpolate"])
+            self.name = "hip_rk (plain)"
+
+    @property
+    def fused(self):
+        return self._engine is not None
+
+    def _to_device(self, state):
+        if isinstance(state, _data.HipDense):
+            return state
+        arr = state.full() if hasattr(state, "full") else np.asarray(state)
+        return _data.HipDense.from_array(arr)
+
+    def set_state(self, t, state):
+        y = self._to_device(state)
+        self._shape = y.shape
+        if self._engine is not None and y.size != self._engine.n:
+            # a block of columns (e.g. a propagator): run the plain form,
+            # whose derivative handles matrices
+            o = self.options
+            self._engine = None
+            self._plain = PlainRK(self.derivative, o["tableau"], o["rtol"], o["atol"],
+                                  o["nsteps"], o["first_step"], o["min_step"], o["max_step"],
+                                  o["interpolate"])
+            self.name = "hip_rk (plain)"
+        if self._engine is not None:
+            self._engine.set_state(t, y.reshape((y.size, 1)))
+        else:
+            self._plain.set_initial_value(y.copy(), t)
+        self._is_set = True
+
+    def get_state(self, copy=True):
+        if self._engine is not None:
+            t, y = self._engine.get_state(copy)
+            return t, y.reshape(self._shape)
+        y = self._plain._y
+        return self._plain._t, (y.copy() if copy else y)
+
+    def _check(self):
+        st = self._plain._status
+        if st < 0:
+            raise IntegratorException(STATUS_MESSAGES[st])
+
+    def integrate(self, t, copy=True):
+        if self._engine is not None:
+            self._engine.run([t])
+        else:
+            self._plain.integrate(t, step=False)
+            self._check()
+        return self.get_state(copy)
+
+    def mcstep(self, t, copy=True):
+        if self._engine is not None:
+            self._engine.run([t], single_step=True)
+        else:
+            self._plain.integrate(t, step=True)
+            self._check()
+        return self.get_state(copy)
+
+    def run_functionals(self, tlist, kinds, fptr, rows, cols, w):
+        """Fused Solver.run: functionals of the state at every time of
+        ``tlist[1:]`` evaluated on the device.  Returns [len(tlist)-1, nfun]."""
+        if self._engine is None:
+            raise IntegratorException("run_functionals needs the fused engine")
+        self._engine.set_functionals(kinds, fptr, rows, cols, w)
+        vals, _, _ = self._engine.run(np.asarray(tlist[1:], dtype=np.float64))
+        return vals
+
+    def stats(self):
+        return self._engine.stats() if self._engine is not None else {}
+
+    @property
+    def options(self):
+        return self._options
+
+    @options.setter
+    def options(self, new_options):
+        Integrator.options.fset(self, new_options)
+        if self._options.get("tableau") not in TABLEAUS:
+            raise ValueError(f"unknown tableau {self._options.get('tableau')!r}")
+        if self._options.get("operator_precision") not in ("fp64", "fp32"):
+            raise ValueError("operator_precision must be 'fp64' or 'fp32'")

@@ -1,0 +1,45 @@
+"""mesolve / MESolver (qutip/solver/mesolve.py:24-277): Lindblad master
+equation in the column-stacked superoperator form,
+d vec(rho)/dt = L vec(rho) with L = liouvillian(H, c_ops)
+(qutip/core/superoperator.py:51-142)."""
+import numpy as np
+
+from ..core import Qobj, QobjEvo, liouvillian
+from .hip_rk import IntegratorHipRK
+from .sesolve import sesolve
+from .solver_base import Solver
+
+__all__ = ["mesolve", "MESolver"]
+
+
+class MESolver(Solver):
+    name = "mesolve"
+    _avail_integrators = {}
+    _preserves_hermiticity = True
+
+    def __init__(self, H, c_ops=None, *, options=None):
+        if isinstance(H, QobjEvo):
+            H = H.qobj
+        c_ops = [c.qobj if isinstance(c, QobjEvo) else c for c in (c_ops or [])]
+        for c in c_ops:
+            if not isinstance(c, Qobj):
+                raise TypeError("c_ops must be Qobj")
+        self._num_collapse = len(c_ops)
+        rhs = liouvillian(H, [c for c in c_ops if not c.issuper])
+        for c in c_ops:
+            if c.issuper:
+                rhs = rhs + c
+        super().__init__(QobjEvo(rhs), options=options)
+
+
+MESolver.add_integrator(IntegratorHipRK, "hip_rk")
+
+
+def mesolve(H, rho0, tlist, c_ops=None, e_ops=None, args=None, options=None, **kwargs):
+    """Master-equation evolution; a ket without collapse operators goes to
+    sesolve like the reference (mesolve.py:155-162)."""
+    c_ops = list(c_ops or [])
+    if not c_ops and isinstance(rho0, Qobj) and rho0.isket and not H.issuper:
+        return sesolve(H, rho0, tlist, e_ops=e_ops, args=args, options=options)
+    solver = MESolver(H, c_ops, options=options)
+    return solver.run(rho0, tlist, e_ops=e_ops, args=args)

@@ -1,0 +1,104 @@
+"""Host-side logic of the product (CPU only): SELL-64 layout, output
+functionals, the object model against the oracle's independent
+construction, option handling."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from numpy.testing import assert_allclose
+
+import qutip_amd as qa
+from qutip_amd import data as qdata
+from oracle import models
+from tests import systems
+
+
+def _sell_matvec(sptr, cols, vals, n, x):
+    """numpy emulation of the SELL-64 kernel's row dot products."""
+    ns = len(sptr) - 1
+    out = np.zeros(ns * 64, dtype=np.complex128)
+    for s in range(ns):
+        w = (sptr[s + 1] - sptr[s]) // 64
+        blk_c = cols[sptr[s]:sptr[s + 1]].reshape(w, 64)
+        blk_v = vals[sptr[s]:sptr[s + 1]].reshape(w, 64)
+        out[s * 64:(s + 1) * 64] = (blk_v * x[blk_c]).sum(axis=0)
+    return out[:n]
+
+
+@pytest.mark.parametrize("n,density", [(1, 1.0), (63, 0.1), (64, 0.05), (200, 0.03), (130, 0.0)])
+def test_sell_layout_reproduces_csr(n, density):
+    rng = np.random.default_rng(n)
+    A = sp.random(n, n, density=density, random_state=rng, format="csr") * (1 + 0.5j)
+    sptr, cols, vals, npad = qdata.sell_arrays(A)
+    assert npad % 64 == 0 and npad >= n
+    assert cols.min() >= 0 and cols.max() < n
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert_allclose(_sell_matvec(sptr, cols, vals, n, x), A @ x, atol=1e-13)
+
+
+def test_sell_padding_overhead_on_ising():
+    m = models.ising(8, 0.1)
+    Lm = models.liouvillian(m["H"], m["c_ops"])
+    sptr, _, _, _ = qdata.sell_arrays(Lm)
+    assert sptr[-1] / Lm.nnz < 1.09  # 7.9 % padding
+
+
+def test_product_liouvillian_equals_oracle():
+    s = systems.jaynes_cummings()
+    Lp = qa.liouvillian(s["H"], s["c_ops"]).data
+    m = models.jaynes_cummings()
+    Lo = models.liouvillian(m["H"], m["c_ops"])
+    assert abs(Lp - Lo).max() < 1e-13
+    s = systems.ising(4, 0.1)
+    m = models.ising(4, 0.1)
+    assert abs(qa.liouvillian(s["H"], s["c_ops"]).data - models.liouvillian(m["H"], m["c_ops"])).max() < 1e-13
+
+
+def test_functionals_give_expectations():
+    """The linear functionals of a vectorised rho reproduce tr(O rho)."""
+    s = systems.jaynes_cummings()
+    solver = qa.MESolver.__new__(qa.MESolver)
+    solver._state_is_ket = False
+    solver._state_shape = (20, 20)
+    kinds, fptr, rows, cols, w = qa.Solver._functionals(solver, s["e_ops"])
+    rng = np.random.default_rng(1)
+    r = rng.standard_normal((20, 20)) + 1j * rng.standard_normal((20, 20))
+    rho = r @ r.conj().T
+    v = rho.ravel(order="F")
+    for f, O in enumerate(s["e_ops"] + [qa.qeye(20)]):
+        val = np.sum(w[fptr[f]:fptr[f + 1]] * v[cols[fptr[f]:fptr[f + 1]]])
+        assert val == pytest.approx(models.expect_dm(sp.csr_matrix(O.data), rho), rel=1e-12)
+
+
+def test_functionals_quadratic_ket():
+    s = systems.ising(4, 0.1)
+    solver = qa.SESolver.__new__(qa.SESolver)
+    solver._state_is_ket = True
+    solver._state_shape = (16, 1)
+    kinds, fptr, rows, cols, w = qa.Solver._functionals(solver, s["e_ops"])
+    assert list(kinds) == [1, 1, 1]
+    rng = np.random.default_rng(2)
+    psi = rng.standard_normal(16) + 1j * rng.standard_normal(16)
+    for f, O in enumerate(s["e_ops"]):
+        sl = slice(fptr[f], fptr[f + 1])
+        val = np.sum(w[sl] * psi[rows[sl]].conj() * psi[cols[sl]])
+        assert val == pytest.approx(models.expect_ket(sp.csr_matrix(O.data), psi), rel=1e-12)
+
+
+def test_unknown_option_raises_keyerror():
+    s = systems.jaynes_cummings()
+    with pytest.raises(KeyError):
+        qa.MESolver(s["H"], s["c_ops"], options={"not_an_option": 1})
+
+
+def test_integrator_registry():
+    assert qa.MESolver.avail_integrators()["hip_rk"] is qa.IntegratorHipRK
+    assert qa.SESolver.avail_integrators()["hip_rk"] is qa.IntegratorHipRK
+    assert qa.IntegratorHipRK.integrator_options["rtol"] == 1e-6
+
+
+def test_device_ops_fail_loudly_without_gpu():
+    import torch  # noqa: F401  (only to make sure no GPU is visible here)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(qa._lib.QutipAmdUnavailable):
+        qa.HipDense.from_array(np.ones(4))
