@@ -935,9 +935,11 @@ int qa_rk_run(qa_rk_solver *S, int ntimes, const double *times, int single_step,
                        (double2 *)states_out);
     QA_CHECK_LAUNCH();
     // Replay the captured iteration in chunks; read the phase back after each.
+    // Every iteration either attempts a step or emits outputs, so a chunk that
+    // changes neither counter means the device loop is stuck: stop there.
     int chunk = 4;
     long long iters = 0;
-    const long long max_iters = (long long)(S->nsteps + 2) * (ntimes + 1) * 2 + 64;
+    long long last_attempts = -1, last_outputs = -1;
     while (true) {
         for (int k = 0; k < chunk; ++k) QA_HIP(hipGraphLaunch(S->exec, s));
         iters += chunk;
@@ -946,10 +948,13 @@ int qa_rk_run(qa_rk_solver *S, int ntimes, const double *times, int single_step,
         QA_HIP(hipStreamSynchronize(s));
         const RKCtl &h = *S->ctl_host;
         if (h.phase == PH_DONE || h.phase == PH_ERROR || h.phase == PH_IDLE) break;
-        if (iters > max_iters) {
-            qa::set_error("qa_rk_run: no progress after %lld iterations", iters);
+        if (h.n_attempts == last_attempts && h.n_outputs == last_outputs) {
+            qa::set_error("qa_rk_run: device loop made no progress (phase %d, output %d)",
+                          h.phase, h.out_idx);
             return QA_ERR_STATE;
         }
+        last_attempts = h.n_attempts;
+        last_outputs = h.n_outputs;
         // next chunk: estimate from the progress so far
         const int done = std::max(h.out_idx, 1);
         const double per_out = (double)iters / done;
