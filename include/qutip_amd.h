@@ -177,8 +177,10 @@ int qa_rk_run(qa_rk_solver *s, int ntimes, const double *times, int single_step,
 /* get_state: time and state (device, n complex128) of the last output. */
 int qa_rk_get_state(qa_rk_solver *s, double *t, void *y_out);
 
-/* stats[0..7]: attempts, accepted steps, dense-output preparations, RHS
- * evaluations, graph launches, status, output index, phase. */
+/* stats[0..7] since the last qa_rk_set_state: step attempts, accepted
+ * steps, dense-output preparations, derivative (RHS) evaluations including
+ * the two of the first-step estimate, graph launches, status, output index,
+ * phase. */
 int qa_rk_stats(qa_rk_solver *s, int64_t *stats);
 
 /* Average duration (ms, HIP events on the engine stream, `reps` launches) of

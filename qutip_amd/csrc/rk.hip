@@ -453,6 +453,7 @@ __global__ void rk_init_kernel(RKCtl *c, double t0, double dt_safe) {
     c->dense_ready = 0;
     c->status = ST_NORMAL;
     c->phase = PH_IDLE;
+    c->n_attempts = c->n_accepted = c->n_dense = c->n_outputs = 0;
 }
 
 // Profiling helper: force the phase a stage kernel waits for.
@@ -497,6 +498,7 @@ struct qa_rk_solver {
     bool graph_dirty = true;
     RKCtl *ctl_host = nullptr;  // pinned mirror
     long long graph_launches = 0;
+    long long setup_rhs = 0;  // derivative evaluations of the first-step estimate
     int state_set = 0;
 };
 
@@ -843,6 +845,8 @@ int qa_rk_set_state(qa_rk_solver *S, double t0, const void *y0_dev) {
     QA_HIP(hipMemcpyAsync(S->y_last, S->y1, (size_t)S->npad * sizeof(double2),
                           hipMemcpyDeviceToDevice, s));
     double dt = S->first_step;
+    S->setup_rhs = 0;
+    S->graph_launches = 0;
     if (dt == 0.0 && S->adaptive) {
         double norm2 = 0.0, kn2 = 0.0;
         rc = qa_norm2_z(S->n, S->y1, &norm2, s);
@@ -873,6 +877,7 @@ int qa_rk_set_state(qa_rk_solver *S, double t0, const void *y0_dev) {
         if (rc) return rc;
         rc = qa_norm2_z(S->n, k1, &kn2, s);
         if (rc) return rc;
+        S->setup_rhs += 2;
         tmp_norm = sqrt(kn2);
         if (tmp_norm >= S->atol * 1e-6)
             dt2 = pow(tol * factorial * pow(norm, S->order), 1.0 / (S->order + 1)) / tmp_norm;
@@ -996,7 +1001,8 @@ int qa_rk_stats(qa_rk_solver *S, int64_t *stats) {
     stats[1] = h.n_accepted;
     stats[2] = h.n_dense;
     stats[3] = (int64_t)h.n_attempts * S->rk_step +
-               (int64_t)h.n_dense * (S->interpolate ? S->rk_extra - S->rk_step : 0);
+               (int64_t)h.n_dense * (S->interpolate ? S->rk_extra - S->rk_step : 0) +
+               S->setup_rhs;
     stats[4] = S->graph_launches;
     stats[5] = h.status;
     stats[6] = h.out_idx;
