@@ -82,7 +82,7 @@ def main():
         sys.exit(2)
 
     import qutip_amd as qa
-    from qutip_amd import _lib
+    from qutip_amd import _lib, autotune
     from qutip_amd.solver.hip_rk import RKEngine
     from tests import systems
 
@@ -114,7 +114,8 @@ def main():
     log(f"fused: {value:.2f} solves/s, err {err:.2e}, stats {stats}")
 
     # roofline of the stage kernel (HIP events on the engine's stream)
-    eng = RKEngine(L, "vern7", 1e-8, 1e-10)
+    lanes = solver._integrator._engine.lanes
+    eng = RKEngine(L, "vern7", 1e-8, 1e-10, lanes=lanes)
     ms = np.zeros(16)
     _lib.call("qa_rk_profile_stages", eng.handle, 20, ms.ctypes.data_as(C.POINTER(C.c_double)))
     byts = stage_bytes(eng, nnz, n, False)
@@ -138,11 +139,12 @@ def main():
                         "complex128; vern7 rtol=1e-8 atol=1e-10; t in [0,10], 200 outputs; "
                         "2 e_ops; fused hip_rk (device step control, hipGraph replay)",
             "n": n, "nnz": nnz, "rk_attempts": stats["attempts"],
-            "rhs_evals": stats["rhs_evals"]},
+            "rhs_evals": stats["rhs_evals"], "sell_lanes_per_row": lanes,
+            "autotune": autotune.cache()},
         "err_vs_fp64": float(err),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "rk_stage_kernel<double2>",
+                     "traffic": None, "kernel": f"rk_stage_kernel<double2, {lanes}>",
                      "avg_launch_us": round(avg_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": int(avg_bytes)},
     }

@@ -12,10 +12,11 @@
  *   - complex numbers are interleaved (re, im) doubles (complex128) or floats
  *     (complex64 operator values when vals_fp32 != 0) -- numpy's layout.
  *   - sparse operators: CSR with int32 row pointers and column indices
- *     (QuTiP's idxint, qutip/core/data/base.pxd:3-6), or SELL-64: rows in
- *     slices of 64, slice s holding width_s entries per row stored
- *     column-major (entry j of row 64 s + l at slice_ptr[s] + 64 j + l),
- *     padding entries with value 0.
+ *     (QuTiP's idxint, qutip/core/data/base.pxd:3-6), or
+ *     SELL-C: rows in slices of C = 64 / L (L = lanes per row, 1/2/4/8);
+ *     slice s holds width_s (a multiple of L) entries per row, entry j of
+ *     row C s + r at slice_ptr[s] + 64 (j / L) + L r + (j % L), so one
+ *     wavefront reads 64 consecutive entries; padding entries have value 0.
  *   - stream: a hipStream_t passed as void*; NULL is the default stream.
  *   - dense matrices are column-major ("Fortran" Dense,
  *     qutip/core/data/dense.pxd:9-33), so vec(rho) is the buffer itself
@@ -83,8 +84,9 @@ int qa_csr_matmat_z(int nrows, int ncols, const int *rowptr, const int *col, con
                     int vals_fp32, int nvec, const void *x, int64_t ldx, void *out, int64_t ldo,
                     double scale_re, double scale_im, int accumulate, void *stream);
 
-/* SELL-64 version of qa_csr_matvec_z (nslices = ceil(nrows / 64)). */
-int qa_sell_matvec_z(int nrows, int nslices, const int *slice_ptr, const int *col,
+/* SELL-C version of qa_csr_matvec_z with `lanes` (1, 2, 4, 8) lanes per row
+ * and C = 64 / lanes rows per slice (nslices = ceil(nrows / C)). */
+int qa_sell_matvec_z(int nrows, int lanes, int nslices, const int *slice_ptr, const int *col,
                      const void *vals, int vals_fp32, const void *x, void *out, double scale_re,
                      double scale_im, int accumulate, void *stream);
 
@@ -130,7 +132,7 @@ int qa_expect_csr_dm_z(int d, const int *rowptr, const int *col, const void *val
 /* ---- fused Runge-Kutta engine ---------------------------------------------
  * Replaces Explicit_RungeKutta (qutip/solver/integrator/explicit_rk.pyx) with
  * its derivative QobjEvo.matmul_data (qutip/core/cy/qobjevo.pyx:1103-1116)
- * for a constant generator A in SELL-64 form: dy/dt = A y.  The whole step
+ * for a constant generator A in SELL-C form: dy/dt = A y.  The whole step
  * loop (stages, error norm, step control, dense output, output-time
  * interpolation and expectation functionals) runs on the device.
  *
@@ -138,12 +140,14 @@ int qa_expect_csr_dm_z(int d, const int *rowptr, const int *col, const void *val
  * operator alive) and owns its own stream and work vectors. */
 typedef struct qa_rk_solver qa_rk_solver;
 
-/* Tableau as in explicit_rk.pyx:117-151: a[rk_extra][rk_extra], b[rk_step],
+/* Operator in SELL-C form with `lanes` lanes per row.
+ * Tableau as in explicit_rk.pyx:117-151: a[rk_extra][rk_extra], b[rk_step],
  * c[rk_extra], e[rk_step] (NULL: fixed step), bi[rk_extra][dorder] (NULL: no
  * dense output).  Options as in IntegratorVern7.integrator_options
  * (qutip/solver/integrator/qutip_integrator.py:52-60). */
-int qa_rk_create(qa_rk_solver **out, int n, int nslices, const int *slice_ptr, const int *col,
-                 const void *vals, int vals_fp32, int order, int rk_step, int rk_extra,
+int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *slice_ptr,
+                 const int *col, const void *vals, int vals_fp32, int order, int rk_step,
+                 int rk_extra,
                  int dorder, const double *a, const double *b, const double *c,
                  const double *e, const double *bi, double rtol, double atol, int nsteps,
                  double first_step, double min_step, double max_step, int interpolate);
@@ -167,7 +171,8 @@ int qa_rk_set_state(qa_rk_solver *s, double t0, const void *y0);
  * Integrator.run (qutip/solver/integrator/integrator.py:197-212).  With
  * single_step != 0 and ntimes == 1 this is integrate(t, step=True)
  * (mcstep).  fun_out (host, ntimes x nfun complex) receives the
- * functionals; states_out (device, ntimes x ceil(n/64)*64 complex, or NULL)
+ * functionals; states_out (device, ntimes x npad complex with
+ * npad = nslices * 64 / lanes, or NULL)
  * the states.  *status gets the integrator status (explicit_rk.pxd:5-12),
  * *t_reached the time of the last output.  Returns QA_ERR_STATE when the
  * integration failed (status < 0). */

@@ -3,7 +3,7 @@
 //
 // Each entry point cites the reference kernel or dispatcher it replaces.
 // Operators: CSR (int32 row pointers and columns, complex128 or complex64
-// values) and SELL-64 (rows in slices of one wavefront, entries stored
+// values) and SELL-C (rows in slices of one wavefront, entries stored
 // column-major inside a slice so the 64 lanes of a wave read consecutive
 // addresses; padding entries have value 0).
 #include <mutex>
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void csr_matmat_kernel(int nrows, const int *_
     *o = r;
 }
 
-template <typename V>
+template <typename V, int L>
 __global__ __launch_bounds__(256) void sell_matvec_kernel(int nrows, int nslices,
                                                           const int *__restrict__ slice_ptr,
                                                           const int *__restrict__ col,
@@ -74,11 +74,12 @@ __global__ __launch_bounds__(256) void sell_matvec_kernel(int nrows, int nslices
                                                           const double2 *__restrict__ x,
                                                           double2 *__restrict__ out, double2 scale,
                                                           int accumulate) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int slice = (int)(r / QA_WAVE), lane = (int)(r % QA_WAVE);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = (int)(t / QA_WAVE), lane = (int)(t % QA_WAVE);
     if (slice >= nslices) return;
-    double2 acc = sell_row_dot(slice_ptr, col, vals, x, slice, lane);
-    if (r < nrows) {
+    const double2 acc = sell_row_dot<L>(slice_ptr, col, vals, x, slice, lane);
+    const int64_t r = (int64_t)slice * (QA_WAVE / L) + lane / L;
+    if (lane % L == 0 && r < nrows) {
         double2 o = cmul(scale, acc);
         if (accumulate) o = cadd(o, out[r]);
         out[r] = o;
@@ -293,6 +294,35 @@ int lanes_for(int nrows, int nnz) {
 }
 }  // namespace
 
+template <typename V>
+static int launch_sell(int lanes, unsigned blocks, hipStream_t s, int nrows, int nslices,
+                       const int *slice_ptr, const int *col, const V *vals, const double2 *x,
+                       double2 *out, double2 sc, int accumulate) {
+    switch (lanes) {
+        case 1:
+            hipLaunchKernelGGL((sell_matvec_kernel<V, 1>), dim3(blocks), dim3(256), 0, s, nrows,
+                               nslices, slice_ptr, col, vals, x, out, sc, accumulate);
+            break;
+        case 2:
+            hipLaunchKernelGGL((sell_matvec_kernel<V, 2>), dim3(blocks), dim3(256), 0, s, nrows,
+                               nslices, slice_ptr, col, vals, x, out, sc, accumulate);
+            break;
+        case 4:
+            hipLaunchKernelGGL((sell_matvec_kernel<V, 4>), dim3(blocks), dim3(256), 0, s, nrows,
+                               nslices, slice_ptr, col, vals, x, out, sc, accumulate);
+            break;
+        case 8:
+            hipLaunchKernelGGL((sell_matvec_kernel<V, 8>), dim3(blocks), dim3(256), 0, s, nrows,
+                               nslices, slice_ptr, col, vals, x, out, sc, accumulate);
+            break;
+        default:
+            qa::set_error("SELL lanes per row must be 1, 2, 4 or 8 (got %d)", lanes);
+            return QA_ERR_ARG;
+    }
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
 extern "C" {
 
 int qa_csr_matvec_z(int nrows, int ncols, const int *rowptr, const int *col, const void *vals,
@@ -333,25 +363,23 @@ int qa_csr_matmat_z(int nrows, int ncols, const int *rowptr, const int *col, con
     return QA_OK;
 }
 
-int qa_sell_matvec_z(int nrows, int nslices, const int *slice_ptr, const int *col,
+int qa_sell_matvec_z(int nrows, int lanes, int nslices, const int *slice_ptr, const int *col,
                      const void *vals, int vals_fp32, const void *x, void *out, double scale_re,
                      double scale_im, int accumulate, void *stream) {
-    QA_REQUIRE(nrows >= 0 && nslices == (nrows + QA_WAVE - 1) / QA_WAVE,
-               "qa_sell_matvec_z: nslices must be ceil(nrows / 64)");
+    QA_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8,
+               "qa_sell_matvec_z: lanes must be 1, 2, 4 or 8");
+    const int C = QA_WAVE / lanes;
+    QA_REQUIRE(nrows >= 0 && nslices == (nrows + C - 1) / C,
+               "qa_sell_matvec_z: nslices must be ceil(nrows / (64 / lanes))");
     if (nrows == 0) return QA_OK;
     const unsigned blocks = (unsigned)(((int64_t)nslices * QA_WAVE + 255) / 256);
     const double2 sc = make_double2(scale_re, scale_im);
     hipStream_t s = (hipStream_t)stream;
     if (vals_fp32)
-        hipLaunchKernelGGL(sell_matvec_kernel<float2>, dim3(blocks), dim3(256), 0, s, nrows,
-                           nslices, slice_ptr, col, (const float2 *)vals, (const double2 *)x,
-                           (double2 *)out, sc, accumulate);
-    else
-        hipLaunchKernelGGL(sell_matvec_kernel<double2>, dim3(blocks), dim3(256), 0, s, nrows,
-                           nslices, slice_ptr, col, (const double2 *)vals, (const double2 *)x,
-                           (double2 *)out, sc, accumulate);
-    QA_CHECK_LAUNCH();
-    return QA_OK;
+        return launch_sell(lanes, blocks, s, nrows, nslices, slice_ptr, col, (const float2 *)vals,
+                           (const double2 *)x, (double2 *)out, sc, accumulate);
+    return launch_sell(lanes, blocks, s, nrows, nslices, slice_ptr, col, (const double2 *)vals,
+                       (const double2 *)x, (double2 *)out, sc, accumulate);
 }
 
 int qa_lincomb_z(int64_t n, const void *base, int nterms, const void *const *vecs,

@@ -92,10 +92,14 @@ __device__ __forceinline__ bool last_block_done(unsigned int *counter) {
     return is_last != 0;
 }
 
-// SELL-64 row dot product: lane `lane` of slice `slice` owns one row; the
-// entries of a slice are stored column-major so a wave reads 64 consecutive
-// values and columns per step.  Two accumulators, four loads in flight.
-template <typename V>
+// SELL-C row dot product with L lanes per row (C = 64 / L rows per slice;
+// one wavefront per slice).  Entry j of row C s + r is stored at
+// slice_ptr[s] + 64 (j / L) + L r + (j % L), so each load instruction of the
+// wave reads 64 consecutive values and columns; lane l = L r + q handles
+// entries q, q + L, ... of row r and the L partial sums are combined with
+// cross-lane shuffles.  Two accumulators keep four loads in flight.  The
+// result is valid in every lane of the row.
+template <int L, typename V>
 __device__ __forceinline__ double2 sell_row_dot(const int *__restrict__ slice_ptr,
                                                 const int *__restrict__ col,
                                                 const V *__restrict__ vals,
@@ -115,7 +119,21 @@ __device__ __forceinline__ double2 sell_row_dot(const int *__restrict__ slice_pt
         cfma(a1, v3, x3);
     }
     for (; p < end; p += QA_WAVE) cfma(a0, ld(vals, p), x[col[p]]);
-    return cadd(a0, a1);
+    double2 acc = cadd(a0, a1);
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) {
+        acc.x += __shfl_xor(acc.x, off, QA_WAVE);
+        acc.y += __shfl_xor(acc.y, off, QA_WAVE);
+    }
+    return acc;
+}
+
+// Sum of `n` per-workgroup partials by the whole workgroup, in a fixed
+// order (strided per thread, then the block tree): deterministic.
+__device__ __forceinline__ double sum_partials(const double *partials, int n, double *smem) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += partials[i];
+    return block_sum(v, smem);
 }
 
 }  // namespace qa

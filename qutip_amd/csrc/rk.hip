@@ -5,7 +5,7 @@
 // control :333-356 and :439-467, stages :358-389, error :391-397, dense output
 // :399-430) with the work laid out for gfx950:
 //
-// * one kernel per RK stage: the SELL-64 spmv k_i = A x_i with an epilogue
+// * one kernel per RK stage: the SELL-C spmv k_i = A x_i with an epilogue
 //   that forms the next stage input x_{i+1} = y_prev + dt sum_j a_{i+1,j} k_j
 //   row by row (the reference does a copy plus one zaxpy per coefficient per
 //   stage, explicit_rk.pyx:432-437);
@@ -202,7 +202,7 @@ struct StageArgs {
     double ecoef[kMaxStages];
 };
 
-template <typename V>
+template <typename V, int L>
 __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
     RKCtl *ctl = P.ctl;
     if (ctl->phase != P.run_phase) return;  // uniform over the whole grid
@@ -213,35 +213,38 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
     const double2 *xin = P.input == IN_YPREV    ? yprev
                          : P.input == IN_YFRONT ? yfront
                          : ((P.stage & 1) ? P.x1 : P.x0);
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int slice = (int)(r / QA_WAVE), lane = (int)(r % QA_WAVE);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = (int)(t / QA_WAVE), lane = (int)(t % QA_WAVE);
+    const int64_t r = (int64_t)slice * (QA_WAVE / L) + lane / L;  // the row of this lane
     double part = 0.0;
     if (slice < P.nslices) {
-        const double2 acc = sell_row_dot(P.slice_ptr, P.col, P.vals, xin, slice, lane);
-        P.K[(int64_t)P.stage * P.npad + r] = acc;
-        if (P.epi == EPI_NEXT) {
-            double2 o = yprev[r];
-            for (int j = 0; j < P.nterms; ++j) {
-                const double f = dt * P.coef[j];
-                if (f != 0.0) raxpy(o, f, j == P.stage ? acc : P.K[(int64_t)j * P.npad + r]);
-            }
-            double2 *xout = ((P.stage + 1) & 1) ? P.x1 : P.x0;
-            xout[r] = o;
-        } else if (P.epi == EPI_FINAL) {
-            double2 yf = yprev[r];
-            double2 er = make_double2(0.0, 0.0);
-            for (int j = 0; j < P.nterms; ++j) {
-                const double fb = dt * P.coef[j], fe = dt * P.ecoef[j];
-                if (fb != 0.0 || fe != 0.0) {
-                    const double2 kj = j == P.stage ? acc : P.K[(int64_t)j * P.npad + r];
-                    if (fb != 0.0) raxpy(yf, fb, kj);
-                    if (fe != 0.0) raxpy(er, fe, kj);
+        const double2 acc = sell_row_dot<L>(P.slice_ptr, P.col, P.vals, xin, slice, lane);
+        if (lane % L == 0) {
+            P.K[(int64_t)P.stage * P.npad + r] = acc;
+            if (P.epi == EPI_NEXT) {
+                double2 o = yprev[r];
+                for (int j = 0; j < P.nterms; ++j) {
+                    const double f = dt * P.coef[j];
+                    if (f != 0.0) raxpy(o, f, j == P.stage ? acc : P.K[(int64_t)j * P.npad + r]);
                 }
-            }
-            yfront[r] = yf;
-            if (r < P.nrows) {
-                const double q = sqrt(cabs2(er)) / (ctl->atol + ctl->rtol * sqrt(cabs2(yf)));
-                part = q * q;
+                double2 *xout = ((P.stage + 1) & 1) ? P.x1 : P.x0;
+                xout[r] = o;
+            } else if (P.epi == EPI_FINAL) {
+                double2 yf = yprev[r];
+                double2 er = make_double2(0.0, 0.0);
+                for (int j = 0; j < P.nterms; ++j) {
+                    const double fb = dt * P.coef[j], fe = dt * P.ecoef[j];
+                    if (fb != 0.0 || fe != 0.0) {
+                        const double2 kj = j == P.stage ? acc : P.K[(int64_t)j * P.npad + r];
+                        if (fb != 0.0) raxpy(yf, fb, kj);
+                        if (fe != 0.0) raxpy(er, fe, kj);
+                    }
+                }
+                yfront[r] = yf;
+                if (r < P.nrows) {
+                    const double q = sqrt(cabs2(er)) / (ctl->atol + ctl->rtol * sqrt(cabs2(yf)));
+                    part = q * q;
+                }
             }
         }
     }
@@ -249,11 +252,9 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
         __shared__ double smem[kStepBlocks / QA_WAVE];
         const double s = block_sum(part, smem);
         if (threadIdx.x == 0) P.partials[blockIdx.x] = s;
-        if (last_block_done(&ctl->counter) && threadIdx.x == 0) {
-            double tot = 0.0;
-            for (unsigned b = 0; b < gridDim.x; ++b) tot += P.partials[b];
-            const double err = ctl->adaptive ? sqrt(tot / (double)P.nrows) : 0.0;
-            after_attempt(ctl, err);
+        if (last_block_done(&ctl->counter)) {
+            const double tot = sum_partials(P.partials, gridDim.x, smem);
+            if (threadIdx.x == 0) after_attempt(ctl, ctl->adaptive ? sqrt(tot / (double)P.nrows) : 0.0);
         }
     }
 }
@@ -465,7 +466,7 @@ __global__ void rk_set_phase_kernel(RKCtl *c, int phase) { c->phase = phase; }
 // host-side engine object
 // ---------------------------------------------------------------------------
 struct qa_rk_solver {
-    int n = 0, nslices = 0, vals_fp32 = 0;
+    int n = 0, nslices = 0, vals_fp32 = 0, lanes = 1;
     int64_t npad = 0;
     const int *slice_ptr = nullptr;
     const int *col = nullptr;
@@ -553,11 +554,24 @@ StageArgs<V> stage_args(qa_rk_solver *S, int i) {
     return P;
 }
 
+int stage_blocks(const qa_rk_solver *S) {
+    const int64_t threads = (int64_t)S->nslices * QA_WAVE;
+    return (int)((threads + kStepBlocks - 1) / kStepBlocks);
+}
+
 template <typename V>
 int launch_stage(qa_rk_solver *S, int i) {
     StageArgs<V> P = stage_args<V>(S, i);
-    const unsigned blocks = (unsigned)(S->npad / kStepBlocks + (S->npad % kStepBlocks ? 1 : 0));
-    hipLaunchKernelGGL(rk_stage_kernel<V>, dim3(blocks), dim3(kStepBlocks), 0, S->stream, P);
+    const dim3 grid((unsigned)stage_blocks(S)), block(kStepBlocks);
+    switch (S->lanes) {
+        case 1: hipLaunchKernelGGL((rk_stage_kernel<V, 1>), grid, block, 0, S->stream, P); break;
+        case 2: hipLaunchKernelGGL((rk_stage_kernel<V, 2>), grid, block, 0, S->stream, P); break;
+        case 4: hipLaunchKernelGGL((rk_stage_kernel<V, 4>), grid, block, 0, S->stream, P); break;
+        case 8: hipLaunchKernelGGL((rk_stage_kernel<V, 8>), grid, block, 0, S->stream, P); break;
+        default:
+            qa::set_error("lanes per row must be 1, 2, 4 or 8");
+            return QA_ERR_ARG;
+    }
     QA_CHECK_LAUNCH();
     return QA_OK;
 }
@@ -657,15 +671,18 @@ void free_all(qa_rk_solver *S) {
 
 extern "C" {
 
-int qa_rk_create(qa_rk_solver **out, int n, int nslices, const int *slice_ptr, const int *col,
-                 const void *vals, int vals_fp32, int order, int rk_step, int rk_extra,
+int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *slice_ptr,
+                 const int *col, const void *vals, int vals_fp32, int order, int rk_step,
+                 int rk_extra,
                  int dorder, const double *a, const double *b, const double *c,
                  const double *e, const double *bi, double rtol, double atol, int nsteps,
                  double first_step, double min_step, double max_step, int interpolate) {
     QA_REQUIRE(out != nullptr, "qa_rk_create: null output");
     *out = nullptr;
-    QA_REQUIRE(n > 0 && nslices == (n + QA_WAVE - 1) / QA_WAVE,
-               "qa_rk_create: need n > 0 and nslices = ceil(n / 64)");
+    QA_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8,
+               "qa_rk_create: lanes must be 1, 2, 4 or 8");
+    QA_REQUIRE(n > 0 && nslices == (n + QA_WAVE / lanes - 1) / (QA_WAVE / lanes),
+               "qa_rk_create: need n > 0 and nslices = ceil(n / (64 / lanes))");
     QA_REQUIRE(slice_ptr && col && vals, "qa_rk_create: null operator pointer");
     QA_REQUIRE(rk_step >= 1 && rk_step <= kMaxStages && rk_extra >= rk_step &&
                    rk_extra <= kMaxStages,
@@ -676,8 +693,9 @@ int qa_rk_create(qa_rk_solver **out, int n, int nslices, const int *slice_ptr, c
     qa_rk_solver *S = new (std::nothrow) qa_rk_solver();
     QA_REQUIRE(S != nullptr, "qa_rk_create: out of host memory");
     S->n = n;
+    S->lanes = lanes;
     S->nslices = nslices;
-    S->npad = (int64_t)nslices * QA_WAVE;
+    S->npad = (int64_t)nslices * (QA_WAVE / lanes);
     S->slice_ptr = slice_ptr;
     S->col = col;
     S->vals = vals;
@@ -721,7 +739,7 @@ int qa_rk_create(qa_rk_solver **out, int n, int nslices, const int *slice_ptr, c
     const int nk = S->interpolate ? rk_extra : rk_step;
     const int64_t nvec = 6 + nk;  // y0 y1 x0 x1 y_last tmp + K
     const size_t vbytes = (size_t)S->npad * sizeof(double2);
-    const size_t nblocks_step = (size_t)(S->npad / kStepBlocks + 1);
+    const size_t nblocks_step = (size_t)stage_blocks(S) + 1;
     size_t bytes = vbytes * nvec + nblocks_step * sizeof(double) + 256;
     he = hipMalloc(&S->block, bytes);
     if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMalloc(rk vectors)", __FILE__, __LINE__));
@@ -854,8 +872,8 @@ int qa_rk_set_state(qa_rk_solver *S, double t0, const void *y0_dev) {
         double norm = sqrt(norm2);
         const double tol = S->atol + norm * S->rtol;
         double2 *k0 = S->K, *k1 = S->K + S->npad;
-        rc = qa_sell_matvec_z(S->n, S->nslices, S->slice_ptr, S->col, S->vals, S->vals_fp32,
-                              S->y1, k0, 1.0, 0.0, 0, s);
+        rc = qa_sell_matvec_z(S->n, S->lanes, S->nslices, S->slice_ptr, S->col, S->vals,
+                              S->vals_fp32, S->y1, k0, 1.0, 0.0, 0, s);
         if (rc) return rc;
         if (norm <= S->atol) norm = 1.0;
         rc = qa_norm2_z(S->n, k0, &kn2, s);
@@ -872,8 +890,8 @@ int qa_rk_set_state(qa_rk_solver *S, double t0, const void *y0_dev) {
         const double cf[1] = {dt1 / 100};
         rc = qa_lincomb_z(S->npad, S->y1, 1, vecs, cf, S->tmp, s);
         if (rc) return rc;
-        rc = qa_sell_matvec_z(S->n, S->nslices, S->slice_ptr, S->col, S->vals, S->vals_fp32,
-                              S->tmp, k1, 1.0, 0.0, 0, s);
+        rc = qa_sell_matvec_z(S->n, S->lanes, S->nslices, S->slice_ptr, S->col, S->vals,
+                              S->vals_fp32, S->tmp, k1, 1.0, 0.0, 0, s);
         if (rc) return rc;
         rc = qa_norm2_z(S->n, k1, &kn2, s);
         if (rc) return rc;

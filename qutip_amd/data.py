@@ -221,30 +221,36 @@ class HipCSR:
     def __repr__(self):
         return f"HipCSR(shape={self.shape}, nnz={self.nnz}, fp32={self.fp32})"
 
-    @property
-    def sell(self):
-        if self._sell is None:
-            self._sell = SellMatrix(self._host, fp32=self.fp32)
+    def sell(self, lanes=1):
+        if self._sell is None or self._sell.lanes != lanes:
+            self._sell = SellMatrix(self._host, fp32=self.fp32, lanes=lanes)
         return self._sell
 
     def __matmul__(self, other):
         return matmul(self, other)
 
 
-def sell_arrays(A, slice_height=64):
-    """SELL-64 layout of a CSR matrix: per slice of 64 rows, width = longest
-    row; entry j of row 64 s + l at slice_ptr[s] + 64 j + l (column-major in
-    the slice).  Padding: value 0, column = the row's last column (or 0)."""
+def sell_arrays(A, lanes=1):
+    """SELL-C layout of a CSR matrix with ``lanes`` (1, 2, 4, 8) lanes per row
+    and C = 64 / lanes rows per slice: per slice the width is the longest
+    row rounded up to a multiple of ``lanes``; entry j of row C s + r goes
+    to slice_ptr[s] + 64 (j // lanes) + lanes r + j % lanes, so a wavefront
+    reads 64 consecutive entries per load.  Padding entries have value 0 and
+    the column of their own row (clipped to the column count)."""
+    if lanes not in (1, 2, 4, 8):
+        raise ValueError("lanes must be 1, 2, 4 or 8")
     A = sp.csr_matrix(A)
     A.sort_indices()
     n = A.shape[0]
-    ns = (n + slice_height - 1) // slice_height
-    npad = ns * slice_height
+    C = 64 // lanes
+    ns = (n + C - 1) // C
+    npad = ns * C
     rn = np.zeros(npad, dtype=np.int64)
     rn[:n] = np.diff(A.indptr)
-    width = rn.reshape(ns, slice_height).max(axis=1)
+    width = rn.reshape(ns, C).max(axis=1)
+    width = (width + lanes - 1) // lanes * lanes
     slice_ptr = np.zeros(ns + 1, dtype=np.int64)
-    slice_ptr[1:] = np.cumsum(width * slice_height)
+    slice_ptr[1:] = np.cumsum(width * C)
     total = int(slice_ptr[-1])
     if total >= 2 ** 31:
         raise ValueError("SELL storage exceeds int32 offsets")
@@ -252,7 +258,7 @@ def sell_arrays(A, slice_height=64):
     vals = np.zeros(max(total, 1), dtype=np.complex128)
     rows = np.repeat(np.arange(n), rn[:n])
     j = np.arange(A.nnz) - np.repeat(A.indptr[:-1], rn[:n])
-    dest = slice_ptr[rows // slice_height] + j * slice_height + rows % slice_height
+    dest = slice_ptr[rows // C] + (j // lanes) * 64 + (rows % C) * lanes + j % lanes
     cols[dest] = A.indices
     vals[dest] = A.data
     # padding columns: keep loads in range and near the row
@@ -261,16 +267,19 @@ def sell_arrays(A, slice_height=64):
     if pad_mask.any():
         pidx = np.nonzero(pad_mask)[0]
         s_of = np.searchsorted(slice_ptr, pidx, side="right") - 1
-        lane = (pidx - slice_ptr[s_of]) % slice_height
-        r = s_of * slice_height + lane
+        lane = (pidx - slice_ptr[s_of]) % 64
+        r = s_of * C + lane // lanes
         cols[pidx] = np.minimum(r, A.shape[1] - 1).astype(np.int32)
     return slice_ptr.astype(np.int32), cols, vals, npad
 
 
 class SellMatrix:
-    def __init__(self, A, fp32=False):
-        sptr, cols, vals, npad = sell_arrays(A)
+    """A SELL-C copy of a CSR matrix in HBM (see sell_arrays)."""
+
+    def __init__(self, A, fp32=False, lanes=1):
+        sptr, cols, vals, npad = sell_arrays(A, lanes)
         self.shape = A.shape
+        self.lanes = lanes
         self.nslices = len(sptr) - 1
         self.npad = npad
         self.stored = int(sptr[-1])
@@ -282,6 +291,13 @@ class SellMatrix:
     @property
     def ptrs(self):
         return self._sptr.ptr, self._cols.ptr, self._vals.ptr
+
+    def matvec(self, x, out, scale=1.0, accumulate=False):
+        sre, sim = _scalar(scale)
+        _lib.call("qa_sell_matvec_z", self.shape[0], self.lanes, self.nslices,
+                  _vp(self._sptr.ptr), _vp(self._cols.ptr), _vp(self._vals.ptr), int(self.fp32),
+                  _vp(x.ptr), _vp(out.ptr), sre, sim, int(accumulate), None)
+        return out
 
 
 # ---------------------------------------------------------------------------

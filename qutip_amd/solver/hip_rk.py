@@ -22,7 +22,7 @@ import ctypes as C
 import numpy as np
 import scipy.sparse as sp
 
-from .. import _lib
+from .. import _lib, autotune
 from .. import data as _data
 from .integrator import Integrator, IntegratorException
 from .tableaus import TABLEAUS
@@ -56,7 +56,7 @@ class RKEngine:
     """Owner of one ``qa_rk_solver`` (fused device RK for dy/dt = A y)."""
 
     def __init__(self, op, tableau="vern7", rtol=1e-6, atol=1e-8, nsteps=1000, first_step=0.0,
-                 min_step=0.0, max_step=0.0, interpolate=True, fp32=False):
+                 min_step=0.0, max_step=0.0, interpolate=True, fp32=False, lanes=None):
         if isinstance(op, _data.HipCSR):
             A = op.to_scipy()
         else:
@@ -66,7 +66,10 @@ class RKEngine:
         tab = TABLEAUS[tableau] if isinstance(tableau, str) else tableau
         self.tableau = tab
         self.n = A.shape[0]
-        self.sell = _data.SellMatrix(A, fp32=fp32)
+        if lanes is None:
+            lanes = autotune.lanes_for(A, fp32)
+        self.lanes = lanes
+        self.sell = _data.SellMatrix(A, fp32=fp32, lanes=lanes)
         self._a = np.ascontiguousarray(tab["a"], dtype=np.float64)
         self._b = np.ascontiguousarray(tab["b"], dtype=np.float64)
         self._c = np.ascontiguousarray(tab["c"], dtype=np.float64)
@@ -76,7 +79,7 @@ class RKEngine:
         h = C.c_void_p()
         sptr, cols, vals = self.sell.ptrs
         rc = L.qa_rk_create(
-            C.byref(h), self.n, self.sell.nslices, C.c_void_p(sptr), C.c_void_p(cols),
+            C.byref(h), self.n, lanes, self.sell.nslices, C.c_void_p(sptr), C.c_void_p(cols),
             C.c_void_p(vals), int(fp32), int(tab["order"]), len(self._b), len(self._c),
             self._bi.shape[1] if self._bi is not None else 0,
             _dp(self._a), _dp(self._b), _dp(self._c),

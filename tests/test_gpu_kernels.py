@@ -56,17 +56,14 @@ def test_csr_matmat_columns():
     assert_allclose(got, A @ X, atol=ATOL, rtol=RTOL)
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8])
 @pytest.mark.parametrize("n", [1, 64, 65, 1000, 65536])
-def test_sell_matvec(n):
-    print(f"sell_matvec n={n}", flush=True)
+def test_sell_matvec(n, lanes):
+    print(f"sell_matvec n={n} lanes={lanes}", flush=True)
     A = rand_csr(n, n, min(1.0, 12.0 / n), n)
     x = rand_vec(n, 2)
-    S = qd.SellMatrix(A)
-    xd = qd.HipDense.from_array(x)
-    out = qd.HipDense((n, 1))
-    sptr, cols, vals = S.ptrs
-    _lib.call("qa_sell_matvec_z", n, S.nslices, C.c_void_p(sptr), C.c_void_p(cols),
-              C.c_void_p(vals), 0, C.c_void_p(xd.ptr), C.c_void_p(out.ptr), 1.0, 0.0, 0, None)
+    S = qd.SellMatrix(A, lanes=lanes)
+    out = S.matvec(qd.HipDense.from_array(x), qd.HipDense((n, 1)))
     assert_allclose(out.to_array(), A @ x, atol=ATOL, rtol=RTOL)
 
 

@@ -12,27 +12,32 @@ from oracle import models
 from tests import systems
 
 
-def _sell_matvec(sptr, cols, vals, n, x):
-    """numpy emulation of the SELL-64 kernel's row dot products."""
+def _sell_matvec(sptr, cols, vals, n, x, lanes):
+    """numpy emulation of the SELL-C kernel: lane l = lanes r + q of slice s
+    sums entries q, q + lanes, ... of row C s + r."""
     ns = len(sptr) - 1
-    out = np.zeros(ns * 64, dtype=np.complex128)
+    C = 64 // lanes
+    out = np.zeros(ns * C, dtype=np.complex128)
     for s in range(ns):
-        w = (sptr[s + 1] - sptr[s]) // 64
-        blk_c = cols[sptr[s]:sptr[s + 1]].reshape(w, 64)
-        blk_v = vals[sptr[s]:sptr[s + 1]].reshape(w, 64)
-        out[s * 64:(s + 1) * 64] = (blk_v * x[blk_c]).sum(axis=0)
+        steps = (sptr[s + 1] - sptr[s]) // 64
+        blk_c = cols[sptr[s]:sptr[s + 1]].reshape(steps, 64)
+        blk_v = vals[sptr[s]:sptr[s + 1]].reshape(steps, 64)
+        lane_sum = (blk_v * x[blk_c]).sum(axis=0)
+        out[s * C:(s + 1) * C] = lane_sum.reshape(C, lanes).sum(axis=1)
     return out[:n]
 
 
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8])
 @pytest.mark.parametrize("n,density", [(1, 1.0), (63, 0.1), (64, 0.05), (200, 0.03), (130, 0.0)])
-def test_sell_layout_reproduces_csr(n, density):
+def test_sell_layout_reproduces_csr(n, density, lanes):
     rng = np.random.default_rng(n)
     A = sp.random(n, n, density=density, random_state=rng, format="csr") * (1 + 0.5j)
-    sptr, cols, vals, npad = qdata.sell_arrays(A)
-    assert npad % 64 == 0 and npad >= n
+    sptr, cols, vals, npad = qdata.sell_arrays(A, lanes)
+    assert npad % (64 // lanes) == 0 and npad >= n
     assert cols.min() >= 0 and cols.max() < n
+    assert sptr[-1] % 64 == 0
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
-    assert_allclose(_sell_matvec(sptr, cols, vals, n, x), A @ x, atol=1e-13)
+    assert_allclose(_sell_matvec(sptr, cols, vals, n, x, lanes), A @ x, atol=1e-13)
 
 
 def test_sell_padding_overhead_on_ising():
