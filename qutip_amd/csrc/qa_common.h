@@ -104,21 +104,55 @@ __device__ __forceinline__ double2 sell_row_dot(const int *__restrict__ slice_pt
                                                 const int *__restrict__ col,
                                                 const V *__restrict__ vals,
                                                 const double2 *__restrict__ x, int slice, int lane) {
-    const int beg = slice_ptr[slice] + lane, end = slice_ptr[slice + 1];
+    // Every lane of a slice makes the same number of steps (the slice width
+    // divided by L), so the loop below is not divergent.  Eight entries are
+    // loaded before any is used: their column loads, value loads and x
+    // gathers overlap instead of forming a chain of dependent memory round
+    // trips.
+    const int base = slice_ptr[slice];
+    const int steps = (slice_ptr[slice + 1] - base) / QA_WAVE;
+    const int p0 = base + lane;
     double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
-    int p = beg;
-    for (; p + 3 * QA_WAVE < end; p += 4 * QA_WAVE) {
-        const int c0 = col[p], c1 = col[p + QA_WAVE], c2 = col[p + 2 * QA_WAVE],
-                  c3 = col[p + 3 * QA_WAVE];
-        const double2 v0 = ld(vals, p), v1 = ld(vals, p + QA_WAVE), v2 = ld(vals, p + 2 * QA_WAVE),
-                      v3 = ld(vals, p + 3 * QA_WAVE);
-        const double2 x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    int k = 0;
+    for (; k + 8 <= steps; k += 8) {
+        int c[8];
+        double2 v[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = col[p0 + (k + u) * QA_WAVE];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = ld(vals, p0 + (k + u) * QA_WAVE);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+            cfma(a0, v[u], xv[u]);
+            cfma(a1, v[u + 1], xv[u + 1]);
+        }
+    }
+    if (k + 4 <= steps) {
+        int c[4];
+        double2 v[4], xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = col[p0 + (k + u) * QA_WAVE];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = ld(vals, p0 + (k + u) * QA_WAVE);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = x[c[u]];
+        cfma(a0, v[0], xv[0]);
+        cfma(a1, v[1], xv[1]);
+        cfma(a0, v[2], xv[2]);
+        cfma(a1, v[3], xv[3]);
+        k += 4;
+    }
+    if (k + 2 <= steps) {
+        const int c0 = col[p0 + k * QA_WAVE], c1 = col[p0 + (k + 1) * QA_WAVE];
+        const double2 v0 = ld(vals, p0 + k * QA_WAVE), v1 = ld(vals, p0 + (k + 1) * QA_WAVE);
+        const double2 x0 = x[c0], x1 = x[c1];
         cfma(a0, v0, x0);
         cfma(a1, v1, x1);
-        cfma(a0, v2, x2);
-        cfma(a1, v3, x3);
+        k += 2;
     }
-    for (; p < end; p += QA_WAVE) cfma(a0, ld(vals, p), x[col[p]]);
+    if (k < steps) cfma(a0, ld(vals, p0 + k * QA_WAVE), x[col[p0 + k * QA_WAVE]]);
     double2 acc = cadd(a0, a1);
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) {

@@ -21,6 +21,7 @@
 // Device state lives in an RKCtl block in HBM.  Vectors are complex128,
 // padded to a multiple of 64 rows.  Operator values are complex128 or
 // complex64 (widened to FP64 in registers; accumulation is always FP64).
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -198,6 +199,7 @@ struct StageArgs {
     double *partials;
     RKCtl *ctl;
     int stage, run_phase, input, epi, nterms;
+    double coef_self, ecoef_self;  // coef[stage], ecoef[stage]
     double coef[kMaxStages];
     double ecoef[kMaxStages];
 };
@@ -218,31 +220,40 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
     const int64_t r = (int64_t)slice * (QA_WAVE / L) + lane / L;  // the row of this lane
     double part = 0.0;
     if (slice < P.nslices) {
-        const double2 acc = sell_row_dot<L>(P.slice_ptr, P.col, P.vals, xin, slice, lane);
-        if (lane % L == 0) {
-            P.K[(int64_t)P.stage * P.npad + r] = acc;
-            if (P.epi == EPI_NEXT) {
-                double2 o = yprev[r];
-                for (int j = 0; j < P.nterms; ++j) {
-                    const double f = dt * P.coef[j];
-                    if (f != 0.0) raxpy(o, f, j == P.stage ? acc : P.K[(int64_t)j * P.npad + r]);
-                }
-                double2 *xout = ((P.stage + 1) & 1) ? P.x1 : P.x0;
-                xout[r] = o;
-            } else if (P.epi == EPI_FINAL) {
-                double2 yf = yprev[r];
-                double2 er = make_double2(0.0, 0.0);
-                for (int j = 0; j < P.nterms; ++j) {
-                    const double fb = dt * P.coef[j], fe = dt * P.ecoef[j];
+        // Epilogue terms that do not depend on this stage's product are formed
+        // first, so their loads are in flight with the product's.
+        const bool owner = (lane % L == 0);
+        double2 o = make_double2(0.0, 0.0), er = make_double2(0.0, 0.0);
+        if (owner && P.epi != EPI_NONE) {
+            o = yprev[r];
+#pragma unroll
+            for (int j = 0; j < kMaxStages; ++j) {
+                if (j < P.stage) {
+                    const double fb = dt * P.coef[j];
+                    const double fe = dt * P.ecoef[j];
                     if (fb != 0.0 || fe != 0.0) {
-                        const double2 kj = j == P.stage ? acc : P.K[(int64_t)j * P.npad + r];
-                        if (fb != 0.0) raxpy(yf, fb, kj);
+                        const double2 kj = P.K[(int64_t)j * P.npad + r];
+                        if (fb != 0.0) raxpy(o, fb, kj);
                         if (fe != 0.0) raxpy(er, fe, kj);
                     }
                 }
-                yfront[r] = yf;
+            }
+        }
+        const double2 acc = sell_row_dot<L>(P.slice_ptr, P.col, P.vals, xin, slice, lane);
+        if (owner) {
+            P.K[(int64_t)P.stage * P.npad + r] = acc;
+            const double fb = dt * P.coef_self;
+            const double fe = dt * P.ecoef_self;
+            if (P.epi == EPI_NEXT) {
+                if (fb != 0.0) raxpy(o, fb, acc);
+                double2 *xout = ((P.stage + 1) & 1) ? P.x1 : P.x0;
+                xout[r] = o;
+            } else if (P.epi == EPI_FINAL) {
+                if (fb != 0.0) raxpy(o, fb, acc);
+                if (fe != 0.0) raxpy(er, fe, acc);
+                yfront[r] = o;
                 if (r < P.nrows) {
-                    const double q = sqrt(cabs2(er)) / (ctl->atol + ctl->rtol * sqrt(cabs2(yf)));
+                    const double q = sqrt(cabs2(er)) / (ctl->atol + ctl->rtol * sqrt(cabs2(o)));
                     part = q * q;
                 }
             }
@@ -252,11 +263,20 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
         __shared__ double smem[kStepBlocks / QA_WAVE];
         const double s = block_sum(part, smem);
         if (threadIdx.x == 0) P.partials[blockIdx.x] = s;
-        if (last_block_done(&ctl->counter)) {
-            const double tot = sum_partials(P.partials, gridDim.x, smem);
-            if (threadIdx.x == 0) after_attempt(ctl, ctl->adaptive ? sqrt(tot / (double)P.nrows) : 0.0);
-        }
     }
+}
+
+// Error norm and step control after the last stage: one workgroup sums the
+// per-workgroup partials of the last stage kernel in a fixed order and runs
+// the controller.  A kernel of its own (rather than a last-workgroup
+// handshake inside the stage kernel) because a device-scope fence and atomic
+// per workgroup costs more than this launch.
+__global__ __launch_bounds__(256) void rk_control_kernel(RKCtl *ctl, const double *partials,
+                                                         int nparts, int nrows) {
+    if (ctl->phase != PH_STEP) return;
+    __shared__ double smem[256 / QA_WAVE];
+    const double tot = sum_partials(partials, nparts, smem);
+    if (threadIdx.x == 0) after_attempt(ctl, ctl->adaptive ? sqrt(tot / (double)nrows) : 0.0);
 }
 
 struct LinCoef {
@@ -497,6 +517,7 @@ struct qa_rk_solver {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_dirty = true;
+    bool use_graph = true;  // QUTIP_AMD_GRAPH=0: launch the iteration directly
     RKCtl *ctl_host = nullptr;  // pinned mirror
     long long graph_launches = 0;
     long long setup_rhs = 0;  // derivative evaluations of the first-step estimate
@@ -562,6 +583,8 @@ int stage_blocks(const qa_rk_solver *S) {
 template <typename V>
 int launch_stage(qa_rk_solver *S, int i) {
     StageArgs<V> P = stage_args<V>(S, i);
+    P.coef_self = P.coef[i];
+    P.ecoef_self = P.ecoef[i];
     const dim3 grid((unsigned)stage_blocks(S)), block(kStepBlocks);
     switch (S->lanes) {
         case 1: hipLaunchKernelGGL((rk_stage_kernel<V, 1>), grid, block, 0, S->stream, P); break;
@@ -605,6 +628,9 @@ int enqueue_iteration(qa_rk_solver *S) {
         rc = S->vals_fp32 ? launch_stage<float2>(S, i) : launch_stage<double2>(S, i);
         if (rc) return rc;
     }
+    hipLaunchKernelGGL(rk_control_kernel, dim3(1), dim3(256), 0, S->stream, S->ctl, S->partials,
+                       stage_blocks(S), S->n);
+    QA_CHECK_LAUNCH();
     if (S->interpolate && S->rk_extra > S->rk_step) {
         if (!S->first_dense_from_front) {
             LinCoef lc;
@@ -710,6 +736,10 @@ int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *s
     S->adaptive = e != nullptr;
     if (e) S->e.assign(e, e + rk_step);
     S->interpolate = (bi != nullptr) && interpolate;
+    {
+        const char *g = getenv("QUTIP_AMD_GRAPH");
+        S->use_graph = !(g && g[0] == '0');
+    }
     if (bi) S->bi.assign(bi, bi + (size_t)rk_extra * dorder);
     S->rtol = rtol;
     S->atol = atol;
@@ -950,7 +980,7 @@ int qa_rk_run(qa_rk_solver *S, int ntimes, const double *times, int single_step,
                           sizeof(int), hipMemcpyHostToDevice, s));
     // the sources above are pageable host memory: complete the copies here
     QA_HIP(hipStreamSynchronize(s));
-    if (S->graph_dirty) {
+    if (S->use_graph && S->graph_dirty) {
         int rc = rebuild_graph(S);
         if (rc) return rc;
     }
@@ -964,7 +994,14 @@ int qa_rk_run(qa_rk_solver *S, int ntimes, const double *times, int single_step,
     long long iters = 0;
     long long last_attempts = -1, last_outputs = -1;
     while (true) {
-        for (int k = 0; k < chunk; ++k) QA_HIP(hipGraphLaunch(S->exec, s));
+        for (int k = 0; k < chunk; ++k) {
+            if (S->use_graph) {
+                QA_HIP(hipGraphLaunch(S->exec, s));
+            } else {
+                int rc = enqueue_iteration(S);
+                if (rc) return rc;
+            }
+        }
         iters += chunk;
         S->graph_launches += chunk;
         QA_HIP(hipMemcpyAsync(S->ctl_host, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToHost, s));
