@@ -302,54 +302,74 @@ __global__ __launch_bounds__(256) void rk_combine_kernel(RKCtl *ctl, int64_t npa
     }
 }
 
+constexpr int kMaxDenseOrder = 8;
+
 struct OutArgs {
     RKCtl *ctl;
     double2 *y0, *y1;
     const double2 *K;
     int64_t npad;
-    int nk, dorder;
-    const double *bi;  // [nk][dorder]
+    int nk, dorder, has_bi;
     int nfun;
     const int *fptr, *fkind, *frow, *fcol;
     const double2 *fw;
     double2 *partials;  // [kMaxPending][nfun][nchunks]
     int nchunks;
     double2 *y_last;
+    double bi[kMaxStages * kMaxDenseOrder];  // dense-output table, [i][j] at i * dorder + j
 };
 
-// Interpolation weights of the pending outputs: w[q][i] = dt_int * b_factor_i
-// (explicit_rk.pyx:412-430); front[q] = 1 when the output time equals t_front.
+// Number of pending outputs (output times inside [t_prev, t_front]) and
+// their interpolation weights w[q][i] = dt_int * b_factor_i
+// (explicit_rk.pyx:412-430), computed by the whole workgroup into LDS;
+// front[q] = 1 when the output time equals t_front (the state is y_front).
 __device__ int pending_outputs(const RKCtl *c, const OutArgs &A, double (*w)[kMaxStages],
-                               int *front) {
-    int nq = 0;
-    while (nq < kMaxPending && c->out_idx + nq < c->ntimes &&
-           c->times[c->out_idx + nq] <= c->t_front)
-        ++nq;
-    for (int q = 0; q < nq; ++q) {
-        const double t = c->times[c->out_idx + q];
-        front[q] = (t == c->t_front);
-        for (int i = 0; i < kMaxStages; ++i) w[q][i] = 0.0;
-        if (front[q] || A.bi == nullptr) continue;
-        const double tau = (t - c->t_prev) / c->dt_int;
-        for (int i = 0; i < A.nk; ++i) {
-            double v = 0.0;
-            for (int j = A.dorder - 1; j >= 0; --j) {
-                v += A.bi[i * A.dorder + j];
-                v *= tau;
-            }
-            w[q][i] = c->dt_int * v;
-        }
+                               int *front, int *nq_s) {
+    if (threadIdx.x == 0) {
+        int nq = 0;
+        while (nq < kMaxPending && c->out_idx + nq < c->ntimes &&
+               c->times[c->out_idx + nq] <= c->t_front)
+            ++nq;
+        *nq_s = nq;
     }
+    __syncthreads();
+    const int nq = *nq_s;
+    const int t = threadIdx.x;
+    if (t < kMaxPending * kMaxStages) {
+        const int q = t / kMaxStages, i = t % kMaxStages;
+        double v = 0.0;
+        if (q < nq) {
+            const double tq = c->times[c->out_idx + q];
+            if (i == 0) front[q] = (tq == c->t_front);
+            if (tq != c->t_front && A.has_bi && i < A.nk) {
+                const double tau = (tq - c->t_prev) / c->dt_int;
+                for (int j = A.dorder - 1; j >= 0; --j) {
+                    v += A.bi[i * A.dorder + j];
+                    v *= tau;
+                }
+                v *= c->dt_int;
+            }
+        }
+        w[q][i] = v;
+    }
+    __syncthreads();
     return nq;
 }
 
+// y(t_q)[idx]: y_front, or y_prev + sum_i w_i k_i[idx] with every load in
+// flight at once (the table is fully unrolled).
 __device__ __forceinline__ double2 state_at(const OutArgs &A, const double2 *yprev,
                                             const double2 *yfront, const double *w, int front,
                                             int64_t idx) {
     if (front) return yfront[idx];
     double2 o = yprev[idx];
-    for (int i = 0; i < A.nk; ++i)
-        if (w[i] != 0.0) raxpy(o, w[i], A.K[(int64_t)i * A.npad + idx]);
+    double2 kv[kMaxStages];
+#pragma unroll
+    for (int i = 0; i < kMaxStages; ++i)
+        kv[i] = (i < A.nk && w[i] != 0.0) ? A.K[(int64_t)i * A.npad + idx] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < kMaxStages; ++i)
+        if (i < A.nk && w[i] != 0.0) raxpy(o, w[i], kv[i]);
     return o;
 }
 
@@ -359,14 +379,15 @@ __global__ __launch_bounds__(256) void rk_store_kernel(OutArgs A) {
     RKCtl *c = A.ctl;
     const int ph = c->phase;
     if (ph != PH_DENSE && ph != PH_OUTPUT) return;
+    // only the final requested output (get_state) or stored states need it
+    if (!c->states_out) {
+        const int last = c->ntimes - 1;
+        if (last - c->out_idx >= kMaxPending || c->times[last] > c->t_front) return;
+    }
     __shared__ double w[kMaxPending][kMaxStages];
     __shared__ int front[kMaxPending];
     __shared__ int nq_s;
-    if (threadIdx.x == 0) nq_s = pending_outputs(c, A, w, front);
-    __syncthreads();
-    const int nq = nq_s;
-    const bool has_last = c->out_idx + nq == c->ntimes;
-    if (!c->states_out && !has_last) return;
+    const int nq = pending_outputs(c, A, w, front, &nq_s);
     const double2 *yprev = c->cur ? A.y1 : A.y0;
     const double2 *yfront = c->cur ? A.y0 : A.y1;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < A.npad;
@@ -394,9 +415,7 @@ __global__ __launch_bounds__(256) void rk_output_kernel(OutArgs A) {
     __shared__ int front[kMaxPending];
     __shared__ int nq_s;
     __shared__ double smem[256 / QA_WAVE];
-    if (threadIdx.x == 0) nq_s = pending_outputs(c, A, w, front);
-    __syncthreads();
-    const int nq = nq_s;
+    const int nq = pending_outputs(c, A, w, front, &nq_s);
     const double2 *yprev = c->cur ? A.y1 : A.y0;
     const double2 *yfront = c->cur ? A.y0 : A.y1;
     const int f = blockIdx.y;
@@ -608,7 +627,9 @@ OutArgs out_args(qa_rk_solver *S) {
     A.npad = S->npad;
     A.nk = S->interpolate ? S->rk_extra : S->rk_step;
     A.dorder = S->dorder;
-    A.bi = S->bi_dev;
+    A.has_bi = S->interpolate ? 1 : 0;
+    for (int k = 0; k < kMaxStages * kMaxDenseOrder; ++k)
+        A.bi[k] = (S->interpolate && k < (int)S->bi.size()) ? S->bi[k] : 0.0;
     A.nfun = S->nfun;
     A.fptr = S->fptr_dev;
     A.fkind = S->fkind_dev;
@@ -714,7 +735,8 @@ int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *s
                    rk_extra <= kMaxStages,
                "qa_rk_create: 1 <= rk_step <= rk_extra <= %d", kMaxStages);
     QA_REQUIRE(a && b && c, "qa_rk_create: tableau a, b, c required");
-    QA_REQUIRE(!bi || (dorder >= 1 && dorder <= 16), "qa_rk_create: bad dense-output order");
+    QA_REQUIRE(!bi || (dorder >= 1 && dorder <= kMaxDenseOrder),
+               "qa_rk_create: dense-output order must be 1..%d", kMaxDenseOrder);
     QA_REQUIRE(nsteps >= 0, "qa_rk_create: nsteps must be >= 0");
     qa_rk_solver *S = new (std::nothrow) qa_rk_solver();
     QA_REQUIRE(S != nullptr, "qa_rk_create: out of host memory");
