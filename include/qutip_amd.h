@@ -197,6 +197,46 @@ int qa_rk_profile_stages(qa_rk_solver *s, int reps, double *ms_per_stage);
 /* Enqueue `count` loop iterations without the graph and wait (profiling). */
 int qa_rk_iterate(qa_rk_solver *s, int count);
 
+/* ---- batched Monte-Carlo trajectories (mcsolve) --------------------------
+ * Replaces the per-trajectory loop of MultiTrajSolver.run / map_func
+ * (qutip/solver/multitraj.py:169-258, parallel.py:267-337) around
+ * MCIntegrator (qutip/solver/mcsolve.py:228-419): all trajectories run in one
+ * persistent kernel, one workgroup per trajectory at a time, each with its
+ * own step sizes, collapse search and jumps. */
+
+/* Size in bytes of the packed tableau the trajectory kernel reads. */
+int qa_mc_tableau_size(void);
+
+/* Pack a tableau (host arrays as for qa_rk_create) into `out` (host buffer
+ * of qa_mc_tableau_size() bytes); the caller copies it to the device. */
+int qa_mc_pack_tableau(void *out, int order, int rk_step, int rk_extra, int dorder,
+                       const double *a, const double *b, const double *c, const double *e,
+                       const double *bi, int interpolate);
+
+/* Run ntraj trajectories of d psi/dt = H_eff psi (H_eff = -iH - 1/2 sum
+ * c^dag c, CSR d x d) from psi0 (device, d complex128) over times (device,
+ * ntimes).  c_ops / e_ops: nchan / neops CSR operators concatenated, with
+ * absolute row pointers (one block of d + 1 per operator).  Operator values
+ * are complex64 when fp32 != 0 (then the state and RHS are complex64 too;
+ * norms and expectation values stay FP64).  draws (device, ntraj x
+ * max_draws): the uniform numbers each trajectory consumes in the reference
+ * order (mcsolve.py:276-279, 388, 405).  Outputs (device): expect_out
+ * ntraj x ntimes x neops complex <O> of the normalised state; col_times /
+ * col_which ntraj x max_jumps; info ntraj x 4 ints (status, jumps, draws
+ * used, RK attempts).  Status 0 is success; -1/-2/-3 the RK failures,
+ * -10 collapse time not found, -11 draws exhausted (re-run with more).
+ * grid <= 0 launches one workgroup per trajectory.  d <= 1024. */
+int qa_mc_run(int d, int ntraj, int ntimes, const double *times_dev, const void *psi0_dev,
+              const int *h_rowptr, const int *h_col, const void *h_vals, int nchan,
+              const int *c_rowptr, const int *c_col, const void *c_vals, int neops,
+              const int *e_rowptr, const int *e_col, const void *e_vals,
+              const void *tableau_dev, const double *draws_dev, int max_draws,
+              double rtol, double atol, int nsteps, double first_step, double min_step,
+              double max_step, int norm_steps, double norm_t_tol, double norm_tol,
+              double norm_min_step, double mc_corr_eps, int fp32, void *expect_out_dev,
+              double *col_times_dev, int *col_which_dev, int max_jumps, int *info_dev,
+              int grid, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,7 @@ from .core import (Qobj, QobjEvo, basis, ket2dm, destroy, create, num, qeye, sig
                    sigmaz, sigmam, sigmap, tensor, spre, spost, liouvillian,
                    lindblad_dissipator, expect, rand_herm, rand_dm)
 from .data import HipDense, HipCSR
-from .solver import (sesolve, mesolve, SESolver, MESolver, Solver, Result, Integrator,
-                     IntegratorException, IntegratorHipRK)
+from .solver import (sesolve, mesolve, mcsolve, SESolver, MESolver, MCSolver, McResult, Solver,
+                     Result, Integrator, IntegratorException, IntegratorHipRK)
 
 __version__ = "0.1.0"

@@ -67,6 +67,11 @@ SIGNATURES = {
     "qa_rk_stats": [vp, P_i64],
     "qa_rk_iterate": [vp, i32],
     "qa_rk_profile_stages": [vp, i32, P_f64],
+    "qa_mc_tableau_size": [],
+    "qa_mc_pack_tableau": [vp, i32, i32, i32, i32, P_f64, P_f64, P_f64, P_f64, P_f64, i32],
+    "qa_mc_run": [i32, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp,
+                  i32, f64, f64, i32, f64, f64, f64, i32, f64, f64, f64, f64, i32, vp, vp, vp,
+                  i32, vp, i32, vp],
 }
 _RESTYPES = {"qa_last_error": C.c_char_p, "qa_rk_stream": vp}
 

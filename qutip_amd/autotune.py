@@ -52,21 +52,25 @@ def heuristic_lanes(nnz_row):
     return L
 
 
-def measure(A, fp32=False, candidates=CANDIDATES, reps=5):
-    """Average stage-kernel time (ms) per candidate lane count."""
+def measure(A, fp32=False, candidates=CANDIDATES, reps=3, rounds=3):
+    """Average stage-kernel time (ms) per candidate lane count.  All candidate
+    engines are built first, one untimed round warms the device, then the
+    candidates are timed in interleaved rounds and the fastest round counts
+    (so clock ramps and cache state do not favour whichever runs last)."""
     import ctypes as C
 
     from . import _lib
     from .solver.hip_rk import RKEngine
-    out = {}
-    for L in candidates:
-        eng = RKEngine(A, "vern7", 1e-6, 1e-8, fp32=fp32, lanes=L)
-        ms = np.zeros(16)
-        _lib.call("qa_rk_profile_stages", eng.handle, reps,
-                  ms.ctypes.data_as(C.POINTER(C.c_double)))
-        out[L] = float(ms.mean())
-        del eng
-    return out
+    engines = {L: RKEngine(A, "vern7", 1e-6, 1e-8, fp32=fp32, lanes=L) for L in candidates}
+    best = {L: float("inf") for L in candidates}
+    ms = np.zeros(16)
+    for rnd in range(rounds + 1):
+        for L, eng in engines.items():
+            _lib.call("qa_rk_profile_stages", eng.handle, reps,
+                      ms.ctypes.data_as(C.POINTER(C.c_double)))
+            if rnd > 0:
+                best[L] = min(best[L], float(ms.mean()))
+    return best
 
 
 def lanes_for(A, fp32=False):

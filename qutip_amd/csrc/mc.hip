@@ -1,0 +1,860 @@
+// Batched Monte-Carlo wave-function trajectories (mcsolve) on gfx950.
+//
+// One workgroup runs one trajectory from the first to the last output time
+// (grid-stride over trajectories), with no host round trip: the explicit
+// Runge-Kutta integrator in its "mcstep" form, the norm-based collapse
+// search and the quantum jumps all run on the device.  Trajectories advance
+// independently with their own step sizes and jump counts; a workgroup that
+// finishes early simply picks up the next trajectory.
+//
+// Restated algorithms (qutip/solver/...):
+//   RK step, step control, dense output     integrator/explicit_rk.pyx:204-467
+//   integrate(t, step=True)  (mcstep)        explicit_rk.pyx:278-331
+//   MCIntegrator.set_state / integrate       mcsolve.py:245-302
+//   _find_collapse_time (log-ratio secant)   mcsolve.py:321-369
+//   _do_collapse (channel choice, jump)      mcsolve.py:371-406
+//   H_eff = -iH - 1/2 sum c^dag c            mcsolve.py:491-496 (built on host)
+// Random numbers: the uniform draws of numpy's Generator (PCG64 seeded by
+// SeedSequence(seed), multitraj.py:354-393) are produced on the host in the
+// order the reference consumes them (mcsolve.py:276-279, 388, 405) and
+// streamed in; a trajectory that uses up its draws stops with a status the
+// host acts on.
+//
+// Layout: each thread owns one or two rows of the state (d <= 1024); the RK
+// stage vectors k_0..k_9 stay in registers and k_10..k_15 in LDS for the
+// whole trajectory; the vector an RHS product reads is staged in LDS; operators
+// (H_eff, c_ops, e_ops) are CSR in HBM, shared by all trajectories and served
+// from L2.  Norms, error norms and expectation values accumulate in FP64;
+// with the complex64 variant the state, stages and RHS products are FP32.
+#include <math.h>
+#include <string.h>
+
+#include "qa_api_internal.h"
+#include "qa_common.h"
+
+using namespace qa;
+
+namespace {
+
+constexpr int kMcStages = 16;
+constexpr int kMcOrder = 8;
+constexpr int kMcMaxChan = 32;
+constexpr int kMcMaxEops = 8;
+constexpr int kMcThreads = 512;  // 2 waves per SIMD: 256 VGPRs per lane
+
+// trajectory status codes (beside the RK codes of explicit_rk.pxd:5-12)
+enum McStatus {
+    MC_OK = 0,
+    MC_TOO_MUCH_WORK = -1,
+    MC_DT_UNDERFLOW = -2,
+    MC_OUTSIDE_RANGE = -3,
+    MC_COLLAPSE_NOT_FOUND = -10,  // mcsolve.py:363-367 RuntimeError
+    MC_DRAWS_EXHAUSTED = -11,     // host re-runs with more pre-drawn numbers
+    MC_NO_PROGRESS = -12
+};
+enum RkStatus { RK_AT_FRONT = 2, RK_INTERPOLATED = 1, RK_NORMAL = 0 };
+
+struct McTableau {
+    double a[kMcStages][kMcStages];
+    double b[kMcStages];
+    double c[kMcStages];
+    double e[kMcStages];
+    double bi[kMcStages][kMcOrder];
+    int order, rk_step, rk_extra, dorder, interpolate, adaptive, fsal, pad;
+};
+
+struct McArgs {
+    int d, ntraj, ntimes, nchan, neops, max_draws, max_jumps, traj_offset;
+    const int *h_rowptr, *h_col;
+    const void *h_vals;
+    const int *c_rowptr, *c_col;  // nchan blocks of d + 1 row pointers (absolute)
+    const void *c_vals;
+    const int *e_rowptr, *e_col;
+    const void *e_vals;
+    const McTableau *tab;
+    const double *times;
+    const double2 *psi0;
+    const double *draws;  // [ntraj][max_draws]
+    double rtol, atol, first_step, min_step, max_step;
+    int nsteps, norm_steps;
+    double norm_t_tol, norm_tol, norm_min_step, mc_corr_eps;
+    double2 *expect_out;  // [ntraj][ntimes][neops]
+    double *col_times;    // [ntraj][max_jumps]
+    int *col_which;
+    int *info;  // [ntraj][4]: status, jumps, draws used, RK attempts
+};
+
+template <typename R>
+struct cplx_t;
+template <>
+struct cplx_t<double> {
+    using type = double2;
+};
+template <>
+struct cplx_t<float> {
+    using type = float2;
+};
+
+template <typename R>
+using C2 = typename cplx_t<R>::type;
+
+template <typename R>
+__device__ __forceinline__ C2<R> mk(R x, R y) {
+    C2<R> v;
+    v.x = x;
+    v.y = y;
+    return v;
+}
+template <typename R>
+__device__ __forceinline__ void cfmaR(C2<R> &acc, C2<R> a, C2<R> b) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+}
+template <typename R>
+__device__ __forceinline__ void raxpyR(C2<R> &acc, R s, C2<R> v) {
+    acc.x = fma(s, v.x, acc.x);
+    acc.y = fma(s, v.y, acc.y);
+}
+template <typename R>
+__device__ __forceinline__ double nrm2(C2<R> v) {
+    return (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+}
+
+constexpr int kRegStages = 10;  // k_0..k_9 in registers, the rest in LDS
+
+// Per-workgroup LDS (dynamic): the vector an RHS product reads, the stage
+// vectors k_10.. (the dense-output stages of vern7) and the FSAL slot, one
+// entry per row, plus reduction scratch.
+template <typename R>
+struct McLds {
+    C2<R> *x;      // [npad]
+    C2<R> *kd;     // [kMcStages - kRegStages][npad]
+    C2<R> *kfsal;  // [npad]
+    double *red;   // [threads / 64]
+    double *probs; // [kMcMaxChan]
+    int npad;
+};
+
+template <typename R>
+inline size_t mc_lds_bytes(int npad, int threads) {
+    return sizeof(C2<R>) * (size_t)npad * (2 + kMcStages - kRegStages) +
+           sizeof(double) * ((threads / 64) + kMcMaxChan) + 64;
+}
+
+// A per-row value: each thread owns NR rows (r = threadIdx.x + q * blockDim.x).
+template <typename R, int NR>
+struct V {
+    C2<R> v[NR];
+};
+
+template <typename R, int NR>
+__device__ __forceinline__ V<R, NR> vzero() {
+    V<R, NR> o;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) o.v[q] = mk<R>(0, 0);
+    return o;
+}
+
+// Per-trajectory integrator state.  Vectors are per thread; scalars are
+// uniform (every thread holds the same value).
+template <typename R, int NR>
+struct Traj {
+    V<R, NR> k[kRegStages];
+    V<R, NR> yprev, yfront, y;
+    double t, t_prev, t_front, dt_safe, dt_int;
+    int status;    // trajectory status (McStatus)
+    int rk_state;  // last RK status (INTERPOLATED marks a prepared dense output)
+    long long attempts;
+};
+
+template <typename R, int NR>
+struct Ctx {
+    const McArgs &A;
+    McLds<R> &S;
+    const McTableau &T;
+    int row[NR];
+    bool act[NR];
+    int h_beg[NR], h_end[NR];
+};
+
+// Stage vector j; j is a compile-time constant after unrolling, so the
+// register / LDS choice costs nothing.
+template <typename R, int NR>
+__device__ __forceinline__ V<R, NR> kget(Ctx<R, NR> &c, Traj<R, NR> &s, int j) {
+    if (j < kRegStages) return s.k[j < kRegStages ? j : 0];
+    V<R, NR> o;
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+        o.v[q] = c.S.kd[(size_t)(j - kRegStages) * c.S.npad + c.row[q]];
+    return o;
+}
+template <typename R, int NR>
+__device__ __forceinline__ void kset(Ctx<R, NR> &c, Traj<R, NR> &s, int j, const V<R, NR> &x) {
+    if (j < kRegStages) {
+        s.k[j < kRegStages ? j : 0] = x;
+    } else {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) c.S.kd[(size_t)(j - kRegStages) * c.S.npad + c.row[q]] = x.v[q];
+    }
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ double bsum(Ctx<R, NR> &c, double v) {
+    const int lane = threadIdx.x & (QA_WAVE - 1), wid = threadIdx.x / QA_WAVE;
+    const int nw = blockDim.x / QA_WAVE;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) c.S.red[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < nw; ++w) t += c.S.red[w];
+    return t;
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ void stage_in(Ctx<R, NR> &c, const V<R, NR> &x) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NR; ++q) c.S.x[c.row[q]] = x.v[q];
+    __syncthreads();
+}
+
+// k = H_eff x for this thread's rows; x is staged in LDS first.  The rows'
+// loads are interleaved (both rows' columns, then values, then gathers).
+template <typename R, int NR>
+__device__ __forceinline__ V<R, NR> rhs(Ctx<R, NR> &c, const V<R, NR> &x) {
+    stage_in(c, x);
+    const C2<R> *hv = (const C2<R> *)c.A.h_vals;
+    V<R, NR> acc = vzero<R, NR>();
+    int p[NR];
+    int maxlen = 0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        p[q] = c.h_beg[q];
+        maxlen = max(maxlen, c.h_end[q] - c.h_beg[q]);
+    }
+    for (int u = 0; u < maxlen; u += 4) {
+        int cc[NR][4];
+        C2<R> vv[NR][4];
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const int pp = p[q] + u + w;
+                const bool ok = pp < c.h_end[q];
+                cc[q][w] = ok ? c.A.h_col[pp] : 0;
+                vv[q][w] = ok ? hv[pp] : mk<R>(0, 0);
+            }
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+#pragma unroll
+            for (int w = 0; w < 4; ++w) cfmaR<R>(acc.v[q], vv[q][w], c.S.x[cc[q][w]]);
+    }
+    return acc;
+}
+
+// (O x)_r for a CSR operator (c_ops / e_ops block); x already in LDS.
+template <typename R, int NR>
+__device__ __forceinline__ V<R, NR> op_rows(Ctx<R, NR> &c, const int *rowptr, const int *col,
+                                            const void *vals) {
+    V<R, NR> acc = vzero<R, NR>();
+    const C2<R> *v = (const C2<R> *)vals;
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+        if (c.act[q])
+            for (int p = rowptr[c.row[q]]; p < rowptr[c.row[q] + 1]; ++p)
+                cfmaR<R>(acc.v[q], v[p], c.S.x[col[p]]);
+    return acc;
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ double norm2(Ctx<R, NR> &c, const V<R, NR> &x) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) s += nrm2<R>(x.v[q]);  // inactive rows hold zeros
+    return bsum(c, s);
+}
+
+// base + dt * sum_{j < n} coef[j] k_j (zero terms skipped like iadd_data)
+template <typename R, int NR>
+__device__ __forceinline__ V<R, NR> combine(Ctx<R, NR> &c, Traj<R, NR> &s, const V<R, NR> &base,
+                                            const double *coef, int n, double dt) {
+    V<R, NR> o = base;
+#pragma unroll
+    for (int j = 0; j < kMcStages; ++j) {
+        if (j < n) {
+            const double f = dt * coef[j];
+            if (f != 0.0) {
+                const V<R, NR> kj = kget(c, s, j);
+#pragma unroll
+                for (int q = 0; q < NR; ++q) raxpyR<R>(o.v[q], (R)f, kj.v[q]);
+            }
+        }
+    }
+    return o;
+}
+
+// _compute_step (explicit_rk.pyx:358-389) + _error (:391-397)
+template <typename R, int NR>
+__device__ __forceinline__ double compute_step(Ctx<R, NR> &c, Traj<R, NR> &s, double dt) {
+    const McTableau &T = c.T;
+    s.attempts++;
+    if (T.fsal) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) s.k[0].v[q] = c.S.kfsal[c.row[q]];
+    } else {
+        s.k[0] = rhs(c, s.yprev);
+    }
+#pragma unroll
+    for (int i = 1; i < kMcStages; ++i) {
+        if (i < T.rk_step) {
+            const V<R, NR> x = combine(c, s, s.yprev, T.a[i], i, dt);
+            kset(c, s, i, rhs(c, x));
+        }
+    }
+    s.yfront = combine(c, s, s.yprev, T.b, T.rk_step, dt);
+    s.t_front = s.t_prev + dt;
+    if (!T.adaptive) return 0.0;
+    const V<R, NR> ev = combine(c, s, vzero<R, NR>(), T.e, T.rk_step, dt);
+    double qsum = 0.0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        if (c.act[q]) {
+            const double ratio = sqrt(nrm2<R>(ev.v[q])) /
+                                 (c.A.atol + c.A.rtol * sqrt(nrm2<R>(s.yfront.v[q])));
+            qsum += ratio * ratio;
+        }
+    }
+    return sqrt(bsum(c, qsum) / (double)c.A.d);
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ void prep_dense_out(Ctx<R, NR> &c, Traj<R, NR> &s) {
+    const McTableau &T = c.T;
+#pragma unroll
+    for (int i = 0; i < kMcStages; ++i) {
+        if (i >= T.rk_step && i < T.rk_extra) {
+            const V<R, NR> x = combine(c, s, s.yprev, T.a[i], i, s.dt_int);
+            kset(c, s, i, rhs(c, x));
+        }
+    }
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ V<R, NR> interpolate(Ctx<R, NR> &c, Traj<R, NR> &s, double t) {
+    const McTableau &T = c.T;
+    const double tau = (t - s.t_prev) / s.dt_int;
+    double bf[kMcStages];
+#pragma unroll
+    for (int i = 0; i < kMcStages; ++i) {
+        double v = 0.0;
+        if (i < T.rk_extra) {
+            for (int j = T.dorder - 1; j >= 0; --j) {
+                v += T.bi[i][j];
+                v *= tau;
+            }
+        }
+        bf[i] = v;
+    }
+    return combine(c, s, s.yprev, bf, T.rk_extra, s.dt_int);
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ double estimate_first_step(Ctx<R, NR> &c, Traj<R, NR> &s,
+                                                      const V<R, NR> &y0) {
+    const McTableau &T = c.T;
+    if (!T.adaptive) return 0.0;
+    double norm = sqrt(norm2(c, y0));
+    const double tol = c.A.atol + norm * c.A.rtol;
+    if (T.fsal) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) s.k[0].v[q] = c.S.kfsal[c.row[q]];
+    } else {
+        s.k[0] = rhs(c, y0);
+    }
+    if (norm <= c.A.atol) norm = 1.0;
+    double tmp = sqrt(norm2(c, s.k[0]));
+    double factorial = 1.0;
+    for (int i = 1; i <= T.order; ++i) factorial *= i;
+    double dt1, dt2;
+    if (tmp >= c.A.atol * 1e-6)
+        dt1 = pow(tol * factorial * pow(norm, T.order), 1.0 / (T.order + 1)) / tmp;
+    else
+        dt1 = pow(tol * factorial, 1.0 / (T.order + 1)) * norm * 0.5;
+    V<R, NR> yt = y0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) raxpyR<R>(yt.v[q], (R)(dt1 / 100), s.k[0].v[q]);
+    s.k[1] = rhs(c, yt);
+    tmp = sqrt(norm2(c, s.k[1]));
+    if (tmp >= c.A.atol * 1e-6)
+        dt2 = pow(tol * factorial * pow(norm, T.order), 1.0 / (T.order + 1)) / tmp;
+    else
+        dt2 = dt1;
+    double dt = fmin(dt1, dt2);
+    if (c.A.max_step != 0.0) dt = fmin(c.A.max_step, dt);
+    if (c.A.min_step != 0.0) dt = fmax(c.A.min_step, dt);
+    return dt;
+}
+
+// set_initial_value (explicit_rk.pyx:204-230)
+template <typename R, int NR>
+__device__ __forceinline__ void set_initial_value(Ctx<R, NR> &c, Traj<R, NR> &s,
+                                                  const V<R, NR> &y0, double t0) {
+    s.t = s.t_prev = s.t_front = t0;
+    s.dt_int = 0.0;
+    s.y = y0;
+    s.yfront = y0;
+    s.yprev = y0;
+    s.rk_state = RK_NORMAL;
+    if (c.T.fsal) {
+        const V<R, NR> kf = rhs(c, y0);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) c.S.kfsal[c.row[q]] = kf.v[q];
+    }
+    s.dt_safe = c.A.first_step != 0.0 ? c.A.first_step : estimate_first_step(c, s, y0);
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ void recompute_safe_step(Ctx<R, NR> &c, Traj<R, NR> &s, double err,
+                                                    double dt) {
+    if (!c.T.adaptive) return;
+    double f;
+    if (err == 0.0) {
+        f = 10.0;
+    } else {
+        f = 0.9 * pow(err, -1.0 / (c.T.order + 1));
+        f = fmin(10.0, f);
+        f = fmax(0.2, f);
+    }
+    s.dt_safe = dt * f;
+    if (c.A.max_step != 0.0) s.dt_safe = fmin(c.A.max_step, s.dt_safe);
+    if (c.A.min_step != 0.0) s.dt_safe = fmax(c.A.min_step, s.dt_safe);
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ double get_timestep(Ctx<R, NR> &c, Traj<R, NR> &s, double t) {
+    const double dt_needed = t - s.t_prev;
+    if (!c.T.adaptive) return dt_needed;
+    if (c.T.interpolate) return s.dt_safe;
+    if (dt_needed <= s.dt_safe) return dt_needed;
+    return dt_needed / (double)((long long)(dt_needed / s.dt_safe) + 1);
+}
+
+// _step_in_err (explicit_rk.pyx:333-356); returns attempts, sets status
+template <typename R, int NR>
+__device__ __forceinline__ int step_in_err(Ctx<R, NR> &c, Traj<R, NR> &s, double t, int max_step) {
+    double error = 1.0;
+    int n = 0;
+    while (error >= 1.0) {
+        const double dt = get_timestep(c, s, t);
+        error = compute_step(c, s, dt);
+        s.dt_int = dt;
+        recompute_safe_step(c, s, error, dt);
+        if (dt == c.A.min_step && error > 1.0) {
+            s.status = MC_DT_UNDERFLOW;
+            break;
+        }
+        n++;
+        if (n > max_step) {
+            s.status = MC_TOO_MUCH_WORK;
+            break;
+        }
+    }
+    if (c.T.fsal) {
+#pragma unroll
+        for (int j = 0; j < kMcStages; ++j) {
+            if (j == c.T.rk_step - 1) {
+                const V<R, NR> kl = kget(c, s, j);
+#pragma unroll
+                for (int q = 0; q < NR; ++q) c.S.kfsal[c.row[q]] = kl.v[q];
+            }
+        }
+    }
+    return n;
+}
+
+// integrate(t, step=True): explicit_rk.pyx:278-331 with step=True.
+template <typename R, int NR>
+__device__ __forceinline__ void mcstep(Ctx<R, NR> &c, Traj<R, NR> &s, double t) {
+    int nsteps_left = c.A.nsteps;
+    if (t == s.t) return;
+    if (t < s.t_prev) {
+        s.status = MC_OUTSIDE_RANGE;
+        return;
+    }
+    if (c.T.interpolate && t < s.t_front) {
+        if (s.rk_state != RK_INTERPOLATED) {
+            prep_dense_out(c, s);
+            s.rk_state = RK_INTERPOLATED;
+        }
+        s.y = interpolate(c, s, t);
+        s.t = t;
+        return;
+    }
+    s.rk_state = RK_NORMAL;
+    if (s.t < s.t_front && t > s.t_front) t = s.t_front;
+    if (s.t_front < t) {
+        s.yprev = s.yfront;
+        s.t_prev = s.t_front;
+        nsteps_left -= step_in_err(c, s, t, nsteps_left);
+    }
+    if (s.status < 0) return;
+    if (s.t_front > t) {
+        prep_dense_out(c, s);
+        s.rk_state = RK_INTERPOLATED;
+        s.t = t;
+        s.y = interpolate(c, s, t);
+    } else {
+        s.rk_state = RK_AT_FRONT;
+        s.t = s.t_front;
+        s.y = s.yfront;
+    }
+}
+
+struct Rng {
+    const double *d;
+    int n, used;
+};
+
+__device__ __forceinline__ bool draw(Rng &g, double &u) {
+    if (g.used >= g.n) return false;
+    u = g.d[g.used++];
+    return true;
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ void write_expect(Ctx<R, NR> &c, const V<R, NR> &y, int traj, int k) {
+    const double n2 = norm2(c, y);
+    stage_in(c, y);
+    for (int e = 0; e < c.A.neops; ++e) {
+        const int *rp = c.A.e_rowptr + (size_t)e * (c.A.d + 1);
+        const V<R, NR> oy = op_rows(c, rp, c.A.e_col, c.A.e_vals);
+        double re = 0.0, im = 0.0;  // conj(y_r) (O y)_r in FP64
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            re += (double)y.v[q].x * oy.v[q].x + (double)y.v[q].y * oy.v[q].y;
+            im += (double)y.v[q].x * oy.v[q].y - (double)y.v[q].y * oy.v[q].x;
+        }
+        re = bsum(c, re);
+        im = bsum(c, im);
+        if (threadIdx.x == 0)
+            c.A.expect_out[((size_t)traj * c.A.ntimes + k) * c.A.neops + e] =
+                make_double2(re / n2, im / n2);
+    }
+}
+
+template <typename R, int NR>
+__global__ __launch_bounds__(kMcThreads) void mc_traj_kernel(McArgs A) {
+    extern __shared__ __align__(16) unsigned char mc_dyn_lds[];
+    __shared__ McTableau T;
+    McLds<R> S;
+    const int nt = blockDim.x;
+    const int npad = nt * NR;
+    {
+        unsigned char *p = mc_dyn_lds;
+        S.npad = npad;
+        S.x = (C2<R> *)p;
+        p += sizeof(C2<R>) * npad;
+        S.kd = (C2<R> *)p;
+        p += sizeof(C2<R>) * npad * (kMcStages - kRegStages);
+        S.kfsal = (C2<R> *)p;
+        p += sizeof(C2<R>) * npad;
+        S.red = (double *)p;
+        p += sizeof(double) * (nt / QA_WAVE);
+        S.probs = (double *)p;
+    }
+    {
+        const double *src = (const double *)A.tab;
+        double *dst = (double *)&T;
+        const int nd = (int)(sizeof(McTableau) / sizeof(double));
+        for (int i = threadIdx.x; i < nd; i += nt) dst[i] = src[i];
+    }
+    for (int i = threadIdx.x; i < (kMcStages - kRegStages + 2) * npad; i += nt)
+        S.x[i] = mk<R>(0, 0);  // x, k_10.. and the FSAL slot start at zero
+    __syncthreads();
+    Ctx<R, NR> c{A, S, T};
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        c.row[q] = threadIdx.x + q * nt;
+        c.act[q] = c.row[q] < A.d;
+        c.h_beg[q] = c.act[q] ? A.h_rowptr[c.row[q]] : 0;
+        c.h_end[q] = c.act[q] ? A.h_rowptr[c.row[q] + 1] : 0;
+    }
+
+    for (int traj = blockIdx.x; traj < A.ntraj; traj += gridDim.x) {
+        Traj<R, NR> s;
+        s.status = MC_OK;
+        s.attempts = 0;
+        Rng g{A.draws + (size_t)traj * A.max_draws, A.max_draws, 0};
+        int jumps = 0;
+        double target = 0.0;
+        V<R, NR> y0;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            const double2 p0 = c.act[q] ? A.psi0[c.row[q]] : make_double2(0.0, 0.0);
+            y0.v[q] = mk<R>((R)p0.x, (R)p0.y);
+        }
+        if (!draw(g, target)) s.status = MC_DRAWS_EXHAUSTED;
+        if (s.status == MC_OK) set_initial_value(c, s, y0, A.times[0]);
+        if (s.status == MC_OK) write_expect(c, y0, traj, 0);
+        for (int k = 1; k < A.ntimes && s.status == MC_OK; ++k) {
+            const double tk = A.times[k];
+            // MCIntegrator.integrate (mcsolve.py:286-302)
+            double t_old = s.t;
+            double norm_old = norm2(c, s.y);
+            V<R, NR> y_old = s.y;
+            long long guard = 0;
+            while (t_old < tk && s.status == MC_OK) {
+                if (++guard > 1000000) {
+                    s.status = MC_NO_PROGRESS;
+                    break;
+                }
+                mcstep(c, s, tk);
+                if (s.status < 0) break;
+                const double t_step = s.t;
+                double nrm = norm2(c, s.y);
+                if (nrm <= target) {
+                    // _find_collapse_time (mcsolve.py:321-369)
+                    int tries = 0;
+                    double tp = t_old, tf = t_step, nold = norm_old, t_guess = tf;
+                    while (tries < A.norm_steps) {
+                        tries++;
+                        if (tf - tp < A.norm_t_tol) {
+                            t_guess = tf;
+                            break;
+                        }
+                        const double dtc = tf - tp;
+                        double ratio = log(nold / target) / log(nold / nrm);
+                        if (ratio < A.norm_min_step) ratio = A.norm_min_step;
+                        if (ratio > 1.0 - A.norm_min_step) ratio = 1.0 - A.norm_min_step;
+                        t_guess = tp + dtc * ratio;
+                        if (t_guess - tp < A.norm_t_tol) t_guess = tp + A.norm_t_tol;
+                        mcstep(c, s, t_guess);
+                        if (s.status < 0) break;
+                        const double n2g = norm2(c, s.y);
+                        if (fabs(target - n2g) < A.norm_tol * target) {
+                            break;
+                        } else if (n2g < target) {
+                            tf = t_guess;
+                            nrm = n2g;
+                        } else {
+                            tp = t_guess;
+                            nold = n2g;
+                        }
+                    }
+                    if (s.status < 0) break;
+                    if (tries >= A.norm_steps) {
+                        s.status = MC_COLLAPSE_NOT_FOUND;
+                        break;
+                    }
+                    // _do_collapse (mcsolve.py:371-406); the state is s.y
+                    const V<R, NR> psi = s.y;
+                    stage_in(c, psi);
+                    int which = 0;
+                    if (A.nchan > 1) {
+                        double tot = 0.0;
+                        for (int ch = 0; ch < A.nchan; ++ch) {
+                            const int *rp = A.c_rowptr + (size_t)ch * (A.d + 1);
+                            const double pr = norm2(c, op_rows(c, rp, A.c_col, A.c_vals));
+                            if (threadIdx.x == 0) S.probs[ch] = pr;
+                            tot += pr;
+                        }
+                        __syncthreads();
+                        double u;
+                        if (!draw(g, u)) {
+                            s.status = MC_DRAWS_EXHAUSTED;
+                            break;
+                        }
+                        double tg = tot * u - S.probs[0];
+                        while (tg > 0.0 && which < A.nchan - 1) {
+                            which++;
+                            tg -= S.probs[which];
+                        }
+                    }
+                    const int *rp = A.c_rowptr + (size_t)which * (A.d + 1);
+                    V<R, NR> ynew = op_rows(c, rp, A.c_col, A.c_vals);
+                    const double nn = sqrt(norm2(c, ynew));
+                    if (nn < A.mc_corr_eps) {
+                        const double np = sqrt(norm2(c, psi));
+#pragma unroll
+                        for (int q = 0; q < NR; ++q) {
+                            ynew.v[q].x = (R)(psi.v[q].x / np);
+                            ynew.v[q].y = (R)(psi.v[q].y / np);
+                        }
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < NR; ++q) {
+                            ynew.v[q].x = (R)(ynew.v[q].x / nn);
+                            ynew.v[q].y = (R)(ynew.v[q].y / nn);
+                        }
+                        if (threadIdx.x == 0 && jumps < A.max_jumps) {
+                            A.col_times[(size_t)traj * A.max_jumps + jumps] = t_guess;
+                            A.col_which[(size_t)traj * A.max_jumps + jumps] = which;
+                        }
+                        jumps++;
+                        if (!draw(g, target)) {
+                            s.status = MC_DRAWS_EXHAUSTED;
+                            break;
+                        }
+                    }
+                    set_initial_value(c, s, ynew, t_guess);
+                    t_old = s.t;
+                    y_old = s.y;
+                    norm_old = 1.0;
+                } else {
+                    t_old = t_step;
+                    y_old = s.y;
+                    norm_old = nrm;
+                }
+            }
+            if (s.status != MC_OK) break;
+            write_expect(c, y_old, traj, k);
+        }
+        if (threadIdx.x == 0) {
+            A.info[(size_t)traj * 4 + 0] = s.status;
+            A.info[(size_t)traj * 4 + 1] = jumps;
+            A.info[(size_t)traj * 4 + 2] = g.used;
+            A.info[(size_t)traj * 4 + 3] = (int)s.attempts;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename R, int NR>
+int launch_mc_nr(const McArgs &A, int threads, int blocks, hipStream_t s) {
+    const size_t lds = mc_lds_bytes<R>(threads * NR, threads);
+    QA_HIP(hipFuncSetAttribute((const void *)mc_traj_kernel<R, NR>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((mc_traj_kernel<R, NR>), dim3(blocks), dim3(threads), lds, s, A);
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
+// Rows per thread: NR = 1 up to 512 rows (one wave per 64 rows), then 2
+// rows per thread on at most 512 threads, which leaves each lane 256 VGPRs.
+inline void mc_shape(int d, int *nr, int *threads) {
+    int n = 1;
+    while (n < 2 && (d + n - 1) / n > kMcThreads) n *= 2;
+    *nr = n;
+    const int rows = (d + n - 1) / n;
+    *threads = ((rows + QA_WAVE - 1) / QA_WAVE) * QA_WAVE;
+}
+
+template <typename R>
+int launch_mc(const McArgs &A, int blocks, hipStream_t s) {
+    int nr, threads;
+    mc_shape(A.d, &nr, &threads);
+    if (nr == 1) return launch_mc_nr<R, 1>(A, threads, blocks, s);
+    return launch_mc_nr<R, 2>(A, threads, blocks, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int qa_mc_tableau_size(void) { return (int)sizeof(McTableau); }
+
+// Pack a tableau (host arrays as for qa_rk_create) into the device layout
+// the trajectory kernel reads; `out` is a host buffer of
+// qa_mc_tableau_size() bytes to be copied to the device by the caller.
+int qa_mc_pack_tableau(void *out, int order, int rk_step, int rk_extra, int dorder,
+                       const double *a, const double *b, const double *c, const double *e,
+                       const double *bi, int interpolate) {
+    QA_REQUIRE(out && a && b && c, "qa_mc_pack_tableau: null argument");
+    QA_REQUIRE(rk_step >= 1 && rk_step <= rk_extra && rk_extra <= kMcStages,
+               "qa_mc_pack_tableau: 1 <= rk_step <= rk_extra <= %d", kMcStages);
+    QA_REQUIRE(!bi || (dorder >= 1 && dorder <= kMcOrder), "qa_mc_pack_tableau: dense order");
+    McTableau *T = (McTableau *)out;
+    memset(T, 0, sizeof(McTableau));
+    for (int i = 0; i < rk_extra; ++i) {
+        for (int j = 0; j < rk_extra; ++j) T->a[i][j] = a[(size_t)i * rk_extra + j];
+        T->c[i] = c[i];
+        if (bi)
+            for (int j = 0; j < dorder; ++j) T->bi[i][j] = bi[(size_t)i * dorder + j];
+    }
+    for (int i = 0; i < rk_step; ++i) {
+        T->b[i] = b[i];
+        T->e[i] = e ? e[i] : 0.0;
+    }
+    T->order = order;
+    T->rk_step = rk_step;
+    T->rk_extra = (bi && interpolate) ? rk_extra : rk_step;
+    T->dorder = bi ? dorder : 0;
+    T->interpolate = (bi && interpolate) ? 1 : 0;
+    T->adaptive = e != nullptr;
+    int fsal = 1;
+    for (int j = 0; j < rk_step; ++j) {
+        const double x = a[(size_t)(rk_step - 1) * rk_extra + j], y = b[j];
+        if (fabs(x - y) > 1e-14 + 1e-5 * fabs(y)) fsal = 0;
+    }
+    T->fsal = fsal;
+    return QA_OK;
+}
+
+int qa_mc_run(int d, int ntraj, int ntimes, const double *times_dev, const void *psi0_dev,
+              const int *h_rowptr, const int *h_col, const void *h_vals, int nchan,
+              const int *c_rowptr, const int *c_col, const void *c_vals, int neops,
+              const int *e_rowptr, const int *e_col, const void *e_vals,
+              const void *tableau_dev, const double *draws_dev, int max_draws,
+              double rtol, double atol, int nsteps, double first_step, double min_step,
+              double max_step, int norm_steps, double norm_t_tol, double norm_tol,
+              double norm_min_step, double mc_corr_eps, int fp32, void *expect_out_dev,
+              double *col_times_dev, int *col_which_dev, int max_jumps, int *info_dev,
+              int grid, void *stream) {
+    QA_REQUIRE(d >= 1 && d <= 2 * kMcThreads, "qa_mc_run: 1 <= d <= %d (rows held per thread)",
+               2 * kMcThreads);
+    QA_REQUIRE(ntraj >= 0 && ntimes >= 1, "qa_mc_run: need ntimes >= 1");
+    QA_REQUIRE(nchan >= 1 && nchan <= kMcMaxChan, "qa_mc_run: 1 <= nchan <= %d", kMcMaxChan);
+    QA_REQUIRE(neops >= 0 && neops <= kMcMaxEops, "qa_mc_run: neops <= %d", kMcMaxEops);
+    QA_REQUIRE(max_draws >= 1 && max_jumps >= 1, "qa_mc_run: need max_draws, max_jumps >= 1");
+    QA_REQUIRE(norm_steps >= 1 && nsteps >= 0, "qa_mc_run: bad step limits");
+    QA_REQUIRE(times_dev && psi0_dev && h_rowptr && tableau_dev && draws_dev && info_dev &&
+                   expect_out_dev && col_times_dev && col_which_dev && c_rowptr,
+               "qa_mc_run: null pointer");
+    if (ntraj == 0) return QA_OK;
+    McArgs A;
+    A.d = d;
+    A.ntraj = ntraj;
+    A.ntimes = ntimes;
+    A.nchan = nchan;
+    A.neops = neops;
+    A.max_draws = max_draws;
+    A.max_jumps = max_jumps;
+    A.traj_offset = 0;
+    A.h_rowptr = h_rowptr;
+    A.h_col = h_col;
+    A.h_vals = h_vals;
+    A.c_rowptr = c_rowptr;
+    A.c_col = c_col;
+    A.c_vals = c_vals;
+    A.e_rowptr = e_rowptr;
+    A.e_col = e_col;
+    A.e_vals = e_vals;
+    A.tab = (const McTableau *)tableau_dev;
+    A.times = times_dev;
+    A.psi0 = (const double2 *)psi0_dev;
+    A.draws = draws_dev;
+    A.rtol = rtol;
+    A.atol = atol;
+    A.first_step = first_step;
+    A.min_step = min_step != 0.0 ? min_step : 1e-15;
+    A.max_step = max_step;
+    A.nsteps = nsteps;
+    A.norm_steps = norm_steps;
+    A.norm_t_tol = norm_t_tol;
+    A.norm_tol = norm_tol;
+    A.norm_min_step = norm_min_step;
+    A.mc_corr_eps = mc_corr_eps;
+    A.expect_out = (double2 *)expect_out_dev;
+    A.col_times = col_times_dev;
+    A.col_which = col_which_dev;
+    A.info = info_dev;
+    if (grid <= 0) grid = ntraj;
+    if (grid > ntraj) grid = ntraj;
+    hipStream_t s = (hipStream_t)stream;
+    return fp32 ? launch_mc<float>(A, grid, s) : launch_mc<double>(A, grid, s);
+}
+
+}  // extern "C"

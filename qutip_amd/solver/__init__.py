@@ -6,5 +6,6 @@ from .result import Result
 from .solver_base import Solver
 from .sesolve import sesolve, SESolver
 from .mesolve import mesolve, MESolver
+from .mcsolve import mcsolve, MCSolver, McResult
 
 Solver.add_integrator(IntegratorHipRK, "hip_rk")

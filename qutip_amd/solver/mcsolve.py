@@ -1,0 +1,331 @@
+"""mcsolve / MCSolver: batched Monte-Carlo wave-function trajectories.
+
+Same physics and options as qutip/solver/mcsolve.py (MCSolver :424-821,
+MCIntegrator :228-419): H_eff = -iH - 1/2 sum_k c_k^dag c_k (:491-496),
+norm-decay waiting times, log-ratio secant search of the collapse time,
+channel choice weighted by <psi|c_k^dag c_k|psi>.  The trajectories, instead
+of being farmed to CPU processes one by one (qutip/solver/parallel.py), all
+run inside one persistent kernel (``qa_mc_run``): each workgroup integrates
+whole trajectories with its own step sizes and jump times.
+
+Random numbers follow the reference stream exactly: trajectory i uses
+``numpy.random.default_rng(seeds[i])`` (PCG64; qutip/solver/multitraj.py:
+354-393) and consumes one draw for the first target norm, then per jump one
+draw for the channel (when there is more than one) and one for the next
+target (mcsolve.py:276-279, 388, 405).  The draws are made here, on the
+host, before the launch and streamed to the device.
+
+The ensemble reduction is MultiTrajResult's: average = sum / ntraj,
+std = sqrt(|mean(x^2) - |mean(x)|^2|) (qutip/solver/multitrajresult.py:
+261-279, 1116-1124).
+"""
+import ctypes as C
+import time
+
+import numpy as np
+import scipy.sparse as sp
+from numpy.random import SeedSequence
+
+from .. import _lib
+from .. import data as _data
+from ..core import Qobj, QobjEvo
+from .integrator import IntegratorException
+from .tableaus import TABLEAUS
+
+__all__ = ["mcsolve", "MCSolver", "McResult", "MC_STATUS"]
+
+MC_STATUS = {
+    0: "ok",
+    -1: ("Too much work done in one call. Try to increase the nsteps parameter "
+         "or increasing the tolerance."),
+    -2: "Step size becomes too small. Try increasing tolerance.",
+    -3: "Step outside available range.",
+    -10: ("Could not find the collapse time within desired tolerance. Increase "
+          "accuracy of the ODE solver or lower the tolerance with the options "
+          "'norm_steps', 'norm_tol', 'norm_t_tol'."),
+    -11: "pre-drawn random numbers exhausted",
+    -12: "no progress",
+}
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _vp(p):
+    return C.c_void_p(p)
+
+
+class McResult:
+    """Ensemble result with the attributes of qutip's McResult used here."""
+
+    def __init__(self, tlist, e_ops, runs_expect, info, col_times, col_which, seeds, stats,
+                 keep_runs):
+        self.times = list(tlist)
+        self.e_ops = e_ops
+        self.seeds = seeds
+        self.stats = stats
+        n = runs_expect.shape[0]
+        self.num_trajectories = n
+        self.ntraj = n
+        vals = []
+        for j, O in enumerate(e_ops):
+            v = runs_expect[:, :, j]
+            if O.isherm:
+                v = v.real
+            vals.append(v)
+        self._runs = vals
+        self.average_expect = [v.mean(axis=0) for v in vals]
+        self.std_expect = [np.sqrt(np.abs((v ** 2).mean(axis=0) - np.abs(v.mean(axis=0)) ** 2))
+                           for v in vals]
+        self.expect = self.average_expect
+        self.runs_expect = [v.copy() for v in vals] if keep_runs else None
+        nj = info[:, 1]
+        self.col_times = [list(col_times[i, :min(nj[i], col_times.shape[1])]) for i in range(n)]
+        self.col_which = [list(col_which[i, :min(nj[i], col_which.shape[1])]) for i in range(n)]
+        self.num_jumps = nj.copy()
+
+    @property
+    def average_e_data(self):
+        return dict(enumerate(self.average_expect))
+
+    def __repr__(self):
+        return f"<McResult ntraj={self.num_trajectories} ntimes={len(self.times)}>"
+
+
+def _read_seeds(seeds, ntraj):
+    """MultiTrajSolver._read_seed (qutip/solver/multitraj.py:354-374)."""
+    if seeds is None:
+        return SeedSequence().spawn(ntraj)
+    if isinstance(seeds, SeedSequence):
+        return seeds.spawn(ntraj)
+    if not isinstance(seeds, list):
+        return SeedSequence(seeds).spawn(ntraj)
+    if len(seeds) >= ntraj:
+        return [s if isinstance(s, SeedSequence) else SeedSequence(s) for s in seeds[:ntraj]]
+    raise ValueError("A seed list must be longer than ntraj")
+
+
+def draws_for(seeds, n):
+    """The first n uniform draws of default_rng(seed) for every seed."""
+    out = np.empty((len(seeds), n), dtype=np.float64)
+    for i, s in enumerate(seeds):
+        out[i] = np.random.default_rng(s).random(n)
+    return out
+
+
+def _csr_block(ops, d, fp32):
+    """Concatenate CSR operators: absolute row pointers, one block of d + 1 per
+    operator."""
+    rowptr, cols, vals = [], [], []
+    off = 0
+    for O in ops:
+        A = sp.csr_matrix(O, dtype=np.complex128)
+        A.sort_indices()
+        if A.shape != (d, d):
+            raise ValueError(f"operator of shape {A.shape}, expected {(d, d)}")
+        rowptr.append(A.indptr.astype(np.int64) + off)
+        cols.append(A.indices)
+        vals.append(A.data)
+        off += A.nnz
+    rp = np.concatenate(rowptr).astype(np.int32) if rowptr else np.zeros(1, np.int32)
+    cl = np.concatenate(cols).astype(np.int32) if off else np.zeros(1, np.int32)
+    vl = np.concatenate(vals) if off else np.zeros(1, np.complex128)
+    vl = vl.astype(np.complex64 if fp32 else np.complex128)
+    return (_data.DeviceBuffer.from_array(rp), _data.DeviceBuffer.from_array(cl),
+            _data.DeviceBuffer.from_array(vl))
+
+
+class MCSolver:
+    """Monte-Carlo wave-function solver, all trajectories in one persistent
+    kernel.  ``options`` keys as qutip's MCSolver.solver_options
+    (mcsolve.py:449-466) and the integrator options of IntegratorVern7, plus
+    ``precision`` ('fp64' | 'fp32': complex64 state and RHS with FP64 norms),
+    ``max_draws`` / ``max_jumps`` (buffer sizes, grown automatically)."""
+    name = "mcsolve"
+    solver_options = {
+        "progress_bar": "",
+        "progress_kwargs": {},
+        "store_final_state": False,
+        "store_states": None,
+        "keep_runs_results": False,
+        "map": "serial",
+        "mpi_options": {},
+        "num_cpus": None,
+        "bitgenerator": None,
+        "method": "vern7",
+        "mc_corr_eps": 1e-10,
+        "norm_steps": 25,
+        "norm_t_tol": 1e-6,
+        "norm_tol": 1e-4,
+        "norm_min_step": 0.1,
+        "improved_sampling": False,
+        # integrator (IntegratorVern7.integrator_options)
+        "atol": 1e-8,
+        "rtol": 1e-6,
+        "nsteps": 1000,
+        "first_step": 0,
+        "max_step": 0,
+        "min_step": 0,
+        "interpolate": True,
+        # device
+        "precision": "fp64",
+        "max_draws": 64,
+        "max_jumps": 64,
+        "grid": 0,
+    }
+
+    def __init__(self, H, c_ops, *, options=None):
+        if isinstance(H, QobjEvo):
+            H = H.qobj
+        if isinstance(c_ops, (Qobj, QobjEvo)):
+            c_ops = [c_ops]
+        c_ops = [c.qobj if isinstance(c, QobjEvo) else c for c in c_ops]
+        if not c_ops:
+            raise ValueError("mcsolve needs at least one collapse operator")
+        if H.issuper:
+            raise NotImplementedError("a Liouvillian H is not supported by the batched kernel")
+        self.H = H
+        self.c_ops = c_ops
+        self.options = dict(self.solver_options)
+        for k, v in (options or {}).items():
+            if k not in self.solver_options:
+                raise KeyError(f"Options ['{k}'] are not supported.")
+            self.options[k] = v
+        if self.options["improved_sampling"]:
+            raise NotImplementedError("improved_sampling is not supported by the batched kernel")
+        if self.options["method"] not in TABLEAUS:
+            raise ValueError(f"method must be one of {sorted(TABLEAUS)}")
+        self.d = H.shape[0]
+        Hc = H.to_csr()
+        Heff = -1j * Hc
+        self._n_ops = []
+        for c in c_ops:
+            cc = c.to_csr()
+            n = (cc.conj().T @ cc).tocsr()
+            self._n_ops.append(n)
+            Heff = Heff - 0.5 * n
+        Heff = sp.csr_matrix(Heff)
+        Heff.sum_duplicates()
+        Heff.eliminate_zeros()
+        Heff.sort_indices()
+        self.H_eff = Heff
+        self._dev = None
+
+    def _device_ops(self, fp32):
+        if self._dev is None or self._dev[0] != fp32:
+            h = _csr_block([self.H_eff], self.d, fp32)
+            c = _csr_block([c.to_csr() for c in self.c_ops], self.d, fp32)
+            tab = TABLEAUS[self.options["method"]]
+            size = _lib.lib().qa_mc_tableau_size()
+            buf = np.zeros(size, dtype=np.uint8)
+            a, b, cc = (np.ascontiguousarray(tab[k], dtype=np.float64) for k in ("a", "b", "c"))
+            e = np.ascontiguousarray(tab["e"], dtype=np.float64)
+            bi = np.ascontiguousarray(tab["bi"], dtype=np.float64)
+            _lib.call("qa_mc_pack_tableau", buf.ctypes.data_as(C.c_void_p), int(tab["order"]),
+                      len(b), len(cc), bi.shape[1], _dp(a), _dp(b), _dp(cc), _dp(e), _dp(bi),
+                      int(bool(self.options["interpolate"])))
+            tab_dev = _data.DeviceBuffer.from_array(buf)
+            self._dev = (fp32, h, c, tab_dev)
+        return self._dev
+
+    def _launch(self, psi0, tlist, e_ops, draws, max_jumps, fp32):
+        o = self.options
+        ntraj, max_draws = draws.shape
+        ntimes = len(tlist)
+        _, h, c, tab = self._device_ops(fp32)
+        e = _csr_block([O.to_csr() for O in e_ops] or [sp.identity(self.d)], self.d, fp32)
+        neops = len(e_ops)
+        times = _data.DeviceBuffer.from_array(np.asarray(tlist, dtype=np.float64))
+        p0 = _data.DeviceBuffer.from_array(np.ascontiguousarray(psi0, dtype=np.complex128))
+        dr = _data.DeviceBuffer.from_array(np.ascontiguousarray(draws))
+        ex = _data.DeviceBuffer(16 * ntraj * ntimes * max(neops, 1))
+        ct = _data.DeviceBuffer(8 * ntraj * max_jumps)
+        cw = _data.DeviceBuffer(4 * ntraj * max_jumps)
+        info = _data.DeviceBuffer(16 * ntraj)
+        L = _lib.lib()
+        rc = L.qa_mc_run(
+            self.d, ntraj, ntimes, _vp(times.ptr), _vp(p0.ptr), _vp(h[0].ptr), _vp(h[1].ptr),
+            _vp(h[2].ptr), len(self.c_ops), _vp(c[0].ptr), _vp(c[1].ptr), _vp(c[2].ptr), neops,
+            _vp(e[0].ptr), _vp(e[1].ptr), _vp(e[2].ptr), _vp(tab.ptr), _vp(dr.ptr), max_draws,
+            float(o["rtol"]), float(o["atol"]), int(o["nsteps"]), float(o["first_step"]),
+            float(o["min_step"]), float(o["max_step"]), int(o["norm_steps"]),
+            float(o["norm_t_tol"]), float(o["norm_tol"]), float(o["norm_min_step"]),
+            float(o["mc_corr_eps"]), int(fp32), _vp(ex.ptr), _vp(ct.ptr), _vp(cw.ptr), max_jumps,
+            _vp(info.ptr), int(o["grid"]), None)
+        _lib.check(rc, "qa_mc_run")
+        _lib.call("qa_device_sync")
+        out = np.empty((ntraj, ntimes, max(neops, 1)), dtype=np.complex128)
+        ex.download(out)
+        cts = np.empty((ntraj, max_jumps), dtype=np.float64)
+        ct.download(cts)
+        cws = np.empty((ntraj, max_jumps), dtype=np.int32)
+        cw.download(cws)
+        inf = np.empty((ntraj, 4), dtype=np.int32)
+        info.download(inf)
+        return out[:, :, :neops], inf, cts, cws
+
+    def run(self, state, tlist, ntraj=1, *, args=None, e_ops=None, timeout=None,
+            target_tol=None, seeds=None):
+        t0 = time.time()
+        if target_tol is not None:
+            raise NotImplementedError("target_tol is not supported by the batched kernel")
+        if isinstance(e_ops, dict):
+            e_ops = list(e_ops.values())
+        e_ops = [] if e_ops is None else ([e_ops] if isinstance(e_ops, Qobj) else list(e_ops))
+        e_ops = [O.qobj(0) if isinstance(O, QobjEvo) else O for O in e_ops]
+        if not state.isket:
+            raise NotImplementedError("the batched kernel runs ket initial states")
+        tlist = np.asarray(tlist, dtype=np.float64)
+        ss = _read_seeds(seeds, ntraj)
+        fp32 = self.options["precision"] == "fp32"
+        psi0 = state.full().ravel()
+        max_draws = int(self.options["max_draws"])
+        max_jumps = int(self.options["max_jumps"])
+        runs = np.zeros((ntraj, len(tlist), len(e_ops)), dtype=np.complex128)
+        info = np.zeros((ntraj, 4), dtype=np.int32)
+        cts = np.zeros((ntraj, max_jumps))
+        cws = np.zeros((ntraj, max_jumps), dtype=np.int32)
+        todo = np.arange(ntraj)
+        t_prep = time.time() - t0
+        t1 = time.time()
+        launches = 0
+        while len(todo):
+            draws = draws_for([ss[i] for i in todo], max_draws)
+            ex, inf, ct, cw = self._launch(psi0, tlist, e_ops, draws, max_jumps, fp32)
+            launches += 1
+            if ct.shape[1] > cts.shape[1]:
+                cts = np.pad(cts, ((0, 0), (0, ct.shape[1] - cts.shape[1])))
+                cws = np.pad(cws, ((0, 0), (0, cw.shape[1] - cws.shape[1])))
+            runs[todo] = ex
+            info[todo] = inf
+            cts[todo, :ct.shape[1]] = ct
+            cws[todo, :cw.shape[1]] = cw
+            bad = inf[:, 0]
+            for code in np.unique(bad[(bad < 0) & (bad != -11)]):
+                msg = MC_STATUS.get(int(code), f"status {code}")
+                if code == -10:
+                    raise RuntimeError(msg)
+                raise IntegratorException(msg)
+            short = (bad == -11) | (inf[:, 1] > max_jumps)
+            todo = todo[short]
+            if len(todo):
+                max_draws *= 4
+                max_jumps = max(max_jumps, int(inf[short, 1].max()) + 8)
+                if max_draws > 1 << 20:
+                    raise RuntimeError("trajectories need more than 2**20 random draws")
+        stats = {"method": f"mcsolve hip ({self.options['method']}, "
+                           f"{'complex64' if fp32 else 'complex128'})",
+                 "preparation time": t_prep, "run time": time.time() - t1,
+                 "launches": launches, "rk_attempts": int(info[:, 3].sum()),
+                 "jumps": int(info[:, 1].sum()), "num_collapse": len(self.c_ops),
+                 "end_condition": "ntraj reached"}
+        return McResult(tlist, e_ops, runs, info, cts, cws, ss, stats,
+                        bool(self.options["keep_runs_results"]))
+
+
+def mcsolve(H, state, tlist, c_ops=(), e_ops=None, ntraj=500, *, args=None, options=None,
+            seeds=None, target_tol=None, timeout=None, **kwargs):
+    """Monte-Carlo evolution (qutip/solver/mcsolve.py:24-197)."""
+    solver = MCSolver(H, list(c_ops), options=options)
+    return solver.run(state, tlist, ntraj, e_ops=e_ops, seeds=seeds, target_tol=target_tol)
