@@ -607,7 +607,7 @@ __global__ __launch_bounds__(kMcThreads) void mc_traj_kernel(McArgs A) {
             V<R, NR> y_old = s.y;
             long long guard = 0;
             while (t_old < tk && s.status == MC_OK) {
-                if (++guard > 1000000) {
+                if (++guard > 100000) {
                     s.status = MC_NO_PROGRESS;
                     break;
                 }
