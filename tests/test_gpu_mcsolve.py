@@ -32,8 +32,12 @@ def test_trajectories_match_oracle(precision):
     res = qa.mcsolve(s["H"], s["psi0"], s["tlist"], s["c_ops"], e_ops=s["e_ops"], ntraj=ntraj,
                      seeds=list(range(ntraj)),
                      options={"keep_runs_results": True, "precision": precision})
-    tol_t = 1e-6 if precision == "fp64" else 1e-4
-    tol_e = 1e-6 if precision == "fp64" else 1e-4
+    # complex64 norms move the secant's stopping point inside its own band
+    # |norm^2 - target| < norm_tol * target: delta t ~ norm_tol / decay rate
+    # (the decay rate here is sum_i gamma <n_i> <= 0.8), and <O> after a
+    # jump moves with it.
+    tol_t = 1e-6 if precision == "fp64" else 1e-3
+    tol_e = 1e-6 if precision == "fp64" else 2e-3
     for i in range(ntraj):
         tr = omc.Trajectory(m["H"], m["c_ops"], tableaus.vern7)
         ref = tr.run(m["psi0"], m["tlist"], np.random.default_rng(np.random.SeedSequence(i)),
