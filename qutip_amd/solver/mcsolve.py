@@ -229,16 +229,27 @@ class MCSolver:
             self._dev = (fp32, h, c, tab_dev)
         return self._dev
 
+    def run_batch(self, psi0, tlist, e_ops, draws, max_jumps=64, fp32=None):
+        """Device part of run(): trajectories from pre-drawn uniform numbers
+        ``draws`` (ntraj x max_draws, numpy or DeviceBuffer) -> (runs_expect,
+        info, col_times, col_which) as numpy arrays."""
+        if fp32 is None:
+            fp32 = self.options["precision"] == "fp32"
+        return self._launch(np.asarray(psi0).ravel(), tlist, e_ops, draws, max_jumps, fp32)
+
     def _launch(self, psi0, tlist, e_ops, draws, max_jumps, fp32):
         o = self.options
-        ntraj, max_draws = draws.shape
+        if isinstance(draws, tuple):  # (DeviceBuffer, ntraj, max_draws)
+            dr, ntraj, max_draws = draws
+        else:
+            ntraj, max_draws = draws.shape
+            dr = _data.DeviceBuffer.from_array(np.ascontiguousarray(draws))
         ntimes = len(tlist)
         _, h, c, tab = self._device_ops(fp32)
         e = _csr_block([O.to_csr() for O in e_ops] or [sp.identity(self.d)], self.d, fp32)
         neops = len(e_ops)
         times = _data.DeviceBuffer.from_array(np.asarray(tlist, dtype=np.float64))
         p0 = _data.DeviceBuffer.from_array(np.ascontiguousarray(psi0, dtype=np.complex128))
-        dr = _data.DeviceBuffer.from_array(np.ascontiguousarray(draws))
         ex = _data.DeviceBuffer(16 * ntraj * ntimes * max(neops, 1))
         ct = _data.DeviceBuffer(8 * ntraj * max_jumps)
         cw = _data.DeviceBuffer(4 * ntraj * max_jumps)
