@@ -151,7 +151,25 @@ int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *s
                  int dorder, const double *a, const double *b, const double *c,
                  const double *e, const double *bi, double rtol, double atol, int nsteps,
                  double first_step, double min_step, double max_step, int interpolate);
+/* The same engine for the Lindblad equation in matrix form (mesolve with
+ * options {"matrix_form": True}: qutip/core/cy/lindblad_matrix_form.pyx:
+ * 105-203): the state is rho, N x N column-major (N a multiple of 16), and
+ * drho/dt = -i H_nh rho + i rho H_nh^dag + sum_c c rho c^dag with
+ * H_nh = H - i/2 sum c^dag c (host array, N x N complex128 column-major);
+ * c_ops: nc such matrices back to back.  The products run on MFMA tiles:
+ * precision 0 = FP32 inputs and accumulation, 1 = each FP32 input split
+ * into bf16 hi + lo with three products (FP32 accumulation; N % 32 == 0).
+ * State, stage combines, error norm and outputs stay complex128 / FP64. */
+int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, const double *c_ops,
+                       int precision, int order, int rk_step, int rk_extra, int dorder,
+                       const double *a, const double *b, const double *c, const double *e,
+                       const double *bi, double rtol, double atol, int nsteps, double first_step,
+                       double min_step, double max_step, int interpolate);
 int qa_rk_destroy(qa_rk_solver *s);
+
+/* out = A x with the engine's generator (device vectors; waits for the
+ * result).  For the dense engine this is LindbladMatrixForm.matmul_data. */
+int qa_rk_apply(qa_rk_solver *s, const void *x, void *out);
 void *qa_rk_stream(qa_rk_solver *s);
 
 /* Output functionals evaluated at every output time: functional f has

@@ -7,7 +7,7 @@ the first device operation loads libqutip_amd.so and fails loudly when it or
 a GPU is missing.
 """
 from . import _lib
-from .core import (Qobj, QobjEvo, basis, ket2dm, destroy, create, num, qeye, sigmax, sigmay,
+from .core import (Qobj, QobjEvo, LindbladMatrixForm, basis, ket2dm, destroy, create, num, qeye, sigmax, sigmay,
                    sigmaz, sigmam, sigmap, tensor, spre, spost, liouvillian,
                    lindblad_dissipator, expect, rand_herm, rand_dm)
 from .data import HipDense, HipCSR

@@ -58,6 +58,9 @@ SIGNATURES = {
     "qa_expect_csr_dm_z": [i32, vp, vp, vp, i32, vp, P_f64, vp],
     "qa_rk_create": [C.POINTER(vp), i32, i32, i32, vp, vp, vp, i32, i32, i32, i32, i32,
                      P_f64, P_f64, P_f64, P_f64, P_f64, f64, f64, i32, f64, f64, f64, i32],
+    "qa_rk_create_dense": [C.POINTER(vp), i32, i32, P_f64, P_f64, i32, i32, i32, i32, i32,
+                           P_f64, P_f64, P_f64, P_f64, P_f64, f64, f64, i32, f64, f64, f64, i32],
+    "qa_rk_apply": [vp, vp, vp],
     "qa_rk_destroy": [vp],
     "qa_rk_stream": [vp],
     "qa_rk_set_functionals": [vp, i32, P_i32, P_i32, P_i32, P_i32, P_f64],

@@ -20,7 +20,7 @@ import scipy.sparse as sp
 
 from . import data as _data
 
-__all__ = ["Qobj", "QobjEvo", "basis", "ket2dm", "destroy", "create", "num", "qeye",
+__all__ = ["Qobj", "QobjEvo", "LindbladMatrixForm", "basis", "ket2dm", "destroy", "create", "num", "qeye",
            "sigmax", "sigmay", "sigmaz", "sigmam", "sigmap", "tensor", "spre", "spost",
            "liouvillian", "lindblad_dissipator", "expect", "rand_herm", "rand_dm"]
 
@@ -402,3 +402,49 @@ class QobjEvo:
     def __matmul__(self, other):
         other = other.qobj if isinstance(other, QobjEvo) else other
         return QobjEvo(self.qobj @ other)
+
+
+class LindbladMatrixForm:
+    """Lindblad right-hand side in matrix form (qutip/core/cy/
+    lindblad_matrix_form.pyx:59-203): drho/dt = -i H_nh rho + i rho H_nh^dag
+    + sum_c c rho c^dag with H_nh = H - i/2 sum_c c^dag c (:92-97).  The
+    state is rho itself (N x N, not vectorised).  ``matmul_data`` runs the
+    MFMA product of the dense engine; the hip_rk integrator picks up
+    ``dense_operators`` and runs the whole step loop on the device."""
+
+    issuper = False
+    isconstant = True
+
+    def __init__(self, H, c_ops, precision="fp32"):
+        if isinstance(H, QobjEvo):
+            H = H.qobj
+        self.H = H
+        self.c_ops = [c.qobj if isinstance(c, QobjEvo) else c for c in (c_ops or [])]
+        self.dims = H.dims
+        self.shape = H.shape
+        self.num_collapse = len(self.c_ops)
+        Hnh = H.full().astype(np.complex128)
+        for cop in self.c_ops:
+            cm = cop.full()
+            Hnh = Hnh - 0.5j * (cm.conj().T @ cm)
+        self.H_nh = Hnh
+        self.precision = precision
+        self._engine = None
+
+    def dense_operators(self):
+        return self.H_nh, [c.full() for c in self.c_ops]
+
+    def __call__(self, t=0.0, args=None):
+        raise TypeError("LindbladMatrixForm cannot be called as a Qobj "
+                        "(lindblad_matrix_form.pyx:217-222)")
+
+    def matmul_data(self, t, rho, out=None):
+        """out += L(rho)  (matmul_data, lindblad_matrix_form.pyx:105-203)."""
+        if self._engine is None:
+            from .solver.hip_rk import RKEngine
+            self._engine = RKEngine(None, dense=self.dense_operators(),
+                                    dense_precision=self.precision)
+        res = self._engine.apply(rho)
+        if out is None:
+            return res
+        return _data.iadd(out, res, 1.0)

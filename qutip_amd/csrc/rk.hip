@@ -188,12 +188,10 @@ __device__ void after_attempt(RKCtl *c, double err) {
 enum Input { IN_YPREV = 0, IN_YFRONT = 1, IN_X = 2 };
 enum Epi { EPI_NONE = 0, EPI_NEXT = 1, EPI_FINAL = 2 };
 
-template <typename V>
-struct StageArgs {
-    const int *slice_ptr;
-    const int *col;
-    const V *vals;
-    int nrows, nslices;
+// Fields of a stage launch shared by the sparse and the dense stage kernels:
+// where the stage vectors live and the stage's combine coefficients.
+struct EpiArgs {
+    int nrows;
     int64_t npad;
     double2 *y0, *y1, *x0, *x1, *K;
     double *partials;
@@ -204,17 +202,82 @@ struct StageArgs {
     double ecoef[kMaxStages];
 };
 
+template <typename V>
+struct StageArgs : EpiArgs {
+    const int *slice_ptr;
+    const int *col;
+    const V *vals;
+    int nslices;
+};
+
+// The vectors a stage reads and writes, resolved from the control block.
+struct StageVecs {
+    double2 *yprev, *yfront;
+    const double2 *xin;
+    double dt;
+};
+
+__device__ __forceinline__ StageVecs stage_vecs(const EpiArgs &P, const RKCtl *ctl) {
+    StageVecs v;
+    const int cur = ctl->cur;
+    v.yprev = cur ? P.y1 : P.y0;
+    v.yfront = cur ? P.y0 : P.y1;
+    v.dt = (P.run_phase == PH_STEP) ? ctl->dt_try : ctl->dt_int;
+    v.xin = P.input == IN_YPREV ? v.yprev : P.input == IN_YFRONT ? v.yfront
+                                                                   : ((P.stage & 1) ? P.x1 : P.x0);
+    return v;
+}
+
+// Epilogue, part 1 (before the product is known): o = y_prev + dt sum_{j <
+// stage} coef_j k_j and er = dt sum_{j < stage} ecoef_j k_j for element r.
+__device__ __forceinline__ void epi_pre(const EpiArgs &P, const StageVecs &v, int64_t r,
+                                        double2 &o, double2 &er) {
+    o = make_double2(0.0, 0.0);
+    er = make_double2(0.0, 0.0);
+    if (P.epi == EPI_NONE) return;
+    o = v.yprev[r];
+#pragma unroll
+    for (int j = 0; j < kMaxStages; ++j) {
+        if (j < P.stage) {
+            const double fb = v.dt * P.coef[j];
+            const double fe = v.dt * P.ecoef[j];
+            if (fb != 0.0 || fe != 0.0) {
+                const double2 kj = P.K[(int64_t)j * P.npad + r];
+                if (fb != 0.0) raxpy(o, fb, kj);
+                if (fe != 0.0) raxpy(er, fe, kj);
+            }
+        }
+    }
+}
+
+// Epilogue, part 2: store k_stage, then the next stage input or y_front and
+// the squared weighted error of element r (returned).
+__device__ __forceinline__ double epi_post(const EpiArgs &P, const RKCtl *ctl, const StageVecs &v,
+                                           int64_t r, double2 acc, double2 o, double2 er) {
+    P.K[(int64_t)P.stage * P.npad + r] = acc;
+    const double fb = v.dt * P.coef_self;
+    const double fe = v.dt * P.ecoef_self;
+    if (P.epi == EPI_NEXT) {
+        if (fb != 0.0) raxpy(o, fb, acc);
+        double2 *xout = ((P.stage + 1) & 1) ? P.x1 : P.x0;
+        xout[r] = o;
+    } else if (P.epi == EPI_FINAL) {
+        if (fb != 0.0) raxpy(o, fb, acc);
+        if (fe != 0.0) raxpy(er, fe, acc);
+        v.yfront[r] = o;
+        if (r < P.nrows) {
+            const double q = sqrt(cabs2(er)) / (ctl->atol + ctl->rtol * sqrt(cabs2(o)));
+            return q * q;
+        }
+    }
+    return 0.0;
+}
+
 template <typename V, int L>
 __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
     RKCtl *ctl = P.ctl;
     if (ctl->phase != P.run_phase) return;  // uniform over the whole grid
-    const int cur = ctl->cur;
-    double2 *yprev = cur ? P.y1 : P.y0;
-    double2 *yfront = cur ? P.y0 : P.y1;
-    const double dt = (P.run_phase == PH_STEP) ? ctl->dt_try : ctl->dt_int;
-    const double2 *xin = P.input == IN_YPREV    ? yprev
-                         : P.input == IN_YFRONT ? yfront
-                         : ((P.stage & 1) ? P.x1 : P.x0);
+    const StageVecs v = stage_vecs(P, ctl);
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int slice = (int)(t / QA_WAVE), lane = (int)(t % QA_WAVE);
     const int64_t r = (int64_t)slice * (QA_WAVE / L) + lane / L;  // the row of this lane
@@ -224,46 +287,191 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
         // first, so their loads are in flight with the product's.
         const bool owner = (lane % L == 0);
         double2 o = make_double2(0.0, 0.0), er = make_double2(0.0, 0.0);
-        if (owner && P.epi != EPI_NONE) {
-            o = yprev[r];
-#pragma unroll
-            for (int j = 0; j < kMaxStages; ++j) {
-                if (j < P.stage) {
-                    const double fb = dt * P.coef[j];
-                    const double fe = dt * P.ecoef[j];
-                    if (fb != 0.0 || fe != 0.0) {
-                        const double2 kj = P.K[(int64_t)j * P.npad + r];
-                        if (fb != 0.0) raxpy(o, fb, kj);
-                        if (fe != 0.0) raxpy(er, fe, kj);
-                    }
-                }
-            }
-        }
-        const double2 acc = sell_row_dot<L>(P.slice_ptr, P.col, P.vals, xin, slice, lane);
-        if (owner) {
-            P.K[(int64_t)P.stage * P.npad + r] = acc;
-            const double fb = dt * P.coef_self;
-            const double fe = dt * P.ecoef_self;
-            if (P.epi == EPI_NEXT) {
-                if (fb != 0.0) raxpy(o, fb, acc);
-                double2 *xout = ((P.stage + 1) & 1) ? P.x1 : P.x0;
-                xout[r] = o;
-            } else if (P.epi == EPI_FINAL) {
-                if (fb != 0.0) raxpy(o, fb, acc);
-                if (fe != 0.0) raxpy(er, fe, acc);
-                yfront[r] = o;
-                if (r < P.nrows) {
-                    const double q = sqrt(cabs2(er)) / (ctl->atol + ctl->rtol * sqrt(cabs2(o)));
-                    part = q * q;
-                }
-            }
-        }
+        if (owner) epi_pre(P, v, r, o, er);
+        const double2 acc = sell_row_dot<L>(P.slice_ptr, P.col, P.vals, v.xin, slice, lane);
+        if (owner) part = epi_post(P, ctl, v, r, acc, o, er);
     }
     if (P.epi == EPI_FINAL) {
         __shared__ double smem[kStepBlocks / QA_WAVE];
         const double s = block_sum(part, smem);
         if (threadIdx.x == 0) P.partials[blockIdx.x] = s;
     }
+}
+
+// ---- dense Lindblad right-hand side on MFMA ----------------------------------
+// drho/dt = -i H_nh rho + i rho H_nh^dag + sum_c c rho c^dag with
+// H_nh = H - i/2 sum c^dag c (qutip/core/cy/lindblad_matrix_form.pyx:92-97,
+// 189-201), rho column-major N x N (the vector buffers of the engine).
+// Per stage: T_c = c X for every collapse operator, then one fused kernel
+// k = Hm X + X Hp + sum_c T_c Cd_c with Hm = -i H_nh, Hp = i H_nh^dag,
+// Cd_c = c^dag, followed by the stage epilogue.  Complex products use the
+// real 4M decomposition on 16x16 MFMA tiles (one wavefront per tile):
+// FP32 inputs and accumulation (v_mfma_f32_16x16x4f32), or each FP32 input
+// split into bf16 hi + lo with the three significant products hi*hi +
+// hi*lo + lo*hi (v_mfma_f32_16x16x32_bf16, FP32 accumulation).
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+struct DenseOp {
+    int N, nc, prec;  // prec 0: FP32 MFMA, 1: bf16 x3 split
+    const float2 *Hm, *Hp, *C, *Cd;  // N x N each; C / Cd: nc blocks
+    float2 *T;                        // nc blocks of scratch
+};
+
+__device__ __forceinline__ float2 tof(double2 a) { return make_float2((float)a.x, (float)a.y); }
+__device__ __forceinline__ float2 tof(float2 a) { return a; }
+
+// re + i im += A[i0:i0+16, :] B[:, j0:j0+16]  (FP32 MFMA, K = N)
+template <typename TA, typename TB>
+__device__ __forceinline__ void zmma_f32(v4f &re, v4f &im, const TA *__restrict__ A,
+                                         const TB *__restrict__ B, int N, int i0, int j0,
+                                         int lane) {
+    const int r = lane & 15, kq = lane >> 4;
+    const TA *pa = A + (size_t)kq * N + i0 + r;
+    const TB *pb = B + (size_t)(j0 + r) * N + kq;
+    for (int kk = 0; kk < N; kk += 16) {
+        float2 a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] = tof(pa[(size_t)(kk + 4 * u) * N]);
+            b[u] = tof(pb[kk + 4 * u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            re = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].x, re, 0, 0, 0);
+            re = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[u].y, b[u].y, re, 0, 0, 0);
+            im = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].y, im, 0, 0, 0);
+            im = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[u].x, im, 0, 0, 0);
+        }
+    }
+}
+
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+// Same product with bf16 x3 splitting (K = N, N a multiple of 32).
+template <typename TA, typename TB>
+__device__ __forceinline__ void zmma_bf16x3(v4f &re, v4f &im, const TA *__restrict__ A,
+                                            const TB *__restrict__ B, int N, int i0, int j0,
+                                            int lane) {
+    const int r = lane & 15, kq = lane >> 4;
+    for (int kk = 0; kk < N; kk += 32) {
+        v8bf arh, arl, aih, ail, naih, nail, brh, brl, bih, bil;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = kk + 8 * kq + u;
+            const float2 a = tof(A[(size_t)k * N + i0 + r]);
+            const float2 b = tof(B[(size_t)(j0 + r) * N + k]);
+            __bf16 h, l;
+            split_bf16(a.x, h, l);
+            arh[u] = h;
+            arl[u] = l;
+            split_bf16(a.y, h, l);
+            aih[u] = h;
+            ail[u] = l;
+            naih[u] = -h;
+            nail[u] = -l;
+            split_bf16(b.x, h, l);
+            brh[u] = h;
+            brl[u] = l;
+            split_bf16(b.y, h, l);
+            bih[u] = h;
+            bil[u] = l;
+        }
+        // re += ar br - ai bi ; im += ar bi + ai br   (hi*hi + hi*lo + lo*hi each)
+        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brh, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brl, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, brh, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bih, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bil, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail, bih, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bih, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bil, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, bih, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brh, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brl, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, brh, im, 0, 0, 0);
+    }
+}
+
+template <typename TA, typename TB>
+__device__ __forceinline__ void zmma(const DenseOp &D, v4f &re, v4f &im, const TA *A, const TB *B,
+                                     int i0, int j0, int lane) {
+    if (D.prec == 1)
+        zmma_bf16x3(re, im, A, B, D.N, i0, j0, lane);
+    else
+        zmma_f32(re, im, A, B, D.N, i0, j0, lane);
+}
+
+// Element (i0 + 4 (lane / 16) + v, j0 + lane % 16) of a 16 x 16 MFMA tile
+// (the accumulator layout of the 16x16 instructions), as a column-major
+// index.
+__device__ __forceinline__ int64_t tile_index(int N, int i0, int j0, int lane, int v) {
+    return (int64_t)(j0 + (lane & 15)) * N + i0 + 4 * (lane >> 4) + v;
+}
+
+// T_c = c X for every collapse operator (grid: N/16, N/16, nc).
+__device__ __forceinline__ void dense_t_tile(const DenseOp &D, const double2 *X, int lane) {
+    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16, c = blockIdx.z;
+    const size_t NN = (size_t)D.N * D.N;
+    v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
+    zmma(D, re, im, D.C + c * NN, X, i0, j0, lane);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) D.T[c * NN + tile_index(D.N, i0, j0, lane, v)] = make_float2(re[v], im[v]);
+}
+
+// k tile = Hm X + X Hp + sum_c T_c Cd_c
+__device__ __forceinline__ void dense_k_tile(const DenseOp &D, const double2 *X, int lane, v4f &re,
+                                             v4f &im) {
+    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+    const size_t NN = (size_t)D.N * D.N;
+    zmma(D, re, im, D.Hm, X, i0, j0, lane);
+    zmma(D, re, im, X, D.Hp, i0, j0, lane);
+    for (int c = 0; c < D.nc; ++c) zmma(D, re, im, D.T + c * NN, D.Cd + c * NN, i0, j0, lane);
+}
+
+__global__ __launch_bounds__(64) void rk_dense_t_kernel(EpiArgs P, DenseOp D) {
+    if (P.ctl->phase != P.run_phase) return;
+    const StageVecs v = stage_vecs(P, P.ctl);
+    dense_t_tile(D, v.xin, threadIdx.x);
+}
+
+__global__ __launch_bounds__(64) void rk_dense_stage_kernel(EpiArgs P, DenseOp D) {
+    RKCtl *ctl = P.ctl;
+    if (ctl->phase != P.run_phase) return;
+    const StageVecs v = stage_vecs(P, ctl);
+    const int lane = threadIdx.x;
+    double2 o[4], er[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        epi_pre(P, v, tile_index(D.N, blockIdx.x * 16, blockIdx.y * 16, lane, q), o[q], er[q]);
+    v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
+    dense_k_tile(D, v.xin, lane, re, im);
+    double part = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        part += epi_post(P, ctl, v, tile_index(D.N, blockIdx.x * 16, blockIdx.y * 16, lane, q),
+                         make_double2((double)re[q], (double)im[q]), o[q], er[q]);
+    if (P.epi == EPI_FINAL) {
+        part = wave_sum(part);
+        if (lane == 0) P.partials[blockIdx.y * gridDim.x + blockIdx.x] = part;
+    }
+}
+
+// Ungated versions: out = L(X) (the data-layer product of the matrix form).
+__global__ __launch_bounds__(64) void dense_t_kernel(DenseOp D, const double2 *X) {
+    dense_t_tile(D, X, threadIdx.x);
+}
+
+__global__ __launch_bounds__(64) void dense_rhs_kernel(DenseOp D, const double2 *X, double2 *out) {
+    v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
+    dense_k_tile(D, X, threadIdx.x, re, im);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        out[tile_index(D.N, blockIdx.x * 16, blockIdx.y * 16, threadIdx.x, q)] =
+            make_double2((double)re[q], (double)im[q]);
 }
 
 // Error norm and step control after the last stage: one workgroup sums the
@@ -505,7 +713,11 @@ __global__ void rk_set_phase_kernel(RKCtl *c, int phase) { c->phase = phase; }
 // host-side engine object
 // ---------------------------------------------------------------------------
 struct qa_rk_solver {
+    int kind = 0;  // 0: sparse SELL-C generator, 1: dense Lindblad (matrix form)
     int n = 0, nslices = 0, vals_fp32 = 0, lanes = 1;
+    int nparts = 0;      // per-workgroup error partials of the last stage
+    DenseOp dop = {};    // kind 1: operators in HBM (complex64)
+    void *dense_block = nullptr;
     int64_t npad = 0;
     const int *slice_ptr = nullptr;
     const int *col = nullptr;
@@ -545,14 +757,8 @@ struct qa_rk_solver {
 
 namespace {
 
-template <typename V>
-StageArgs<V> stage_args(qa_rk_solver *S, int i) {
-    StageArgs<V> P;
-    P.slice_ptr = S->slice_ptr;
-    P.col = S->col;
-    P.vals = (const V *)S->vals;
+void fill_epi(EpiArgs &P, qa_rk_solver *S, int i) {
     P.nrows = S->n;
-    P.nslices = S->nslices;
     P.npad = S->npad;
     P.y0 = S->y0;
     P.y1 = S->y1;
@@ -591,6 +797,18 @@ StageArgs<V> stage_args(qa_rk_solver *S, int i) {
             P.nterms = 0;
         }
     }
+    P.coef_self = P.coef[i];
+    P.ecoef_self = P.ecoef[i];
+}
+
+template <typename V>
+StageArgs<V> stage_args(qa_rk_solver *S, int i) {
+    StageArgs<V> P;
+    fill_epi(P, S, i);
+    P.slice_ptr = S->slice_ptr;
+    P.col = S->col;
+    P.vals = (const V *)S->vals;
+    P.nslices = S->nslices;
     return P;
 }
 
@@ -600,10 +818,8 @@ int stage_blocks(const qa_rk_solver *S) {
 }
 
 template <typename V>
-int launch_stage(qa_rk_solver *S, int i) {
+int launch_sparse_stage(qa_rk_solver *S, int i) {
     StageArgs<V> P = stage_args<V>(S, i);
-    P.coef_self = P.coef[i];
-    P.ecoef_self = P.ecoef[i];
     const dim3 grid((unsigned)stage_blocks(S)), block(kStepBlocks);
     switch (S->lanes) {
         case 1: hipLaunchKernelGGL((rk_stage_kernel<V, 1>), grid, block, 0, S->stream, P); break;
@@ -616,6 +832,42 @@ int launch_stage(qa_rk_solver *S, int i) {
     }
     QA_CHECK_LAUNCH();
     return QA_OK;
+}
+
+int launch_dense_stage(qa_rk_solver *S, int i) {
+    EpiArgs P;
+    fill_epi(P, S, i);
+    const int nt = S->dop.N / 16;
+    if (S->dop.nc > 0) {
+        hipLaunchKernelGGL(rk_dense_t_kernel, dim3(nt, nt, S->dop.nc), dim3(64), 0, S->stream, P,
+                           S->dop);
+        QA_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(rk_dense_stage_kernel, dim3(nt, nt), dim3(64), 0, S->stream, P, S->dop);
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
+int launch_stage(qa_rk_solver *S, int i) {
+    if (S->kind == 1) return launch_dense_stage(S, i);
+    return S->vals_fp32 ? launch_sparse_stage<float2>(S, i) : launch_sparse_stage<double2>(S, i);
+}
+
+// out = A x (ungated): the derivative for the first-step estimate.
+int engine_rhs(qa_rk_solver *S, const double2 *x, double2 *out) {
+    if (S->kind == 1) {
+        const int nt = S->dop.N / 16;
+        if (S->dop.nc > 0) {
+            hipLaunchKernelGGL(dense_t_kernel, dim3(nt, nt, S->dop.nc), dim3(64), 0, S->stream,
+                               S->dop, x);
+            QA_CHECK_LAUNCH();
+        }
+        hipLaunchKernelGGL(dense_rhs_kernel, dim3(nt, nt), dim3(64), 0, S->stream, S->dop, x, out);
+        QA_CHECK_LAUNCH();
+        return QA_OK;
+    }
+    return qa_sell_matvec_z(S->n, S->lanes, S->nslices, S->slice_ptr, S->col, S->vals,
+                            S->vals_fp32, x, out, 1.0, 0.0, 0, S->stream);
 }
 
 OutArgs out_args(qa_rk_solver *S) {
@@ -646,11 +898,11 @@ OutArgs out_args(qa_rk_solver *S) {
 int enqueue_iteration(qa_rk_solver *S) {
     int rc;
     for (int i = 0; i < S->rk_step; ++i) {
-        rc = S->vals_fp32 ? launch_stage<float2>(S, i) : launch_stage<double2>(S, i);
+        rc = launch_stage(S, i);
         if (rc) return rc;
     }
     hipLaunchKernelGGL(rk_control_kernel, dim3(1), dim3(256), 0, S->stream, S->ctl, S->partials,
-                       stage_blocks(S), S->n);
+                       S->nparts, S->n);
     QA_CHECK_LAUNCH();
     if (S->interpolate && S->rk_extra > S->rk_step) {
         if (!S->first_dense_from_front) {
@@ -665,7 +917,7 @@ int enqueue_iteration(qa_rk_solver *S) {
             QA_CHECK_LAUNCH();
         }
         for (int i = S->rk_step; i < S->rk_extra; ++i) {
-            rc = S->vals_fp32 ? launch_stage<float2>(S, i) : launch_stage<double2>(S, i);
+            rc = launch_stage(S, i);
             if (rc) return rc;
         }
     }
@@ -706,7 +958,7 @@ int rebuild_graph(qa_rk_solver *S) {
 void free_all(qa_rk_solver *S) {
     if (S->exec) (void)hipGraphExecDestroy(S->exec);
     if (S->graph) (void)hipGraphDestroy(S->graph);
-    void *ptrs[] = {S->block, S->ctl, S->bi_dev, S->times_dev, S->fun_out_dev, S->fptr_dev,
+    void *ptrs[] = {S->block, S->dense_block, S->ctl, S->bi_dev, S->times_dev, S->fun_out_dev, S->fptr_dev,
                     S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -718,36 +970,17 @@ void free_all(qa_rk_solver *S) {
 
 extern "C" {
 
-int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *slice_ptr,
-                 const int *col, const void *vals, int vals_fp32, int order, int rk_step,
-                 int rk_extra,
-                 int dorder, const double *a, const double *b, const double *c,
-                 const double *e, const double *bi, double rtol, double atol, int nsteps,
-                 double first_step, double min_step, double max_step, int interpolate) {
-    QA_REQUIRE(out != nullptr, "qa_rk_create: null output");
-    *out = nullptr;
-    QA_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8,
-               "qa_rk_create: lanes must be 1, 2, 4 or 8");
-    QA_REQUIRE(n > 0 && nslices == (n + QA_WAVE / lanes - 1) / (QA_WAVE / lanes),
-               "qa_rk_create: need n > 0 and nslices = ceil(n / (64 / lanes))");
-    QA_REQUIRE(slice_ptr && col && vals, "qa_rk_create: null operator pointer");
-    QA_REQUIRE(rk_step >= 1 && rk_step <= kMaxStages && rk_extra >= rk_step &&
-                   rk_extra <= kMaxStages,
-               "qa_rk_create: 1 <= rk_step <= rk_extra <= %d", kMaxStages);
-    QA_REQUIRE(a && b && c, "qa_rk_create: tableau a, b, c required");
-    QA_REQUIRE(!bi || (dorder >= 1 && dorder <= kMaxDenseOrder),
-               "qa_rk_create: dense-output order must be 1..%d", kMaxDenseOrder);
-    QA_REQUIRE(nsteps >= 0, "qa_rk_create: nsteps must be >= 0");
-    qa_rk_solver *S = new (std::nothrow) qa_rk_solver();
-    QA_REQUIRE(S != nullptr, "qa_rk_create: out of host memory");
-    S->n = n;
-    S->lanes = lanes;
-    S->nslices = nslices;
-    S->npad = (int64_t)nslices * (QA_WAVE / lanes);
-    S->slice_ptr = slice_ptr;
-    S->col = col;
-    S->vals = vals;
-    S->vals_fp32 = vals_fp32;
+}  // extern "C"
+
+namespace {
+
+// Tableau, options, stream, vectors and control block of a new engine whose
+// operator-specific fields (kind, n, npad, nparts, operator pointers) are
+// already set.  Frees the engine and returns the error code on failure.
+int rk_finish_create(qa_rk_solver *S, int order, int rk_step, int rk_extra, int dorder,
+                     const double *a, const double *b, const double *c, const double *e,
+                     const double *bi, double rtol, double atol, int nsteps, double first_step,
+                     double min_step, double max_step, int interpolate) {
     S->order = order;
     S->rk_step = rk_step;
     S->rk_extra = rk_extra;
@@ -786,13 +1019,16 @@ int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *s
         delete S;
         return code;
     };
-    hipError_t he = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
-    if (he != hipSuccess) return fail(qa::hip_fail(he, "hipStreamCreate", __FILE__, __LINE__));
+    hipError_t he;
+    if (!S->stream) {
+        he = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+        if (he != hipSuccess) return fail(qa::hip_fail(he, "hipStreamCreate", __FILE__, __LINE__));
+    }
     const int nk = S->interpolate ? rk_extra : rk_step;
     const int64_t nvec = 6 + nk;  // y0 y1 x0 x1 y_last tmp + K
     const size_t vbytes = (size_t)S->npad * sizeof(double2);
-    const size_t nblocks_step = (size_t)stage_blocks(S) + 1;
-    size_t bytes = vbytes * nvec + nblocks_step * sizeof(double) + 256;
+    const size_t nparts = (size_t)S->nparts + 1;
+    size_t bytes = vbytes * nvec + nparts * sizeof(double) + 256;
     he = hipMalloc(&S->block, bytes);
     if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMalloc(rk vectors)", __FILE__, __LINE__));
     char *p = (char *)S->block;
@@ -804,7 +1040,7 @@ int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *s
     S->tmp = (double2 *)p; p += vbytes;
     S->K = (double2 *)p; p += vbytes * nk;
     S->partials = (double *)p;
-    rc = qa_fill_zero(S->block, bytes - 256 - nblocks_step * sizeof(double), S->stream);
+    rc = qa_fill_zero(S->block, bytes - 256 - nparts * sizeof(double), S->stream);
     if (rc) return fail(rc);
     he = hipMalloc((void **)&S->ctl, sizeof(RKCtl));
     if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMalloc(ctl)", __FILE__, __LINE__));
@@ -824,20 +1060,133 @@ int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *s
     init.phase = PH_IDLE;
     he = hipMemcpyAsync(S->ctl, &init, sizeof(init), hipMemcpyHostToDevice, S->stream);
     if (he != hipSuccess) return fail(qa::hip_fail(he, "ctl upload", __FILE__, __LINE__));
-    if (S->interpolate) {
-        he = hipMalloc((void **)&S->bi_dev, S->bi.size() * sizeof(double));
-        if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMalloc(bi)", __FILE__, __LINE__));
-        he = hipMemcpyAsync(S->bi_dev, S->bi.data(), S->bi.size() * sizeof(double),
-                            hipMemcpyHostToDevice, S->stream);
-        if (he != hipSuccess) return fail(qa::hip_fail(he, "bi upload", __FILE__, __LINE__));
-    }
     he = hipStreamSynchronize(S->stream);
     if (he != hipSuccess) return fail(qa::hip_fail(he, "sync", __FILE__, __LINE__));
     // no functionals until set
     int zero_ptr[2] = {0, 0};
     rc = qa_rk_set_functionals(S, 0, nullptr, zero_ptr, nullptr, nullptr, nullptr);
     if (rc) return fail(rc);
+    return QA_OK;
+}
+
+int check_tableau(int rk_step, int rk_extra, int dorder, const double *a, const double *b,
+                  const double *c, const double *bi, int nsteps) {
+    QA_REQUIRE(rk_step >= 1 && rk_step <= kMaxStages && rk_extra >= rk_step &&
+                   rk_extra <= kMaxStages,
+               "1 <= rk_step <= rk_extra <= %d", kMaxStages);
+    QA_REQUIRE(a && b && c, "tableau a, b, c required");
+    QA_REQUIRE(!bi || (dorder >= 1 && dorder <= kMaxDenseOrder),
+               "dense-output order must be 1..%d", kMaxDenseOrder);
+    QA_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
+    return QA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *slice_ptr,
+                 const int *col, const void *vals, int vals_fp32, int order, int rk_step,
+                 int rk_extra, int dorder, const double *a, const double *b, const double *c,
+                 const double *e, const double *bi, double rtol, double atol, int nsteps,
+                 double first_step, double min_step, double max_step, int interpolate) {
+    QA_REQUIRE(out != nullptr, "qa_rk_create: null output");
+    *out = nullptr;
+    QA_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8,
+               "qa_rk_create: lanes must be 1, 2, 4 or 8");
+    QA_REQUIRE(n > 0 && nslices == (n + QA_WAVE / lanes - 1) / (QA_WAVE / lanes),
+               "qa_rk_create: need n > 0 and nslices = ceil(n / (64 / lanes))");
+    QA_REQUIRE(slice_ptr && col && vals, "qa_rk_create: null operator pointer");
+    int rc = check_tableau(rk_step, rk_extra, dorder, a, b, c, bi, nsteps);
+    if (rc) return rc;
+    qa_rk_solver *S = new (std::nothrow) qa_rk_solver();
+    QA_REQUIRE(S != nullptr, "qa_rk_create: out of host memory");
+    S->kind = 0;
+    S->n = n;
+    S->lanes = lanes;
+    S->nslices = nslices;
+    S->npad = (int64_t)nslices * (QA_WAVE / lanes);
+    S->nparts = stage_blocks(S);
+    S->slice_ptr = slice_ptr;
+    S->col = col;
+    S->vals = vals;
+    S->vals_fp32 = vals_fp32;
+    rc = rk_finish_create(S, order, rk_step, rk_extra, dorder, a, b, c, e, bi, rtol, atol, nsteps,
+                          first_step, min_step, max_step, interpolate);
+    if (rc) return rc;
     *out = S;
+    return QA_OK;
+}
+
+int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, const double *c_ops,
+                       int precision, int order, int rk_step, int rk_extra, int dorder,
+                       const double *a, const double *b, const double *c, const double *e,
+                       const double *bi, double rtol, double atol, int nsteps, double first_step,
+                       double min_step, double max_step, int interpolate) {
+    QA_REQUIRE(out != nullptr, "qa_rk_create_dense: null output");
+    *out = nullptr;
+    QA_REQUIRE(N >= 16 && N % 16 == 0, "qa_rk_create_dense: N must be a positive multiple of 16");
+    QA_REQUIRE(precision == 0 || precision == 1, "qa_rk_create_dense: precision 0 (fp32) or 1 (bf16x3)");
+    QA_REQUIRE(precision == 0 || N % 32 == 0, "qa_rk_create_dense: bf16x3 needs N % 32 == 0");
+    QA_REQUIRE(nc >= 0 && nc <= 64 && H_nh && (nc == 0 || c_ops), "qa_rk_create_dense: operators");
+    int rc = check_tableau(rk_step, rk_extra, dorder, a, b, c, bi, nsteps);
+    if (rc) return rc;
+    qa_rk_solver *S = new (std::nothrow) qa_rk_solver();
+    QA_REQUIRE(S != nullptr, "qa_rk_create_dense: out of host memory");
+    S->kind = 1;
+    S->n = N * N;
+    S->npad = (int64_t)N * N;
+    S->nparts = (N / 16) * (N / 16);
+    // Hm = -i H_nh, Hp = i H_nh^dag, C, Cd = C^dag, as complex64, column-major
+    const size_t NN = (size_t)N * N;
+    std::vector<float2> h((2 + 2 * (size_t)nc) * NN);
+    float2 *Hm = h.data(), *Hp = Hm + NN, *Cs = Hp + NN, *Cds = Cs + nc * NN;
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i) {
+            const double re = H_nh[2 * ((size_t)j * N + i)], im = H_nh[2 * ((size_t)j * N + i) + 1];
+            Hm[(size_t)j * N + i] = make_float2((float)im, (float)-re);  // -i (re + i im)
+            // (H_nh^dag)[j][i] = conj(H_nh[i][j]); times i
+            Hp[(size_t)i * N + j] = make_float2((float)im, (float)re);   // i (re - i im)
+        }
+    for (int k = 0; k < nc; ++k)
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) {
+                const double *src = c_ops + 2 * (k * NN + (size_t)j * N + i);
+                Cs[k * NN + (size_t)j * N + i] = make_float2((float)src[0], (float)src[1]);
+                Cds[k * NN + (size_t)i * N + j] = make_float2((float)src[0], (float)-src[1]);
+            }
+    hipError_t he = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+    if (he == hipSuccess)
+        he = hipMalloc(&S->dense_block, sizeof(float2) * ((2 + 3 * (size_t)nc) * NN + 1));
+    if (he == hipSuccess)
+        he = hipMemcpy(S->dense_block, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+        free_all(S);
+        delete S;
+        return qa::hip_fail(he, "dense operator upload", __FILE__, __LINE__);
+    }
+    float2 *d = (float2 *)S->dense_block;
+    S->dop.N = N;
+    S->dop.nc = nc;
+    S->dop.prec = precision;
+    S->dop.Hm = d;
+    S->dop.Hp = d + NN;
+    S->dop.C = d + 2 * NN;
+    S->dop.Cd = d + (2 + (size_t)nc) * NN;
+    S->dop.T = d + (2 + 2 * (size_t)nc) * NN;
+    rc = rk_finish_create(S, order, rk_step, rk_extra, dorder, a, b, c, e, bi, rtol, atol, nsteps,
+                          first_step, min_step, max_step, interpolate);
+    if (rc) return rc;
+    *out = S;
+    return QA_OK;
+}
+
+// out = L(x) with the engine's generator (device vectors of n entries).
+int qa_rk_apply(qa_rk_solver *S, const void *x, void *out) {
+    QA_REQUIRE(S && x && out, "qa_rk_apply: null argument");
+    int rc = engine_rhs(S, (const double2 *)x, (double2 *)out);
+    if (rc) return rc;
+    QA_HIP(hipStreamSynchronize(S->stream));
     return QA_OK;
 }
 
@@ -924,8 +1273,7 @@ int qa_rk_set_state(qa_rk_solver *S, double t0, const void *y0_dev) {
         double norm = sqrt(norm2);
         const double tol = S->atol + norm * S->rtol;
         double2 *k0 = S->K, *k1 = S->K + S->npad;
-        rc = qa_sell_matvec_z(S->n, S->lanes, S->nslices, S->slice_ptr, S->col, S->vals,
-                              S->vals_fp32, S->y1, k0, 1.0, 0.0, 0, s);
+        rc = engine_rhs(S, S->y1, k0);
         if (rc) return rc;
         if (norm <= S->atol) norm = 1.0;
         rc = qa_norm2_z(S->n, k0, &kn2, s);
@@ -942,8 +1290,7 @@ int qa_rk_set_state(qa_rk_solver *S, double t0, const void *y0_dev) {
         const double cf[1] = {dt1 / 100};
         rc = qa_lincomb_z(S->npad, S->y1, 1, vecs, cf, S->tmp, s);
         if (rc) return rc;
-        rc = qa_sell_matvec_z(S->n, S->lanes, S->nslices, S->slice_ptr, S->col, S->vals,
-                              S->vals_fp32, S->tmp, k1, 1.0, 0.0, 0, s);
+        rc = engine_rhs(S, S->tmp, k1);
         if (rc) return rc;
         rc = qa_norm2_z(S->n, k1, &kn2, s);
         if (rc) return rc;
@@ -1126,7 +1473,7 @@ int qa_rk_profile_stages(qa_rk_solver *S, int reps, double *ms_per_stage) {
         for (int r = 0; r < reps && rc == QA_OK; ++r) {
             hipLaunchKernelGGL(rk_set_phase_kernel, dim3(1), dim3(1), 0, s, S->ctl, phase);
             QA_HIP(hipEventRecord(e0, s));
-            rc = S->vals_fp32 ? launch_stage<float2>(S, i) : launch_stage<double2>(S, i);
+            rc = launch_stage(S, i);
             QA_HIP(hipEventRecord(e1, s));
             QA_HIP(hipEventSynchronize(e1));
             float ms = 0.f;

@@ -14,8 +14,9 @@ explicit_rk.pyx, tableaus vern7 / tsit5):
 
 Options are those of IntegratorVern7 (qutip/solver/integrator/
 qutip_integrator.py:52-60) plus ``tableau`` ('vern7' | 'tsit5'), ``fused``
-(bool) and ``operator_precision`` ('fp64' | 'fp32': complex64 operator
-values, FP64 state, accumulation and error norm).
+(bool), ``operator_precision`` ('fp64' | 'fp32': complex64 operator
+values, FP64 state, accumulation and error norm) and ``dense_precision``
+('fp32' | 'bf16x3': the MFMA products of the matrix-form Lindblad RHS).
 """
 import ctypes as C
 
@@ -56,38 +57,60 @@ class RKEngine:
     """Owner of one ``qa_rk_solver`` (fused device RK for dy/dt = A y)."""
 
     def __init__(self, op, tableau="vern7", rtol=1e-6, atol=1e-8, nsteps=1000, first_step=0.0,
-                 min_step=0.0, max_step=0.0, interpolate=True, fp32=False, lanes=None):
-        if isinstance(op, _data.HipCSR):
-            A = op.to_scipy()
-        else:
-            A = sp.csr_matrix(op)
-        if A.shape[0] != A.shape[1]:
-            raise ValueError("the generator must be square")
+                 min_step=0.0, max_step=0.0, interpolate=True, fp32=False, lanes=None,
+                 dense=None, dense_precision="fp32"):
+        """``op``: the sparse generator A (dy/dt = A y), or None with
+        ``dense=(H_nh, [c_ops])`` (numpy arrays) for the matrix-form
+        Lindblad right-hand side on MFMA."""
         tab = TABLEAUS[tableau] if isinstance(tableau, str) else tableau
         self.tableau = tab
-        self.n = A.shape[0]
-        if lanes is None:
-            lanes = autotune.lanes_for(A, fp32)
-        self.lanes = lanes
-        self.sell = _data.SellMatrix(A, fp32=fp32, lanes=lanes)
         self._a = np.ascontiguousarray(tab["a"], dtype=np.float64)
         self._b = np.ascontiguousarray(tab["b"], dtype=np.float64)
         self._c = np.ascontiguousarray(tab["c"], dtype=np.float64)
         self._e = np.ascontiguousarray(tab["e"], dtype=np.float64) if tab.get("e") is not None else None
         self._bi = np.ascontiguousarray(tab["bi"], dtype=np.float64) if tab.get("bi") is not None else None
+        tab_args = (int(tab["order"]), len(self._b), len(self._c),
+                    self._bi.shape[1] if self._bi is not None else 0,
+                    _dp(self._a), _dp(self._b), _dp(self._c),
+                    _dp(self._e) if self._e is not None else None,
+                    _dp(self._bi) if self._bi is not None else None,
+                    float(rtol), float(atol), int(nsteps), float(first_step), float(min_step),
+                    float(max_step), int(bool(interpolate)))
         L = _lib.lib()
         h = C.c_void_p()
-        sptr, cols, vals = self.sell.ptrs
-        rc = L.qa_rk_create(
-            C.byref(h), self.n, lanes, self.sell.nslices, C.c_void_p(sptr), C.c_void_p(cols),
-            C.c_void_p(vals), int(fp32), int(tab["order"]), len(self._b), len(self._c),
-            self._bi.shape[1] if self._bi is not None else 0,
-            _dp(self._a), _dp(self._b), _dp(self._c),
-            _dp(self._e) if self._e is not None else None,
-            _dp(self._bi) if self._bi is not None else None,
-            float(rtol), float(atol), int(nsteps), float(first_step), float(min_step),
-            float(max_step), int(bool(interpolate)))
-        _lib.check(rc, "qa_rk_create")
+        if dense is not None:
+            Hnh, cops = dense
+            N = Hnh.shape[0]
+            self.n = N * N
+            self.lanes = 0
+            self.sell = None
+            self.dense_precision = dense_precision
+            Hf = np.asfortranarray(Hnh, dtype=np.complex128)
+            cf = np.concatenate([np.asfortranarray(c, dtype=np.complex128).ravel(order="F")
+                                 for c in cops]) if cops else np.zeros(1, np.complex128)
+            self._dense_host = (Hf, cf)
+            prec = {"fp32": 0, "bf16x3": 1}[dense_precision]
+            rc = L.qa_rk_create_dense(
+                C.byref(h), N, len(cops), _dp(Hf.ravel(order="F").view(np.float64)),
+                _dp(np.ascontiguousarray(cf).view(np.float64)), prec, *tab_args)
+            _lib.check(rc, "qa_rk_create_dense")
+        else:
+            if isinstance(op, _data.HipCSR):
+                A = op.to_scipy()
+            else:
+                A = sp.csr_matrix(op)
+            if A.shape[0] != A.shape[1]:
+                raise ValueError("the generator must be square")
+            self.n = A.shape[0]
+            if lanes is None:
+                lanes = autotune.lanes_for(A, fp32)
+            self.lanes = lanes
+            self.sell = _data.SellMatrix(A, fp32=fp32, lanes=lanes)
+            sptr, cols, vals = self.sell.ptrs
+            rc = L.qa_rk_create(
+                C.byref(h), self.n, lanes, self.sell.nslices, C.c_void_p(sptr),
+                C.c_void_p(cols), C.c_void_p(vals), int(fp32), *tab_args)
+            _lib.check(rc, "qa_rk_create")
         self.handle = h
         self.nfun = 0
         self._y_dev = _data.HipDense((self.n, 1))
@@ -144,6 +167,12 @@ class RKEngine:
 
     def iterate(self, count):
         _lib.call("qa_rk_iterate", self.handle, int(count))
+
+    def apply(self, x, out=None):
+        """out = A x (device vectors of n entries)."""
+        out = _data.HipDense(x.shape) if out is None else out
+        _lib.call("qa_rk_apply", self.handle, C.c_void_p(x.ptr), C.c_void_p(out.ptr))
+        return out
 
 
 class PlainRK:
@@ -367,6 +396,7 @@ class IntegratorHipRK(Integrator):
         "tableau": "vern7",
         "fused": True,
         "operator_precision": "fp64",
+        "dense_precision": "fp32",
     }
     method = "hip_rk"
     name = "hip_rk"
@@ -375,8 +405,16 @@ class IntegratorHipRK(Integrator):
         o = self.options
         self._engine = None
         self._plain = None
-        op = constant_operator(self.derivative) if o["fused"] else None
-        if op is not None:
+        owner = getattr(self.derivative, "__self__", None)
+        dense = getattr(owner, "dense_operators", None) if o["fused"] else None
+        op = constant_operator(self.derivative) if (o["fused"] and dense is None) else None
+        if dense is not None:
+            self._engine = RKEngine(None, o["tableau"], o["rtol"], o["atol"], o["nsteps"],
+                                    o["first_step"], o["min_step"], o["max_step"],
+                                    o["interpolate"], dense=dense(),
+                                    dense_precision=o["dense_precision"])
+            self.name = f"hip_rk (fused, matrix form, {o['dense_precision']} MFMA)"
+        elif op is not None:
             self._engine = RKEngine(op, o["tableau"], o["rtol"], o["atol"], o["nsteps"],
                                     o["first_step"], o["min_step"], o["max_step"],
                                     o["interpolate"], fp32=(o["operator_precision"] == "fp32"))
@@ -466,3 +504,5 @@ class IntegratorHipRK(Integrator):
             raise ValueError(f"unknown tableau {self._options.get('tableau')!r}")
         if self._options.get("operator_precision") not in ("fp64", "fp32"):
             raise ValueError("operator_precision must be 'fp64' or 'fp32'")
+        if self._options.get("dense_precision") not in ("fp32", "bf16x3"):
+            raise ValueError("dense_precision must be 'fp32' or 'bf16x3'")

@@ -4,7 +4,7 @@ d vec(rho)/dt = L vec(rho) with L = liouvillian(H, c_ops)
 (qutip/core/superoperator.py:51-142)."""
 import numpy as np
 
-from ..core import Qobj, QobjEvo, liouvillian
+from ..core import LindbladMatrixForm, Qobj, QobjEvo, liouvillian
 from .hip_rk import IntegratorHipRK
 from .sesolve import sesolve
 from .solver_base import Solver
@@ -13,9 +13,13 @@ __all__ = ["mesolve", "MESolver"]
 
 
 class MESolver(Solver):
+    """With ``options={"matrix_form": True}`` the state stays an N x N matrix
+    and the right-hand side is LindbladMatrixForm (mesolve.py:251-277): the
+    MFMA dense path (``dense_precision`` 'fp32' or 'bf16x3')."""
     name = "mesolve"
     _avail_integrators = {}
     _preserves_hermiticity = True
+    solver_options = {**Solver.solver_options, "matrix_form": False}
 
     def __init__(self, H, c_ops=None, *, options=None):
         if isinstance(H, QobjEvo):
@@ -25,11 +29,19 @@ class MESolver(Solver):
             if not isinstance(c, Qobj):
                 raise TypeError("c_ops must be Qobj")
         self._num_collapse = len(c_ops)
-        rhs = liouvillian(H, [c for c in c_ops if not c.issuper])
-        for c in c_ops:
-            if c.issuper:
-                rhs = rhs + c
-        super().__init__(QobjEvo(rhs), options=options)
+        self._matrix_form = bool((options or {}).get("matrix_form", False))
+        if self._matrix_form:
+            if H.issuper or any(c.issuper for c in c_ops):
+                raise ValueError("matrix_form=True cannot be used with superoperator H")
+            self._vectorize_state = False
+            rhs = LindbladMatrixForm(H, c_ops)
+        else:
+            rhs = liouvillian(H, [c for c in c_ops if not c.issuper])
+            for c in c_ops:
+                if c.issuper:
+                    rhs = rhs + c
+            rhs = QobjEvo(rhs)
+        super().__init__(rhs, options=options)
 
 
 MESolver.add_integrator(IntegratorHipRK, "hip_rk")

@@ -51,8 +51,8 @@ class Solver:
     def __init__(self, rhs, *, options=None):
         if isinstance(rhs, Qobj):
             rhs = QobjEvo(rhs)
-        if not isinstance(rhs, QobjEvo):
-            raise TypeError("The rhs must be a Qobj or QobjEvo")
+        if not hasattr(rhs, "matmul_data"):
+            raise TypeError("The rhs must be a Qobj, QobjEvo or LindbladMatrixForm")
         self.rhs = rhs
         self._options = {}
         self.options = options
@@ -117,7 +117,7 @@ class Solver:
 
     # -- state preparation (solver_base.py:81-157) ------------------------------
     def _prepare_state(self, state):
-        if self.rhs.issuper and state.isket:
+        if (self.rhs.issuper or getattr(self, "_matrix_form", False)) and state.isket:
             state = ket2dm(state)
         if state.isket:
             norm = state.norm()

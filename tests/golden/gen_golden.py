@@ -10,11 +10,12 @@ is not importable here).  Files:
   ising10_mesolve.npz dissipative TFIM L=10 (gamma=0.05, t in [0,20], 100 pts):
                       the master-equation target of the mcsolve ensemble
   kerr_mesolve.npz    Kerr N=256 (first 40 output times, t <= 7.8): <n>, parity
+  kerr_full_mesolve.npz  the same over t in [0, 30], 150 points (--kerr-full)
   ising6_sesolve.npz  closed TFIM L=6 sesolve: <sum sz/L>, <sz1 sz2>
 
-Usage: python tests/golden/gen_golden.py [--all | --ising10]  (--all adds the
-L=10 case, which builds a 1M x 1M Liouvillian and takes several minutes;
---ising10 writes only that one).
+Usage: python tests/golden/gen_golden.py [--all | --ising10 | --kerr-full]
+(--all adds the L=10 case, which builds a 1M x 1M Liouvillian and takes
+several minutes; --ising10 / --kerr-full write only that one).
 """
 import pathlib
 import sys
@@ -37,6 +38,11 @@ def save(name, **kw):
 
 def main(argv):
     t0 = time.time()
+    if "--kerr-full" in argv:
+        m = models.kerr(256)
+        ex = exact.mesolve_expect(m["H"], m["c_ops"], m["psi0"], m["tlist"], m["e_ops"])
+        save("kerr_full_mesolve.npz", tlist=m["tlist"], expect=ex)
+        return
     if "--ising10" in argv:
         m = models.ising(10, 0.05, tf=20.0, npts=100)
         ex = exact.mesolve_expect(m["H"], m["c_ops"], m["psi0"], m["tlist"], m["e_ops"])
