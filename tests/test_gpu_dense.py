@@ -40,7 +40,8 @@ def test_matrix_form_rhs_vs_numpy(N, prec):
     want = _dense_rhs(H, cs, rho)
     rel = np.abs(got - want).max() / np.abs(want).max()
     print(f"  relative max error {rel:.2e}", flush=True)
-    assert rel < 2e-5
+    # FP32: 24-bit products; bf16 x3 drops the lo*lo term: ~2^-16 per factor
+    assert rel < (2e-5 if prec == "fp32" else 1e-4)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
