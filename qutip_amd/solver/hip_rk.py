@@ -379,6 +379,12 @@ def constant_operator(derivative):
         return d.to_scipy()
     if sp.issparse(d):
         return sp.csr_matrix(d)
+    if hasattr(d, "hip") and isinstance(d.hip, _data.HipCSR):  # qutip wrapper type
+        return d.hip.to_scipy()
+    if hasattr(d, "as_scipy"):  # qutip CSR / Dia
+        return sp.csr_matrix(d.as_scipy())
+    if hasattr(d, "to_array"):  # qutip Dense
+        return sp.csr_matrix(d.to_array())
     return sp.csr_matrix(np.asarray(d))
 
 

@@ -107,3 +107,11 @@ def test_device_ops_fail_loudly_without_gpu():
         pytest.skip("a GPU is visible")
     with pytest.raises(qa._lib.QutipAmdUnavailable):
         qa.HipDense.from_array(np.ones(4))
+
+
+def test_qutip_plugin_imports_without_qutip():
+    from qutip_amd import qutip_plugin
+    assert qutip_plugin.available() in (True, False)
+    if not qutip_plugin.available():
+        with pytest.raises(ImportError):
+            qutip_plugin.register()
