@@ -62,6 +62,11 @@ int qa_stream_create(void **stream);
 int qa_stream_destroy(void *stream);
 int qa_stream_sync(void *stream);
 int qa_device_sync(void);
+/* events: completion markers on a stream, polled without blocking */
+int qa_event_create(void **event);
+int qa_event_destroy(void *event);
+int qa_event_record(void *event, void *stream);
+int qa_event_query(void *event, int *done);
 
 /* ---- data-layer kernels --------------------------------------------------
  * These back the dispatcher specialisations of the device data types.      */
@@ -196,6 +201,17 @@ int qa_rk_set_state(qa_rk_solver *s, double t0, const void *y0);
  * integration failed (status < 0). */
 int qa_rk_run(qa_rk_solver *s, int ntimes, const double *times, int single_step,
               double *fun_out, void *states_out, int *status, double *t_reached);
+
+/* The same run without blocking, for many engines in flight at once (the
+ * solve server): qa_rk_begin uploads the times and starts the run;
+ * qa_rk_advance returns at once -- when the previous chunk of device loop
+ * iterations has finished it enqueues the next one -- and sets *done to 1
+ * (finished), -1 (failed) or 0; qa_rk_finish waits, copies the functionals
+ * and reports the status like qa_rk_run. */
+int qa_rk_begin(qa_rk_solver *s, int ntimes, const double *times, int single_step,
+                void *states_out);
+int qa_rk_advance(qa_rk_solver *s, int *done);
+int qa_rk_finish(qa_rk_solver *s, double *fun_out, int *status, double *t_reached);
 
 /* get_state: time and state (device, n complex128) of the last output. */
 int qa_rk_get_state(qa_rk_solver *s, double *t, void *y_out);

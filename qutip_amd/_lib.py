@@ -45,6 +45,10 @@ SIGNATURES = {
     "qa_stream_destroy": [vp],
     "qa_stream_sync": [vp],
     "qa_device_sync": [],
+    "qa_event_create": [C.POINTER(vp)],
+    "qa_event_destroy": [vp],
+    "qa_event_record": [vp, vp],
+    "qa_event_query": [vp, P_i32],
     "qa_csr_matvec_z": [i32, i32, vp, vp, vp, i32, i32, vp, vp, f64, f64, i32, vp],
     "qa_csr_matmat_z": [i32, i32, vp, vp, vp, i32, i32, vp, i64, vp, i64, f64, f64, i32, vp],
     "qa_sell_matvec_z": [i32, i32, i32, vp, vp, vp, i32, vp, vp, f64, f64, i32, vp],
@@ -66,6 +70,9 @@ SIGNATURES = {
     "qa_rk_set_functionals": [vp, i32, P_i32, P_i32, P_i32, P_i32, P_f64],
     "qa_rk_set_state": [vp, f64, vp],
     "qa_rk_run": [vp, i32, P_f64, i32, P_f64, vp, P_i32, P_f64],
+    "qa_rk_begin": [vp, i32, P_f64, i32, vp],
+    "qa_rk_advance": [vp, P_i32],
+    "qa_rk_finish": [vp, P_f64, P_i32, P_f64],
     "qa_rk_get_state": [vp, P_f64, vp],
     "qa_rk_stats": [vp, P_i64],
     "qa_rk_iterate": [vp, i32],

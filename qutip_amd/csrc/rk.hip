@@ -753,6 +753,12 @@ struct qa_rk_solver {
     long long graph_launches = 0;
     long long setup_rhs = 0;  // derivative evaluations of the first-step estimate
     int state_set = 0;
+    // asynchronous run (qa_rk_begin / qa_rk_advance / qa_rk_finish)
+    hipEvent_t event = nullptr;
+    int run_state = 0;  // 0 idle, 1 running, 2 done, 3 failed
+    int run_ntimes = 0, chunk = 4;
+    long long run_iters = 0, last_attempts = -1, last_outputs = -1;
+    bool in_flight = false;
 };
 
 namespace {
@@ -958,6 +964,7 @@ int rebuild_graph(qa_rk_solver *S) {
 void free_all(qa_rk_solver *S) {
     if (S->exec) (void)hipGraphExecDestroy(S->exec);
     if (S->graph) (void)hipGraphDestroy(S->graph);
+    if (S->event) (void)hipEventDestroy(S->event);
     void *ptrs[] = {S->block, S->dense_block, S->ctl, S->bi_dev, S->times_dev, S->fun_out_dev, S->fptr_dev,
                     S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev};
     for (void *p : ptrs)
@@ -1313,14 +1320,16 @@ int qa_rk_set_state(qa_rk_solver *S, double t0, const void *y0_dev) {
     return QA_OK;
 }
 
-int qa_rk_run(qa_rk_solver *S, int ntimes, const double *times, int single_step,
-              double *fun_out, void *states_out, int *status, double *t_reached) {
-    QA_REQUIRE(S != nullptr, "qa_rk_run: null solver");
-    QA_REQUIRE(S->state_set, "qa_rk_run: set_state was not called");
-    QA_REQUIRE(ntimes >= 1 && times != nullptr, "qa_rk_run: need at least one time");
+int qa_rk_begin(qa_rk_solver *S, int ntimes, const double *times, int single_step,
+                void *states_out) {
+    QA_REQUIRE(S != nullptr, "qa_rk_begin: null solver");
+    QA_REQUIRE(S->state_set, "qa_rk_begin: set_state was not called");
+    QA_REQUIRE(ntimes >= 1 && times != nullptr, "qa_rk_begin: need at least one time");
     for (int i = 1; i < ntimes; ++i)
-        QA_REQUIRE(times[i] > times[i - 1], "qa_rk_run: times must be increasing");
+        QA_REQUIRE(times[i] > times[i - 1], "qa_rk_begin: times must be increasing");
     hipStream_t s = S->stream;
+    // the previous run may still be draining its last chunk
+    QA_HIP(hipStreamSynchronize(s));
     if (ntimes > S->times_cap) {
         if (S->times_dev) QA_HIP(hipFree(S->times_dev));
         S->times_dev = nullptr;
@@ -1353,56 +1362,119 @@ int qa_rk_run(qa_rk_solver *S, int ntimes, const double *times, int single_step,
         int rc = rebuild_graph(S);
         if (rc) return rc;
     }
+    if (!S->event) QA_HIP(hipEventCreateWithFlags(&S->event, hipEventDisableTiming));
     hipLaunchKernelGGL(rk_begin_kernel, dim3(1), dim3(1), 0, s, S->ctl, ntimes,
                        (double2 *)states_out);
     QA_CHECK_LAUNCH();
-    // Replay the captured iteration in chunks; read the phase back after each.
-    // Every iteration either attempts a step or emits outputs, so a chunk that
-    // changes neither counter means the device loop is stuck: stop there.
-    int chunk = 4;
-    long long iters = 0;
-    long long last_attempts = -1, last_outputs = -1;
-    while (true) {
-        for (int k = 0; k < chunk; ++k) {
-            if (S->use_graph) {
-                QA_HIP(hipGraphLaunch(S->exec, s));
-            } else {
-                int rc = enqueue_iteration(S);
-                if (rc) return rc;
-            }
-        }
-        iters += chunk;
-        S->graph_launches += chunk;
-        QA_HIP(hipMemcpyAsync(S->ctl_host, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToHost, s));
-        QA_HIP(hipStreamSynchronize(s));
+    S->run_ntimes = ntimes;
+    S->run_iters = 0;
+    S->chunk = 4;
+    S->last_attempts = -1;
+    S->last_outputs = -1;
+    S->in_flight = false;
+    S->run_state = 1;
+    return QA_OK;
+}
+
+// Non-blocking progress: when the previous chunk has finished (event
+// query), read the device phase from the pinned mirror and either report
+// completion or enqueue the next chunk of loop iterations (graph replays)
+// followed by an asynchronous copy of the control block.  *done: 0 running,
+// 1 finished, -1 failed (the integrator status is in qa_rk_finish).
+// Every iteration either attempts a step or emits outputs, so a chunk that
+// changes neither counter means the device loop is stuck: that is an error.
+int qa_rk_advance(qa_rk_solver *S, int *done) {
+    QA_REQUIRE(S != nullptr && done != nullptr, "qa_rk_advance: null argument");
+    QA_REQUIRE(S->run_state != 0, "qa_rk_advance: no run in progress (qa_rk_begin)");
+    *done = 0;
+    if (S->run_state == 2) {
+        *done = 1;
+        return QA_OK;
+    }
+    if (S->run_state == 3) {
+        *done = -1;
+        return QA_OK;
+    }
+    hipStream_t s = S->stream;
+    if (S->in_flight) {
+        const hipError_t q = hipEventQuery(S->event);
+        if (q == hipErrorNotReady) return QA_OK;
+        QA_HIP(q);
+        S->in_flight = false;
         const RKCtl &h = *S->ctl_host;
-        if (h.phase == PH_DONE || h.phase == PH_ERROR || h.phase == PH_IDLE) break;
-        if (h.n_attempts == last_attempts && h.n_outputs == last_outputs) {
-            qa::set_error("qa_rk_run: device loop made no progress (phase %d, output %d)",
-                          h.phase, h.out_idx);
+        if (h.phase == PH_DONE || h.phase == PH_IDLE) {
+            S->run_state = 2;
+            *done = 1;
+            return QA_OK;
+        }
+        if (h.phase == PH_ERROR) {
+            S->run_state = 3;
+            *done = -1;
+            return QA_OK;
+        }
+        if (h.n_attempts == S->last_attempts && h.n_outputs == S->last_outputs) {
+            S->run_state = 3;
+            qa::set_error("device loop made no progress (phase %d, output %d)", h.phase, h.out_idx);
             return QA_ERR_STATE;
         }
-        last_attempts = h.n_attempts;
-        last_outputs = h.n_outputs;
-        // next chunk: estimate from the progress so far
-        const int done = std::max(h.out_idx, 1);
-        const double per_out = (double)iters / done;
-        long long est = (long long)(per_out * (ntimes - h.out_idx)) + 1;
-        chunk = (int)std::max(1LL, std::min(est, 64LL));
+        S->last_attempts = h.n_attempts;
+        S->last_outputs = h.n_outputs;
+        const int ndone = std::max(h.out_idx, 1);
+        const double per_out = (double)S->run_iters / ndone;
+        const long long est = (long long)(per_out * (S->run_ntimes - h.out_idx)) + 1;
+        S->chunk = (int)std::max(1LL, std::min(est, 64LL));
     }
+    for (int k = 0; k < S->chunk; ++k) {
+        if (S->use_graph) {
+            QA_HIP(hipGraphLaunch(S->exec, s));
+        } else {
+            int rc = enqueue_iteration(S);
+            if (rc) return rc;
+        }
+    }
+    S->run_iters += S->chunk;
+    S->graph_launches += S->chunk;
+    QA_HIP(hipMemcpyAsync(S->ctl_host, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToHost, s));
+    QA_HIP(hipEventRecord(S->event, s));
+    S->in_flight = true;
+    return QA_OK;
+}
+
+int qa_rk_finish(qa_rk_solver *S, double *fun_out, int *status, double *t_reached) {
+    QA_REQUIRE(S != nullptr, "qa_rk_finish: null solver");
+    QA_REQUIRE(S->run_state == 2 || S->run_state == 3, "qa_rk_finish: the run has not ended");
+    QA_HIP(hipStreamSynchronize(S->stream));
     const RKCtl &h = *S->ctl_host;
     if (status) *status = h.status;
     if (t_reached) *t_reached = h.t_last;
+    const bool failed = S->run_state == 3;
+    S->run_state = 0;
     if (fun_out && S->nfun > 0) {
-        const int ndone = h.phase == PH_DONE ? ntimes : h.out_idx;
+        const int ndone = h.phase == PH_DONE ? S->run_ntimes : h.out_idx;
         QA_HIP(hipMemcpy(fun_out, S->fun_out_dev, sizeof(double2) * (size_t)ndone * S->nfun,
                          hipMemcpyDeviceToHost));
     }
-    if (h.phase == PH_ERROR) {
+    if (failed || h.phase == PH_ERROR) {
         qa::set_error("integration failed with status %d", h.status);
         return QA_ERR_STATE;
     }
     return QA_OK;
+}
+
+int qa_rk_run(qa_rk_solver *S, int ntimes, const double *times, int single_step,
+              double *fun_out, void *states_out, int *status, double *t_reached) {
+    int rc = qa_rk_begin(S, ntimes, times, single_step, states_out);
+    if (rc) return rc;
+    int done = 0;
+    while (!done) {
+        rc = qa_rk_advance(S, &done);
+        if (rc) {
+            S->run_state = 0;
+            return rc;
+        }
+        if (!done && S->in_flight) QA_HIP(hipEventSynchronize(S->event));
+    }
+    return qa_rk_finish(S, fun_out, status, t_reached);
 }
 
 int qa_rk_get_state(qa_rk_solver *S, double *t, void *y_out) {

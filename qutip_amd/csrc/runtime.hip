@@ -145,6 +145,37 @@ int qa_stream_sync(void *stream) {
     return QA_OK;
 }
 
+int qa_event_create(void **event) {
+    QA_REQUIRE(event != nullptr, "qa_event_create: null output");
+    hipEvent_t e;
+    QA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    *event = (void *)e;
+    return QA_OK;
+}
+
+int qa_event_destroy(void *event) {
+    if (event) QA_HIP(hipEventDestroy((hipEvent_t)event));
+    return QA_OK;
+}
+
+int qa_event_record(void *event, void *stream) {
+    QA_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return QA_OK;
+}
+
+// *done = 1 when every operation before the record has finished (non-blocking).
+int qa_event_query(void *event, int *done) {
+    QA_REQUIRE(done != nullptr, "qa_event_query: null output");
+    const hipError_t e = hipEventQuery((hipEvent_t)event);
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return QA_OK;
+    }
+    QA_HIP(e);
+    *done = 1;
+    return QA_OK;
+}
+
 int qa_device_sync(void) {
     QA_HIP(hipDeviceSynchronize());
     return QA_OK;

@@ -237,8 +237,13 @@ class MCSolver:
             fp32 = self.options["precision"] == "fp32"
         return self._launch(np.asarray(psi0).ravel(), tlist, e_ops, draws, max_jumps, fp32)
 
-    def _launch(self, psi0, tlist, e_ops, draws, max_jumps, fp32):
+    def launch_async(self, psi0, tlist, e_ops, draws, max_jumps=64, stream=None, fp32=None):
+        """Enqueue the trajectory kernel on ``stream`` and return a handle that
+        keeps the device buffers alive; ``collect_async`` reads the results
+        once the stream has reached this point."""
         o = self.options
+        if fp32 is None:
+            fp32 = o["precision"] == "fp32"
         if isinstance(draws, tuple):  # (DeviceBuffer, ntraj, max_draws)
             dr, ntraj, max_draws = draws
         else:
@@ -248,33 +253,51 @@ class MCSolver:
         _, h, c, tab = self._device_ops(fp32)
         e = _csr_block([O.to_csr() for O in e_ops] or [sp.identity(self.d)], self.d, fp32)
         neops = len(e_ops)
-        times = _data.DeviceBuffer.from_array(np.asarray(tlist, dtype=np.float64))
-        p0 = _data.DeviceBuffer.from_array(np.ascontiguousarray(psi0, dtype=np.complex128))
-        ex = _data.DeviceBuffer(16 * ntraj * ntimes * max(neops, 1))
-        ct = _data.DeviceBuffer(8 * ntraj * max_jumps)
-        cw = _data.DeviceBuffer(4 * ntraj * max_jumps)
-        info = _data.DeviceBuffer(16 * ntraj)
+        H = {"e": e, "dr": dr, "ntraj": ntraj, "ntimes": ntimes, "neops": neops,
+             "max_jumps": max_jumps, "stream": stream}
+        H["times"] = _data.DeviceBuffer.from_array(np.asarray(tlist, dtype=np.float64))
+        H["p0"] = _data.DeviceBuffer.from_array(np.ascontiguousarray(np.asarray(psi0).ravel(),
+                                                                     dtype=np.complex128))
+        H["ex"] = _data.DeviceBuffer(16 * ntraj * ntimes * max(neops, 1))
+        H["ct"] = _data.DeviceBuffer(8 * ntraj * max_jumps)
+        H["cw"] = _data.DeviceBuffer(4 * ntraj * max_jumps)
+        H["info"] = _data.DeviceBuffer(16 * ntraj)
         L = _lib.lib()
         rc = L.qa_mc_run(
-            self.d, ntraj, ntimes, _vp(times.ptr), _vp(p0.ptr), _vp(h[0].ptr), _vp(h[1].ptr),
-            _vp(h[2].ptr), len(self.c_ops), _vp(c[0].ptr), _vp(c[1].ptr), _vp(c[2].ptr), neops,
-            _vp(e[0].ptr), _vp(e[1].ptr), _vp(e[2].ptr), _vp(tab.ptr), _vp(dr.ptr), max_draws,
-            float(o["rtol"]), float(o["atol"]), int(o["nsteps"]), float(o["first_step"]),
-            float(o["min_step"]), float(o["max_step"]), int(o["norm_steps"]),
-            float(o["norm_t_tol"]), float(o["norm_tol"]), float(o["norm_min_step"]),
-            float(o["mc_corr_eps"]), int(fp32), _vp(ex.ptr), _vp(ct.ptr), _vp(cw.ptr), max_jumps,
-            _vp(info.ptr), int(o["grid"]), None)
+            self.d, ntraj, ntimes, _vp(H["times"].ptr), _vp(H["p0"].ptr), _vp(h[0].ptr),
+            _vp(h[1].ptr), _vp(h[2].ptr), len(self.c_ops), _vp(c[0].ptr), _vp(c[1].ptr),
+            _vp(c[2].ptr), neops, _vp(e[0].ptr), _vp(e[1].ptr), _vp(e[2].ptr), _vp(tab.ptr),
+            _vp(dr.ptr), max_draws, float(o["rtol"]), float(o["atol"]), int(o["nsteps"]),
+            float(o["first_step"]), float(o["min_step"]), float(o["max_step"]),
+            int(o["norm_steps"]), float(o["norm_t_tol"]), float(o["norm_tol"]),
+            float(o["norm_min_step"]), float(o["mc_corr_eps"]), int(fp32), _vp(H["ex"].ptr),
+            _vp(H["ct"].ptr), _vp(H["cw"].ptr), max_jumps, _vp(H["info"].ptr), int(o["grid"]),
+            stream)
         _lib.check(rc, "qa_mc_run")
-        _lib.call("qa_device_sync")
+        return H
+
+    def collect_async(self, H, with_jumps=False):
+        """Results of a finished launch_async: (runs_expect, info[, col_times,
+        col_which])."""
+        ntraj, ntimes, neops, mj = H["ntraj"], H["ntimes"], H["neops"], H["max_jumps"]
+        if H["stream"] is not None:
+            _lib.call("qa_stream_sync", H["stream"])
         out = np.empty((ntraj, ntimes, max(neops, 1)), dtype=np.complex128)
-        ex.download(out)
-        cts = np.empty((ntraj, max_jumps), dtype=np.float64)
-        ct.download(cts)
-        cws = np.empty((ntraj, max_jumps), dtype=np.int32)
-        cw.download(cws)
+        H["ex"].download(out)
         inf = np.empty((ntraj, 4), dtype=np.int32)
-        info.download(inf)
+        H["info"].download(inf)
+        if not with_jumps:
+            return out[:, :, :neops], inf
+        cts = np.empty((ntraj, mj), dtype=np.float64)
+        H["ct"].download(cts)
+        cws = np.empty((ntraj, mj), dtype=np.int32)
+        H["cw"].download(cws)
         return out[:, :, :neops], inf, cts, cws
+
+    def _launch(self, psi0, tlist, e_ops, draws, max_jumps, fp32):
+        H = self.launch_async(psi0, tlist, e_ops, draws, max_jumps, None, fp32)
+        _lib.call("qa_device_sync")
+        return self.collect_async(H, with_jumps=True)
 
     def run(self, state, tlist, ntraj=1, *, args=None, e_ops=None, timeout=None,
             target_tol=None, seeds=None):

@@ -48,3 +48,31 @@ def kerr(N=256, delta=0.5, K=0.1, F=2.0, kappa=1.0):
     parity = qa.Qobj(np.diag((-1.0) ** np.arange(N)), dims=[[N], [N]])
     return dict(H=H, c_ops=[np.sqrt(kappa) * a], psi0=qa.basis(N, 0),
                 e_ops=[a.dag() @ a, parity], tlist=np.linspace(0, 30, 150))
+
+
+def serving_trace(n, rate=200.0, seed=7, dims=(16, 32, 64, 128, 256, 512), tf=10.0, npts=50,
+                  kinds=("se", "me", "mc"), kind_p=(0.4, 0.4, 0.2), ntraj=(64, 512)):
+    """The serving workload: Poisson arrivals (rate per second), d uniform in
+    ``dims``, H = rand_herm(d, density=0.1), 1-4 collapse operators
+    sqrt(gamma) rand_dm(d) with gamma ~ U[0.01, 0.1], 40% sesolve / 40% mesolve
+    / 20% mcsolve (ntraj ~ U{64..512}), t in [0, 10] with 50 outputs,
+    psi0 = basis(d, 0), e_ops = {|0><0|, H}; every draw from default_rng(seed)."""
+    from qutip_amd.serve import SolveRequest
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        d = int(rng.choice(dims))
+        kind = str(rng.choice(kinds, p=kind_p))
+        nc = int(rng.integers(1, 5))
+        gammas = rng.uniform(0.01, 0.1, nc)
+        nt = int(rng.integers(ntraj[0], ntraj[1] + 1))
+        s_op = int(rng.integers(2 ** 31))
+        H = qa.rand_herm(d, density=0.1, seed=s_op)
+        c_ops = [np.sqrt(g) * qa.rand_dm(d, seed=s_op + 1 + k) for k, g in enumerate(gammas)]
+        proj = qa.basis(d, 0) @ qa.basis(d, 0).dag()
+        reqs.append(SolveRequest(kind, H, qa.basis(d, 0), np.linspace(0, tf, npts),
+                                 c_ops=c_ops if kind != "se" else [], e_ops=[proj, H],
+                                 ntraj=nt if kind == "mc" else 0, seed=s_op, arrival=t, rid=i))
+    return reqs
