@@ -114,6 +114,129 @@ def bench_mcsolve(qa, systems, log, ntraj=2048, reps=3):
     return out
 
 
+def bench_dense(qa, systems, log, reps=2):
+    """Matrix-form mesolve of the driven-dissipative Kerr oscillator, N=256
+    (dense rho 256 x 256), t in [0, 30] with 150 outputs, e_ops a^dag a and
+    parity, vern7 at rtol=1e-8 atol=1e-10, on MFMA in FP32 and in the bf16 x3
+    split (FP32 accumulate); error against the FP64 fixture.  Algorithmic
+    FLOP per RHS: the four complex 256^3 products Hm X, X Hp, T = c X and
+    T c^dag (4 * 8 * 256^3)."""
+    from qutip_amd import _lib
+    s = systems.kerr(256)
+    g = np.load(ROOT / "tests" / "golden" / "kerr_full_mesolve.npz")
+    tl = g["tlist"]
+    N = 256
+    flop_rhs = 4 * 8 * N ** 3
+    out = {"workload": "mesolve Kerr oscillator N=256 matrix form (dense rho 256x256), "
+                       "t in [0,30], 150 outputs, vern7 rtol=1e-8 atol=1e-10",
+           "algorithmic_flop_per_rhs": flop_rhs}
+    for prec in ("fp32", "bf16x3"):
+        solver = qa.MESolver(s["H"], s["c_ops"], options={
+            "matrix_form": True, "dense_precision": prec, "rtol": 1e-8, "atol": 1e-10})
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            res = solver.run(s["psi0"], tl, e_ops=s["e_ops"])
+            _lib.call("qa_device_sync")
+            ts.append(time.perf_counter() - t0)
+        err = max(np.abs(np.asarray(res.expect[k]) - g["expect"][k]).max() for k in range(2))
+        st = solver._integrator.stats()
+        eng = solver._integrator._engine
+        ms = np.zeros(16)
+        _lib.call("qa_rk_profile_stages", eng.handle, 10, ms.ctypes.data_as(C.POINTER(C.c_double)))
+        rhs_ms = float(ms[:10].mean())
+        tf = flop_rhs / (rhs_ms * 1e-3) / 1e12
+        best = min(ts)
+        out[prec] = {"solves_per_s": round(1 / best, 4), "s_per_solve": round(best, 3),
+                     "err_vs_fp64": float(err), "rk_attempts": st["attempts"],
+                     "rhs_evals": st["rhs_evals"], "stage_us": round(rhs_ms * 1e3, 2),
+                     "mfma_tflops": round(tf, 2),
+                     "mfma_peak_tflops": 157.3 if prec == "fp32" else 2500.0}
+        log(f"dense {prec}: {1 / best:.3f} solves/s err {err:.2e} stage {rhs_ms * 1e3:.1f} us "
+            f"({tf:.1f} TFLOP/s)")
+    return out
+
+
+def bench_serving(systems, log, nreq=400, warm=40):
+    """The serving mix: Poisson arrivals at 200 req/s (seed 7), d uniform in
+    {16..512}, 40% sesolve / 40% mesolve / 20% mcsolve, admitted by HBM
+    budget and slots, overlapped on per-request streams.  Latency =
+    completion - arrival; the first ``warm`` requests are excluded from the
+    percentiles."""
+    from qutip_amd.serve import SolveServer
+    t0 = time.perf_counter()
+    reqs = systems.serving_trace(nreq)
+    tbuild = time.perf_counter() - t0
+    # warm the kernels and the autotuner cache on a short prefix
+    SolveServer(max_slots=64).run(systems.serving_trace(12), realtime=False)
+    srv = SolveServer(max_slots=64)
+    t0 = time.perf_counter()
+    done = srv.run(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    done = sorted(done, key=lambda r: r.id)
+    lat = np.array([r.latency() for r in done[warm:]])
+    errors = [r.id for r in done if r.error]
+    kinds = {k: sum(1 for r in done if r.kind == k) for k in ("se", "me", "mc")}
+    out = {"workload": f"{nreq} requests, Poisson 200 req/s seed 7, d in {{16..512}}, "
+                       f"40/40/20 se/me/mc, t_final 10, 50 outputs",
+           "solves_per_s": round(len(done) / wall, 2), "wall_s": round(wall, 3),
+           "arrival_span_s": round(reqs[-1].arrival, 3),
+           "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+           "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+           "peak_in_flight": srv.peak_in_flight, "errors": len(errors), "kinds": kinds,
+           "trace_build_s": round(tbuild, 2)}
+    log(f"serving: {out['solves_per_s']} solves/s p50 {out['p50_latency_s']} s "
+        f"p99 {out['p99_latency_s']} s peak {srv.peak_in_flight}")
+    return out
+
+
+COPY_BYTES = 64 << 20
+TRAFFIC_FILE = ROOT / "profiles" / "rk_stage_kernel_ising8_traffic.txt"
+
+
+def pmc_workload():
+    """The launches the HBM-traffic counters are read on (run under
+    rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE): three device-to-device
+    copies of 64 MiB (the calibration: a plain copy reads and writes exactly
+    that many bytes), then 3 launches of each of the 16 stage kernels of the
+    L=8 Ising solve (direct launches, no graph)."""
+    import qutip_amd as qa
+    from qutip_amd import _lib, data as qd
+    from qutip_amd.solver.hip_rk import RKEngine
+    from tests import systems
+    s = systems.ising(8, 0.1)
+    solver = qa.MESolver(s["H"], s["c_ops"], options={"rtol": 1e-8, "atol": 1e-10})
+    solver.run(s["psi0"], s["tlist"][:3], e_ops=s["e_ops"])
+    L = solver.rhs.qobj.to_csr()
+    eng = RKEngine(L, "vern7", 1e-8, 1e-10, lanes=solver._integrator._engine.lanes)
+    rho0 = np.zeros(L.shape[0], dtype=np.complex128)
+    rho0[0] = 1.0
+    eng.set_state(0.0, qd.HipDense.from_array(rho0))
+    a, b = qd.DeviceBuffer(COPY_BYTES), qd.DeviceBuffer(COPY_BYTES)
+    _lib.call("qa_fill_zero", C.c_void_p(a.ptr), C.c_size_t(COPY_BYTES), None)
+    for _ in range(3):
+        _lib.call("qa_memcpy_d2d", C.c_void_p(b.ptr), C.c_void_p(a.ptr), C.c_size_t(COPY_BYTES),
+                  None)
+    _lib.call("qa_device_sync")
+    ms = np.zeros(16)
+    _lib.call("qa_rk_profile_stages", eng.handle, 3, ms.ctypes.data_as(C.POINTER(C.c_double)))
+    _lib.call("qa_device_sync")
+    log(f"pmc workload: 3 copies of {COPY_BYTES} B, 16 x 3 stage launches "
+        f"(lanes {eng.lanes}, nnz {L.nnz})")
+
+
+def read_traffic():
+    """HBM bytes per stage-kernel launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy calibration), or None."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,7 +245,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form solve")
     ap.add_argument("--no-mc", action="store_true", help="skip the batched mcsolve leg")
+    ap.add_argument("--no-dense", action="store_true", help="skip the dense Kerr leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--pmc", action="store_true",
+                    help="only run the launches the HBM-traffic counters are read on")
     args = ap.parse_args()
+    if args.pmc:
+        pmc_workload()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the "
               "solves are independent and would shard by request/trajectory, see DESIGN.md",
@@ -192,12 +323,30 @@ def main():
         "err_vs_fp64": float(err),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": f"rk_stage_kernel<double2, {lanes}>",
+                     "traffic": read_traffic(), "kernel": f"rk_stage_kernel<double2, {lanes}>",
                      "avg_launch_us": round(avg_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": int(avg_bytes)},
     }
+    # the chosen precision form: complex64 operator values, FP64 state,
+    # accumulation, error norm and expectation values
+    fsolver = qa.MESolver(s["H"], s["c_ops"], options={**opts, "operator_precision": "fp32"})
+    fres = fsolver.run(s["psi0"], tlist, e_ops=s["e_ops"])
+    _lib.call("qa_device_sync")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fres = fsolver.run(s["psi0"], tlist, e_ops=s["e_ops"])
+    _lib.call("qa_device_sync")
+    tf32 = (time.perf_counter() - t0) / args.steps
+    ferr = max(np.abs(np.asarray(fres.expect[k]) - golden["expect"][k]).max() for k in range(2))
+    out["fp32_operator"] = {"solves_per_s": round(1 / tf32, 4), "err_vs_fp64": float(ferr),
+                            "rk_attempts": fsolver._integrator.stats()["attempts"]}
+    log(f"fp32 operator: {1 / tf32:.2f} solves/s, err {ferr:.2e}")
     if not args.no_mc:
         out["mcsolve"] = bench_mcsolve(qa, systems, log)
+    if not args.no_dense:
+        out["dense_kerr"] = bench_dense(qa, systems, log)
+    if not args.no_serve:
+        out["serving"] = bench_serving(systems, log, nreq=args.serve_requests)
     if not args.no_plain:
         psolver = qa.MESolver(s["H"], s["c_ops"], options={**opts, "fused": False})
         t0 = time.perf_counter()

@@ -248,8 +248,10 @@ int qa_mc_pack_tableau(void *out, int order, int rk_step, int rk_extra, int dord
                        const double *bi, int interpolate);
 
 /* Run ntraj trajectories of d psi/dt = H_eff psi (H_eff = -iH - 1/2 sum
- * c^dag c, CSR d x d) from psi0 (device, d complex128) over times (device,
- * ntimes).  c_ops / e_ops: nchan / neops CSR operators concatenated, with
+ * c^dag c, d x d) from psi0 (device, d complex128) over times (device,
+ * ntimes).  H_eff is a column-major ELL of width h_width (the largest row
+ * length rounded up to a multiple of 4): entry u of row r at h_col / h_vals
+ * [u d + r], shorter rows padded with value 0 and any column in [0, d).  c_ops / e_ops: nchan / neops CSR operators concatenated, with
  * absolute row pointers (one block of d + 1 per operator).  Operator values
  * are complex64 when fp32 != 0 (then the state and RHS are complex64 too;
  * norms and expectation values stay FP64).  draws (device, ntraj x
@@ -261,7 +263,7 @@ int qa_mc_pack_tableau(void *out, int order, int rk_step, int rk_extra, int dord
  * -10 collapse time not found, -11 draws exhausted (re-run with more).
  * grid <= 0 launches one workgroup per trajectory.  d <= 1024. */
 int qa_mc_run(int d, int ntraj, int ntimes, const double *times_dev, const void *psi0_dev,
-              const int *h_rowptr, const int *h_col, const void *h_vals, int nchan,
+              int h_width, const int *h_col, const void *h_vals, int nchan,
               const int *c_rowptr, const int *c_col, const void *c_vals, int neops,
               const int *e_rowptr, const int *e_col, const void *e_vals,
               const void *tableau_dev, const double *draws_dev, int max_draws,

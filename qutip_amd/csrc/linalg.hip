@@ -77,7 +77,9 @@ __global__ __launch_bounds__(256) void sell_matvec_kernel(int nrows, int nslices
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int slice = (int)(t / QA_WAVE), lane = (int)(t % QA_WAVE);
     if (slice >= nslices) return;
-    const double2 acc = sell_row_dot<L>(slice_ptr, col, vals, x, slice, lane);
+    int base, steps;
+    sell_slice(slice_ptr, slice, base, steps);
+    const double2 acc = sell_row_dot<L>(col, vals, x, base, steps, lane);
     const int64_t r = (int64_t)slice * (QA_WAVE / L) + lane / L;
     if (lane % L == 0 && r < nrows) {
         double2 o = cmul(scale, acc);

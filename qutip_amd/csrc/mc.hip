@@ -22,9 +22,11 @@
 //
 // Layout: each thread owns one or two rows of the state (d <= 1024); the RK
 // stage vectors k_0..k_9 stay in registers and k_10..k_15 in LDS for the
-// whole trajectory; the vector an RHS product reads is staged in LDS; operators
-// (H_eff, c_ops, e_ops) are CSR in HBM, shared by all trajectories and served
-// from L2.  Norms, error norms and expectation values accumulate in FP64;
+// whole trajectory; the vector an RHS product reads is staged in LDS.  H_eff is
+// a column-major ELL in HBM (entry u of row r at u d + r), so the 64 lanes of
+// a wave, which own 64 consecutive rows, load 64 consecutive columns and
+// values per instruction; c_ops and e_ops (used at jumps and outputs only)
+// are CSR.  All operators are shared by all trajectories and served from L2.  Norms, error norms and expectation values accumulate in FP64;
 // with the complex64 variant the state, stages and RHS products are FP32.
 #include <math.h>
 #include <string.h>
@@ -65,7 +67,8 @@ struct McTableau {
 
 struct McArgs {
     int d, ntraj, ntimes, nchan, neops, max_draws, max_jumps, traj_offset;
-    const int *h_rowptr, *h_col;
+    int h_width;       // ELL width of H_eff
+    const int *h_col;  // [h_width][d] column-major ELL (entry u of row r at u d + r)
     const void *h_vals;
     const int *c_rowptr, *c_col;  // nchan blocks of d + 1 row pointers (absolute)
     const void *c_vals;
@@ -176,7 +179,7 @@ struct Ctx {
     const McTableau &T;
     int row[NR];
     bool act[NR];
-    int h_beg[NR], h_end[NR];
+    int rowc[NR];  // the row clamped into [0, d) for the ELL loads
 };
 
 // Stage vector j; j is a compile-time constant after unrolling, so the
@@ -221,37 +224,44 @@ __device__ __forceinline__ void stage_in(Ctx<R, NR> &c, const V<R, NR> &x) {
     __syncthreads();
 }
 
-// k = H_eff x for this thread's rows; x is staged in LDS first.  The rows'
-// loads are interleaved (both rows' columns, then values, then gathers).
+// M ELL entries (u0 .. u0 + M - 1) of this thread's rows: all column and
+// value loads first, then the LDS gathers and multiply-adds.
+template <int M, typename R, int NR>
+__device__ __forceinline__ void ell_chunk(Ctx<R, NR> &c, int u0, V<R, NR> &acc) {
+    const C2<R> *hv = (const C2<R> *)c.A.h_vals;
+    const size_t d = (size_t)c.A.d;
+    int cc[NR][M];
+    C2<R> vv[NR][M];
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+#pragma unroll
+        for (int w = 0; w < M; ++w) cc[q][w] = c.A.h_col[(size_t)(u0 + w) * d + c.rowc[q]];
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+#pragma unroll
+        for (int w = 0; w < M; ++w) vv[q][w] = hv[(size_t)(u0 + w) * d + c.rowc[q]];
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+#pragma unroll
+        for (int w = 0; w < M; ++w) cfmaR<R>(acc.v[q], vv[q][w], c.S.x[cc[q][w]]);
+}
+
+// k = H_eff x for this thread's rows; x is staged in LDS first.  The ELL
+// width is a multiple of 4 (the host pads it), so entries go in chunks of 8
+// (4 for two complex128 rows per thread, which leaves room for the stage
+// vectors in registers) and at most one chunk of 4; no load is predicated.
 template <typename R, int NR>
 __device__ __forceinline__ V<R, NR> rhs(Ctx<R, NR> &c, const V<R, NR> &x) {
+    constexpr int K = (NR == 2 && sizeof(R) == 8) ? 4 : 8;
     stage_in(c, x);
-    const C2<R> *hv = (const C2<R> *)c.A.h_vals;
     V<R, NR> acc = vzero<R, NR>();
-    int p[NR];
-    int maxlen = 0;
+    const int W = c.A.h_width;
+    int u = 0;
+    for (; u + K <= W; u += K) ell_chunk<K>(c, u, acc);
+    if (K == 8 && u < W) ell_chunk<4>(c, u, acc);
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-        p[q] = c.h_beg[q];
-        maxlen = max(maxlen, c.h_end[q] - c.h_beg[q]);
-    }
-    for (int u = 0; u < maxlen; u += 4) {
-        int cc[NR][4];
-        C2<R> vv[NR][4];
-#pragma unroll
-        for (int q = 0; q < NR; ++q)
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                const int pp = p[q] + u + w;
-                const bool ok = pp < c.h_end[q];
-                cc[q][w] = ok ? c.A.h_col[pp] : 0;
-                vv[q][w] = ok ? hv[pp] : mk<R>(0, 0);
-            }
-#pragma unroll
-        for (int q = 0; q < NR; ++q)
-#pragma unroll
-            for (int w = 0; w < 4; ++w) cfmaR<R>(acc.v[q], vv[q][w], c.S.x[cc[q][w]]);
-    }
+    for (int q = 0; q < NR; ++q)
+        if (!c.act[q]) acc.v[q] = mk<R>(0, 0);  // padding rows stay zero
     return acc;
 }
 
@@ -579,8 +589,7 @@ __global__ __launch_bounds__(kMcThreads) void mc_traj_kernel(McArgs A) {
     for (int q = 0; q < NR; ++q) {
         c.row[q] = threadIdx.x + q * nt;
         c.act[q] = c.row[q] < A.d;
-        c.h_beg[q] = c.act[q] ? A.h_rowptr[c.row[q]] : 0;
-        c.h_end[q] = c.act[q] ? A.h_rowptr[c.row[q] + 1] : 0;
+        c.rowc[q] = c.act[q] ? c.row[q] : 0;
     }
 
     for (int traj = blockIdx.x; traj < A.ntraj; traj += gridDim.x) {
@@ -602,9 +611,10 @@ __global__ __launch_bounds__(kMcThreads) void mc_traj_kernel(McArgs A) {
         for (int k = 1; k < A.ntimes && s.status == MC_OK; ++k) {
             const double tk = A.times[k];
             // MCIntegrator.integrate (mcsolve.py:286-302)
+            // (the state at t_old is always s.y: every path below that moves
+            // t_old also moves s.y there)
             double t_old = s.t;
             double norm_old = norm2(c, s.y);
-            V<R, NR> y_old = s.y;
             long long guard = 0;
             while (t_old < tk && s.status == MC_OK) {
                 if (++guard > 100000) {
@@ -701,16 +711,14 @@ __global__ __launch_bounds__(kMcThreads) void mc_traj_kernel(McArgs A) {
                     }
                     set_initial_value(c, s, ynew, t_guess);
                     t_old = s.t;
-                    y_old = s.y;
                     norm_old = 1.0;
                 } else {
                     t_old = t_step;
-                    y_old = s.y;
                     norm_old = nrm;
                 }
             }
             if (s.status != MC_OK) break;
-            write_expect(c, y_old, traj, k);
+            write_expect(c, s.y, traj, k);
         }
         if (threadIdx.x == 0) {
             A.info[(size_t)traj * 4 + 0] = s.status;
@@ -794,7 +802,7 @@ int qa_mc_pack_tableau(void *out, int order, int rk_step, int rk_extra, int dord
 }
 
 int qa_mc_run(int d, int ntraj, int ntimes, const double *times_dev, const void *psi0_dev,
-              const int *h_rowptr, const int *h_col, const void *h_vals, int nchan,
+              int h_width, const int *h_col, const void *h_vals, int nchan,
               const int *c_rowptr, const int *c_col, const void *c_vals, int neops,
               const int *e_rowptr, const int *e_col, const void *e_vals,
               const void *tableau_dev, const double *draws_dev, int max_draws,
@@ -810,8 +818,11 @@ int qa_mc_run(int d, int ntraj, int ntimes, const double *times_dev, const void 
     QA_REQUIRE(neops >= 0 && neops <= kMcMaxEops, "qa_mc_run: neops <= %d", kMcMaxEops);
     QA_REQUIRE(max_draws >= 1 && max_jumps >= 1, "qa_mc_run: need max_draws, max_jumps >= 1");
     QA_REQUIRE(norm_steps >= 1 && nsteps >= 0, "qa_mc_run: bad step limits");
-    QA_REQUIRE(times_dev && psi0_dev && h_rowptr && tableau_dev && draws_dev && info_dev &&
-                   expect_out_dev && col_times_dev && col_which_dev && c_rowptr,
+    QA_REQUIRE(h_width >= 0 && h_width % 4 == 0 && h_width <= d + 3,
+               "qa_mc_run: h_width must be a multiple of 4, at most d + 3");
+    QA_REQUIRE(times_dev && psi0_dev && tableau_dev && draws_dev && info_dev &&
+                   expect_out_dev && col_times_dev && col_which_dev && c_rowptr &&
+                   (h_width == 0 || (h_col && h_vals)),
                "qa_mc_run: null pointer");
     if (ntraj == 0) return QA_OK;
     McArgs A;
@@ -823,7 +834,7 @@ int qa_mc_run(int d, int ntraj, int ntimes, const double *times_dev, const void 
     A.max_draws = max_draws;
     A.max_jumps = max_jumps;
     A.traj_offset = 0;
-    A.h_rowptr = h_rowptr;
+    A.h_width = h_width;
     A.h_col = h_col;
     A.h_vals = h_vals;
     A.c_rowptr = c_rowptr;

@@ -97,62 +97,62 @@ __device__ __forceinline__ bool last_block_done(unsigned int *counter) {
 // slice_ptr[s] + 64 (j / L) + L r + (j % L), so each load instruction of the
 // wave reads 64 consecutive values and columns; lane l = L r + q handles
 // entries q, q + L, ... of row r and the L partial sums are combined with
-// cross-lane shuffles.  Two accumulators keep four loads in flight.  The
-// result is valid in every lane of the row.
+// cross-lane shuffles.  The result is valid in every lane of the row.
+//
+// The slice bounds come from sell_slice(), which a kernel can call before
+// other loads (the control-block read of the RK stage kernels) so that the
+// two round trips overlap.
+__device__ __forceinline__ void sell_slice(const int *__restrict__ slice_ptr, int slice,
+                                           int &base, int &steps) {
+    base = slice_ptr[slice];
+    steps = (slice_ptr[slice + 1] - base) / QA_WAVE;
+}
+
+// M consecutive entries (steps k .. k + M - 1) of this lane: all column and
+// value loads are issued, then all gathers, then the multiply-adds, so the
+// chunk costs one column round trip and one gather round trip.
+template <int M, typename V>
+__device__ __forceinline__ void sell_chunk(const int *__restrict__ col, const V *__restrict__ vals,
+                                           const double2 *__restrict__ x, int p, double2 &a0,
+                                           double2 &a1) {
+    int c[M];
+    double2 v[M], xv[M];
+#pragma unroll
+    for (int u = 0; u < M; ++u) c[u] = col[p + u * QA_WAVE];
+#pragma unroll
+    for (int u = 0; u < M; ++u) v[u] = ld(vals, p + u * QA_WAVE);
+#pragma unroll
+    for (int u = 0; u < M; ++u) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < M; ++u) cfma((u & 1) ? a1 : a0, v[u], xv[u]);
+}
+
 template <int L, typename V>
-__device__ __forceinline__ double2 sell_row_dot(const int *__restrict__ slice_ptr,
-                                                const int *__restrict__ col,
+__device__ __forceinline__ double2 sell_row_dot(const int *__restrict__ col,
                                                 const V *__restrict__ vals,
-                                                const double2 *__restrict__ x, int slice, int lane) {
+                                                const double2 *__restrict__ x, int base,
+                                                int steps, int lane) {
     // Every lane of a slice makes the same number of steps (the slice width
-    // divided by L), so the loop below is not divergent.  Eight entries are
-    // loaded before any is used: their column loads, value loads and x
-    // gathers overlap instead of forming a chain of dependent memory round
-    // trips.
-    const int base = slice_ptr[slice];
-    const int steps = (slice_ptr[slice + 1] - base) / QA_WAVE;
+    // divided by L), so the loop below is not divergent.  Entries go in
+    // chunks of eight; the last chunk (1..8 entries, a wave-uniform count)
+    // is dispatched once to a fully unrolled body of exactly that length, so
+    // no load is predicated and none waits for another.
     const int p0 = base + lane;
     double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
     int k = 0;
-    for (; k + 8 <= steps; k += 8) {
-        int c[8];
-        double2 v[8], xv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) c[u] = col[p0 + (k + u) * QA_WAVE];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = ld(vals, p0 + (k + u) * QA_WAVE);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
-#pragma unroll
-        for (int u = 0; u < 8; u += 2) {
-            cfma(a0, v[u], xv[u]);
-            cfma(a1, v[u + 1], xv[u + 1]);
-        }
+    for (; k + 8 < steps; k += 8) sell_chunk<8>(col, vals, x, p0 + k * QA_WAVE, a0, a1);
+    const int p = p0 + k * QA_WAVE;
+    switch (steps - k) {
+        case 8: sell_chunk<8>(col, vals, x, p, a0, a1); break;
+        case 7: sell_chunk<7>(col, vals, x, p, a0, a1); break;
+        case 6: sell_chunk<6>(col, vals, x, p, a0, a1); break;
+        case 5: sell_chunk<5>(col, vals, x, p, a0, a1); break;
+        case 4: sell_chunk<4>(col, vals, x, p, a0, a1); break;
+        case 3: sell_chunk<3>(col, vals, x, p, a0, a1); break;
+        case 2: sell_chunk<2>(col, vals, x, p, a0, a1); break;
+        case 1: sell_chunk<1>(col, vals, x, p, a0, a1); break;
+        default: break;
     }
-    if (k + 4 <= steps) {
-        int c[4];
-        double2 v[4], xv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) c[u] = col[p0 + (k + u) * QA_WAVE];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = ld(vals, p0 + (k + u) * QA_WAVE);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) xv[u] = x[c[u]];
-        cfma(a0, v[0], xv[0]);
-        cfma(a1, v[1], xv[1]);
-        cfma(a0, v[2], xv[2]);
-        cfma(a1, v[3], xv[3]);
-        k += 4;
-    }
-    if (k + 2 <= steps) {
-        const int c0 = col[p0 + k * QA_WAVE], c1 = col[p0 + (k + 1) * QA_WAVE];
-        const double2 v0 = ld(vals, p0 + k * QA_WAVE), v1 = ld(vals, p0 + (k + 1) * QA_WAVE);
-        const double2 x0 = x[c0], x1 = x[c1];
-        cfma(a0, v0, x0);
-        cfma(a1, v1, x1);
-        k += 2;
-    }
-    if (k < steps) cfma(a0, ld(vals, p0 + k * QA_WAVE), x[col[p0 + k * QA_WAVE]]);
     double2 acc = cadd(a0, a1);
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) {

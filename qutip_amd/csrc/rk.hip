@@ -276,11 +276,17 @@ __device__ __forceinline__ double epi_post(const EpiArgs &P, const RKCtl *ctl, c
 template <typename V, int L>
 __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
     RKCtl *ctl = P.ctl;
-    if (ctl->phase != P.run_phase) return;  // uniform over the whole grid
-    const StageVecs v = stage_vecs(P, ctl);
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int slice = (int)(t / QA_WAVE), lane = (int)(t % QA_WAVE);
     const int64_t r = (int64_t)slice * (QA_WAVE / L) + lane / L;  // the row of this lane
+    // The slice bounds and the control block are read before any branch,
+    // so their round trips overlap.
+    const int sl = slice < P.nslices ? slice : P.nslices - 1;
+    const int sp0 = P.slice_ptr[sl], sp1 = P.slice_ptr[sl + 1];
+    const int phase = ctl->phase;
+    const StageVecs v = stage_vecs(P, ctl);
+    if (phase != P.run_phase) return;  // uniform over the whole grid
+    const int base = sp0, steps = (sp1 - sp0) / QA_WAVE;
     double part = 0.0;
     if (slice < P.nslices) {
         // Epilogue terms that do not depend on this stage's product are formed
@@ -288,7 +294,7 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
         const bool owner = (lane % L == 0);
         double2 o = make_double2(0.0, 0.0), er = make_double2(0.0, 0.0);
         if (owner) epi_pre(P, v, r, o, er);
-        const double2 acc = sell_row_dot<L>(P.slice_ptr, P.col, P.vals, v.xin, slice, lane);
+        const double2 acc = sell_row_dot<L>(P.col, P.vals, v.xin, base, steps, lane);
         if (owner) part = epi_post(P, ctl, v, r, acc, o, er);
     }
     if (P.epi == EPI_FINAL) {
@@ -314,6 +320,7 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 struct DenseOp {
     int N, nc, prec;  // prec 0: FP32 MFMA, 1: bf16 x3 split
+    int nsplit;       // waves per 16 x 16 tile (split K)
     const float2 *Hm, *Hp, *C, *Cd;  // N x N each; C / Cd: nc blocks
     float2 *T;                        // nc blocks of scratch
 };
@@ -325,11 +332,11 @@ __device__ __forceinline__ float2 tof(float2 a) { return a; }
 template <typename TA, typename TB>
 __device__ __forceinline__ void zmma_f32(v4f &re, v4f &im, const TA *__restrict__ A,
                                          const TB *__restrict__ B, int N, int i0, int j0,
-                                         int lane) {
+                                         int lane, int k0, int k1) {
     const int r = lane & 15, kq = lane >> 4;
     const TA *pa = A + (size_t)kq * N + i0 + r;
     const TB *pb = B + (size_t)(j0 + r) * N + kq;
-    for (int kk = 0; kk < N; kk += 16) {
+    for (int kk = k0; kk < k1; kk += 16) {
         float2 a[4], b[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -355,9 +362,9 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
 template <typename TA, typename TB>
 __device__ __forceinline__ void zmma_bf16x3(v4f &re, v4f &im, const TA *__restrict__ A,
                                             const TB *__restrict__ B, int N, int i0, int j0,
-                                            int lane) {
+                                            int lane, int k0, int k1) {
     const int r = lane & 15, kq = lane >> 4;
-    for (int kk = 0; kk < N; kk += 32) {
+    for (int kk = k0; kk < k1; kk += 32) {
         v8bf arh, arl, aih, ail, naih, nail, brh, brl, bih, bil;
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -396,13 +403,22 @@ __device__ __forceinline__ void zmma_bf16x3(v4f &re, v4f &im, const TA *__restri
     }
 }
 
+// The K range of wave w of a workgroup of `nsplit` waves: the tile's sum is
+// split over the waves (more waves in flight, shorter serial loops) and the
+// partial tiles are added through LDS in wave order (deterministic).
+__device__ __forceinline__ void k_range(int N, int nsplit, int w, int &k0, int &k1) {
+    const int len = N / nsplit;
+    k0 = w * len;
+    k1 = k0 + len;
+}
+
 template <typename TA, typename TB>
 __device__ __forceinline__ void zmma(const DenseOp &D, v4f &re, v4f &im, const TA *A, const TB *B,
-                                     int i0, int j0, int lane) {
+                                     int i0, int j0, int lane, int k0, int k1) {
     if (D.prec == 1)
-        zmma_bf16x3(re, im, A, B, D.N, i0, j0, lane);
+        zmma_bf16x3(re, im, A, B, D.N, i0, j0, lane, k0, k1);
     else
-        zmma_f32(re, im, A, B, D.N, i0, j0, lane);
+        zmma_f32(re, im, A, B, D.N, i0, j0, lane, k0, k1);
 }
 
 // Element (i0 + 4 (lane / 16) + v, j0 + lane % 16) of a 16 x 16 MFMA tile
@@ -412,47 +428,90 @@ __device__ __forceinline__ int64_t tile_index(int N, int i0, int j0, int lane, i
     return (int64_t)(j0 + (lane & 15)) * N + i0 + 4 * (lane >> 4) + v;
 }
 
-// T_c = c X for every collapse operator (grid: N/16, N/16, nc).
-__device__ __forceinline__ void dense_t_tile(const DenseOp &D, const double2 *X, int lane) {
+constexpr int kMaxSplit = 4;
+
+// Sum the per-wave partial tiles into wave 0 (fixed order), via LDS.
+__device__ __forceinline__ void reduce_split(v4f &re, v4f &im, int lane, int w, int nsplit,
+                                             float *lds) {
+    if (nsplit == 1) return;
+    __syncthreads();
+    if (w > 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            lds[((w - 1) * 8 + v) * 64 + lane] = re[v];
+            lds[((w - 1) * 8 + 4 + v) * 64 + lane] = im[v];
+        }
+    }
+    __syncthreads();
+    if (w == 0) {
+        for (int q = 1; q < nsplit; ++q) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                re[v] += lds[((q - 1) * 8 + v) * 64 + lane];
+                im[v] += lds[((q - 1) * 8 + 4 + v) * 64 + lane];
+            }
+        }
+    }
+}
+
+// T_c = c X for every collapse operator (grid: N/16, N/16, nc; nsplit waves).
+__device__ __forceinline__ void dense_t_tile(const DenseOp &D, const double2 *X) {
+    __shared__ float lds[(kMaxSplit - 1) * 8 * 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nsplit = blockDim.x >> 6;
     const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16, c = blockIdx.z;
     const size_t NN = (size_t)D.N * D.N;
+    int k0, k1;
+    k_range(D.N, nsplit, w, k0, k1);
     v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
-    zmma(D, re, im, D.C + c * NN, X, i0, j0, lane);
+    zmma(D, re, im, D.C + c * NN, X, i0, j0, lane, k0, k1);
+    reduce_split(re, im, lane, w, nsplit, lds);
+    if (w == 0) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) D.T[c * NN + tile_index(D.N, i0, j0, lane, v)] = make_float2(re[v], im[v]);
+        for (int v = 0; v < 4; ++v)
+            D.T[c * NN + tile_index(D.N, i0, j0, lane, v)] = make_float2(re[v], im[v]);
+    }
 }
 
-// k tile = Hm X + X Hp + sum_c T_c Cd_c
-__device__ __forceinline__ void dense_k_tile(const DenseOp &D, const double2 *X, int lane, v4f &re,
+// k tile = Hm X + X Hp + sum_c T_c Cd_c; the result is complete in wave 0.
+__device__ __forceinline__ void dense_k_tile(const DenseOp &D, const double2 *X, v4f &re,
                                              v4f &im) {
+    __shared__ float lds[(kMaxSplit - 1) * 8 * 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nsplit = blockDim.x >> 6;
     const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
     const size_t NN = (size_t)D.N * D.N;
-    zmma(D, re, im, D.Hm, X, i0, j0, lane);
-    zmma(D, re, im, X, D.Hp, i0, j0, lane);
-    for (int c = 0; c < D.nc; ++c) zmma(D, re, im, D.T + c * NN, D.Cd + c * NN, i0, j0, lane);
+    int k0, k1;
+    k_range(D.N, nsplit, w, k0, k1);
+    zmma(D, re, im, D.Hm, X, i0, j0, lane, k0, k1);
+    zmma(D, re, im, X, D.Hp, i0, j0, lane, k0, k1);
+    for (int c = 0; c < D.nc; ++c)
+        zmma(D, re, im, D.T + c * NN, D.Cd + c * NN, i0, j0, lane, k0, k1);
+    reduce_split(re, im, lane, w, nsplit, lds);
 }
 
-__global__ __launch_bounds__(64) void rk_dense_t_kernel(EpiArgs P, DenseOp D) {
+__global__ __launch_bounds__(256) void rk_dense_t_kernel(EpiArgs P, DenseOp D) {
     if (P.ctl->phase != P.run_phase) return;
     const StageVecs v = stage_vecs(P, P.ctl);
-    dense_t_tile(D, v.xin, threadIdx.x);
+    dense_t_tile(D, v.xin);
 }
 
-__global__ __launch_bounds__(64) void rk_dense_stage_kernel(EpiArgs P, DenseOp D) {
+__global__ __launch_bounds__(256) void rk_dense_stage_kernel(EpiArgs P, DenseOp D) {
     RKCtl *ctl = P.ctl;
     if (ctl->phase != P.run_phase) return;
     const StageVecs v = stage_vecs(P, ctl);
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
     double2 o[4], er[4];
+    if (w == 0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-        epi_pre(P, v, tile_index(D.N, blockIdx.x * 16, blockIdx.y * 16, lane, q), o[q], er[q]);
+        for (int q = 0; q < 4; ++q) epi_pre(P, v, tile_index(D.N, i0, j0, lane, q), o[q], er[q]);
+    }
     v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
-    dense_k_tile(D, v.xin, lane, re, im);
+    dense_k_tile(D, v.xin, re, im);
+    if (w != 0) return;
     double part = 0.0;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-        part += epi_post(P, ctl, v, tile_index(D.N, blockIdx.x * 16, blockIdx.y * 16, lane, q),
+        part += epi_post(P, ctl, v, tile_index(D.N, i0, j0, lane, q),
                          make_double2((double)re[q], (double)im[q]), o[q], er[q]);
     if (P.epi == EPI_FINAL) {
         part = wave_sum(part);
@@ -461,16 +520,17 @@ __global__ __launch_bounds__(64) void rk_dense_stage_kernel(EpiArgs P, DenseOp D
 }
 
 // Ungated versions: out = L(X) (the data-layer product of the matrix form).
-__global__ __launch_bounds__(64) void dense_t_kernel(DenseOp D, const double2 *X) {
-    dense_t_tile(D, X, threadIdx.x);
+__global__ __launch_bounds__(256) void dense_t_kernel(DenseOp D, const double2 *X) {
+    dense_t_tile(D, X);
 }
 
-__global__ __launch_bounds__(64) void dense_rhs_kernel(DenseOp D, const double2 *X, double2 *out) {
+__global__ __launch_bounds__(256) void dense_rhs_kernel(DenseOp D, const double2 *X, double2 *out) {
     v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
-    dense_k_tile(D, X, threadIdx.x, re, im);
+    dense_k_tile(D, X, re, im);
+    if ((threadIdx.x >> 6) != 0) return;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-        out[tile_index(D.N, blockIdx.x * 16, blockIdx.y * 16, threadIdx.x, q)] =
+        out[tile_index(D.N, blockIdx.x * 16, blockIdx.y * 16, threadIdx.x & 63, q)] =
             make_double2((double)re[q], (double)im[q]);
 }
 
@@ -844,12 +904,13 @@ int launch_dense_stage(qa_rk_solver *S, int i) {
     EpiArgs P;
     fill_epi(P, S, i);
     const int nt = S->dop.N / 16;
+    const dim3 block(64 * S->dop.nsplit);
     if (S->dop.nc > 0) {
-        hipLaunchKernelGGL(rk_dense_t_kernel, dim3(nt, nt, S->dop.nc), dim3(64), 0, S->stream, P,
+        hipLaunchKernelGGL(rk_dense_t_kernel, dim3(nt, nt, S->dop.nc), block, 0, S->stream, P,
                            S->dop);
         QA_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(rk_dense_stage_kernel, dim3(nt, nt), dim3(64), 0, S->stream, P, S->dop);
+    hipLaunchKernelGGL(rk_dense_stage_kernel, dim3(nt, nt), block, 0, S->stream, P, S->dop);
     QA_CHECK_LAUNCH();
     return QA_OK;
 }
@@ -863,12 +924,13 @@ int launch_stage(qa_rk_solver *S, int i) {
 int engine_rhs(qa_rk_solver *S, const double2 *x, double2 *out) {
     if (S->kind == 1) {
         const int nt = S->dop.N / 16;
+        const dim3 block(64 * S->dop.nsplit);
         if (S->dop.nc > 0) {
-            hipLaunchKernelGGL(dense_t_kernel, dim3(nt, nt, S->dop.nc), dim3(64), 0, S->stream,
+            hipLaunchKernelGGL(dense_t_kernel, dim3(nt, nt, S->dop.nc), block, 0, S->stream,
                                S->dop, x);
             QA_CHECK_LAUNCH();
         }
-        hipLaunchKernelGGL(dense_rhs_kernel, dim3(nt, nt), dim3(64), 0, S->stream, S->dop, x, out);
+        hipLaunchKernelGGL(dense_rhs_kernel, dim3(nt, nt), block, 0, S->stream, S->dop, x, out);
         QA_CHECK_LAUNCH();
         return QA_OK;
     }
@@ -1176,6 +1238,14 @@ int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, co
     S->dop.N = N;
     S->dop.nc = nc;
     S->dop.prec = precision;
+    // split K over up to 4 waves while every wave keeps whole K steps
+    // (16 for FP32, 32 for bf16 x3)
+    {
+        const int kstep = precision == 1 ? 32 : 16;
+        int ns = kMaxSplit;
+        while (ns > 1 && N % (ns * kstep) != 0) ns /= 2;
+        S->dop.nsplit = ns;
+    }
     S->dop.Hm = d;
     S->dop.Hp = d + NN;
     S->dop.C = d + 2 * NN;

@@ -136,6 +136,28 @@ def _csr_block(ops, d, fp32):
             _data.DeviceBuffer.from_array(vl))
 
 
+def _ell_block(A, d, fp32):
+    """Column-major ELL of a d x d operator for the trajectory kernel: width W
+    = the longest row rounded up to a multiple of 4; entry u of row r at
+    [u * d + r]; shorter rows padded with value 0 and their own row index as
+    the column."""
+    A = sp.csr_matrix(A, dtype=np.complex128)
+    A.sort_indices()
+    if A.shape != (d, d):
+        raise ValueError(f"operator of shape {A.shape}, expected {(d, d)}")
+    lens = np.diff(A.indptr)
+    W = -(-int(lens.max()) // 4) * 4 if d else 0
+    col = np.tile(np.arange(d, dtype=np.int32), (max(W, 1), 1))
+    val = np.zeros((max(W, 1), d), dtype=np.complex128)
+    rows = np.repeat(np.arange(d), lens)
+    slot = np.arange(A.nnz) - np.repeat(A.indptr[:-1], lens)
+    col[slot, rows] = A.indices
+    val[slot, rows] = A.data
+    val = val.astype(np.complex64 if fp32 else np.complex128)
+    return (W, _data.DeviceBuffer.from_array(np.ascontiguousarray(col)),
+            _data.DeviceBuffer.from_array(np.ascontiguousarray(val)))
+
+
 class MCSolver:
     """Monte-Carlo wave-function solver, all trajectories in one persistent
     kernel.  ``options`` keys as qutip's MCSolver.solver_options
@@ -214,7 +236,7 @@ class MCSolver:
 
     def _device_ops(self, fp32):
         if self._dev is None or self._dev[0] != fp32:
-            h = _csr_block([self.H_eff], self.d, fp32)
+            h = _ell_block(self.H_eff, self.d, fp32)
             c = _csr_block([c.to_csr() for c in self.c_ops], self.d, fp32)
             tab = TABLEAUS[self.options["method"]]
             size = _lib.lib().qa_mc_tableau_size()
@@ -264,7 +286,7 @@ class MCSolver:
         H["info"] = _data.DeviceBuffer(16 * ntraj)
         L = _lib.lib()
         rc = L.qa_mc_run(
-            self.d, ntraj, ntimes, _vp(H["times"].ptr), _vp(H["p0"].ptr), _vp(h[0].ptr),
+            self.d, ntraj, ntimes, _vp(H["times"].ptr), _vp(H["p0"].ptr), int(h[0]),
             _vp(h[1].ptr), _vp(h[2].ptr), len(self.c_ops), _vp(c[0].ptr), _vp(c[1].ptr),
             _vp(c[2].ptr), neops, _vp(e[0].ptr), _vp(e[1].ptr), _vp(e[2].ptr), _vp(tab.ptr),
             _vp(dr.ptr), max_draws, float(o["rtol"]), float(o["atol"]), int(o["nsteps"]),

@@ -67,7 +67,8 @@ def test_ising8_mesolve_fp32_operator(golden):
                      options={**OPTS, "operator_precision": "fp32"})
     err = _max_err(res, g["expect"])
     print(f"ising8 fp32-operator max err {err:.3e}", flush=True)
-    assert err < 1e-5
+    # the <O> bar of the chosen precision form: 1e-6 against the FP64 fixture
+    assert err < 1e-6
 
 
 @pytest.mark.parametrize("tab,interp", [("tsit5", True), ("vern7", False)])
