@@ -67,7 +67,23 @@ def cpu_baseline(L_csr, rho0, tlist, e_ops_funcs):
     return dt, vals, st
 
 
-def bench_mcsolve(qa, systems, log, ntraj=2048, reps=3):
+def mc_cpu_baseline(ntraj=40):
+    """The restated reference trajectory loop (numpy/scipy vern7, one core)
+    on the first ``ntraj`` seeds of the same workload: trajectories/s."""
+    from oracle import mc as omc, models, tableaus
+    m = models.ising(10, 0.05, tf=20.0, npts=100)
+    t0 = time.perf_counter()
+    for i in range(ntraj):
+        tr = omc.Trajectory(m["H"], m["c_ops"], tableaus.vern7)
+        tr.run(m["psi0"], m["tlist"], np.random.default_rng(np.random.SeedSequence(i)),
+               m["e_ops"])
+    dt = time.perf_counter() - t0
+    return {"value": round(ntraj / dt, 3), "unit": "traj/s", "cores": 1, "kind": "port",
+            "sample": f"{ntraj} trajectories (seeds 0..{ntraj - 1}) of the same workload, "
+                      f"{dt:.1f} s"}
+
+
+def bench_mcsolve(qa, systems, log, ntraj=2048, reps=3, cpu=True):
     """Batched mcsolve: TFIM L=10 (d=1024, H_eff 11264 nnz, 10 collapse
     channels), 2048 trajectories with seeds 0..2047, t in [0,20], 100 outputs,
     vern7 at the integrator defaults; complex128 and complex64 (FP64 norms).
@@ -111,10 +127,13 @@ def bench_mcsolve(qa, systems, log, ntraj=2048, reps=3):
                      "max_dev_over_sigma_vs_mesolve": round(worst_sigma, 3)}
         log(f"mcsolve {prec}: {ntraj / best:.0f} traj/s ({best * 1e3:.1f} ms), worst "
             f"{worst_sigma:.2f} sigma, jumps {int(info[:, 1].sum())}")
+    if cpu:
+        out["cpu_baseline"] = mc_cpu_baseline()
+        log(f"mcsolve CPU baseline: {out['cpu_baseline']}")
     return out
 
 
-def bench_dense(qa, systems, log, reps=2):
+def bench_dense(qa, systems, log, reps=2, tf=None):
     """Matrix-form mesolve of the driven-dissipative Kerr oscillator, N=256
     (dense rho 256 x 256), t in [0, 30] with 150 outputs, e_ops a^dag a and
     parity, vern7 at rtol=1e-8 atol=1e-10, on MFMA in FP32 and in the bf16 x3
@@ -125,10 +144,14 @@ def bench_dense(qa, systems, log, reps=2):
     s = systems.kerr(256)
     g = np.load(ROOT / "tests" / "golden" / "kerr_full_mesolve.npz")
     tl = g["tlist"]
+    ref = g["expect"]
+    if tf is not None:  # a shorter window (profiling runs)
+        keep = tl <= tf
+        tl, ref = tl[keep], ref[:, keep]
     N = 256
     flop_rhs = 4 * 8 * N ** 3
     out = {"workload": "mesolve Kerr oscillator N=256 matrix form (dense rho 256x256), "
-                       "t in [0,30], 150 outputs, vern7 rtol=1e-8 atol=1e-10",
+                       f"t in [0,{tl[-1]:g}], {len(tl)} outputs, vern7 rtol=1e-8 atol=1e-10",
            "algorithmic_flop_per_rhs": flop_rhs}
     for prec in ("fp32", "bf16x3"):
         solver = qa.MESolver(s["H"], s["c_ops"], options={
@@ -139,12 +162,12 @@ def bench_dense(qa, systems, log, reps=2):
             res = solver.run(s["psi0"], tl, e_ops=s["e_ops"])
             _lib.call("qa_device_sync")
             ts.append(time.perf_counter() - t0)
-        err = max(np.abs(np.asarray(res.expect[k]) - g["expect"][k]).max() for k in range(2))
+        err = max(np.abs(np.asarray(res.expect[k]) - ref[k]).max() for k in range(2))
         st = solver._integrator.stats()
         eng = solver._integrator._engine
-        ms = np.zeros(16)
-        _lib.call("qa_rk_profile_stages", eng.handle, 10, ms.ctypes.data_as(C.POINTER(C.c_double)))
-        rhs_ms = float(ms[:10].mean())
+        t_step, t_dense = C.c_double(), C.c_double()
+        _lib.call("qa_rk_profile_sequence", eng.handle, 10, C.byref(t_step), C.byref(t_dense))
+        rhs_ms = t_step.value
         tf = flop_rhs / (rhs_ms * 1e-3) / 1e12
         best = min(ts)
         out[prec] = {"solves_per_s": round(1 / best, 4), "s_per_solve": round(best, 3),
@@ -198,43 +221,47 @@ def pmc_workload():
     """The launches the HBM-traffic counters are read on (run under
     rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE): three device-to-device
     copies of 64 MiB (the calibration: a plain copy reads and writes exactly
-    that many bytes), then 3 launches of each of the 16 stage kernels of the
-    L=8 Ising solve (direct launches, no graph)."""
-    import qutip_amd as qa
+    that many bytes), then, for each SELL shape (1, 2, 4, 8 lanes per row),
+    3 launches of each of the 16 stage kernels of the L=8 Ising solve in
+    solve order (direct launches, no graph)."""
     from qutip_amd import _lib, data as qd
     from qutip_amd.solver.hip_rk import RKEngine
     from tests import systems
+    import qutip_amd as qa
     s = systems.ising(8, 0.1)
-    solver = qa.MESolver(s["H"], s["c_ops"], options={"rtol": 1e-8, "atol": 1e-10})
-    solver.run(s["psi0"], s["tlist"][:3], e_ops=s["e_ops"])
-    L = solver.rhs.qobj.to_csr()
-    eng = RKEngine(L, "vern7", 1e-8, 1e-10, lanes=solver._integrator._engine.lanes)
-    rho0 = np.zeros(L.shape[0], dtype=np.complex128)
-    rho0[0] = 1.0
-    eng.set_state(0.0, qd.HipDense.from_array(rho0))
+    L = qa.MESolver(s["H"], s["c_ops"]).rhs.qobj.to_csr()
     a, b = qd.DeviceBuffer(COPY_BYTES), qd.DeviceBuffer(COPY_BYTES)
     _lib.call("qa_fill_zero", C.c_void_p(a.ptr), C.c_size_t(COPY_BYTES), None)
     for _ in range(3):
         _lib.call("qa_memcpy_d2d", C.c_void_p(b.ptr), C.c_void_p(a.ptr), C.c_size_t(COPY_BYTES),
                   None)
     _lib.call("qa_device_sync")
+    rho0 = np.zeros(L.shape[0], dtype=np.complex128)
+    rho0[0] = 1.0
     ms = np.zeros(16)
-    _lib.call("qa_rk_profile_stages", eng.handle, 3, ms.ctypes.data_as(C.POINTER(C.c_double)))
-    _lib.call("qa_device_sync")
-    log(f"pmc workload: 3 copies of {COPY_BYTES} B, 16 x 3 stage launches "
-        f"(lanes {eng.lanes}, nnz {L.nnz})")
+    for lanes in (1, 2, 4, 8):
+        eng = RKEngine(L, "vern7", 1e-8, 1e-10, lanes=lanes)
+        eng.set_state(0.0, qd.HipDense.from_array(rho0))
+        _lib.call("qa_rk_profile_stages", eng.handle, 3, ms.ctypes.data_as(C.POINTER(C.c_double)))
+        _lib.call("qa_device_sync")
+    log(f"pmc workload: 3 copies of {COPY_BYTES} B, then 16 x 3 stage launches for each of "
+        f"lanes 1, 2, 4, 8 (nnz {L.nnz})")
 
 
-def read_traffic():
+def read_traffic(lanes):
     """HBM bytes per stage-kernel launch from the committed counter summary
-    (FETCH_SIZE + WRITE_SIZE scaled by the copy calibration), or None."""
+    (FETCH_SIZE + WRITE_SIZE scaled by the copy calibration) when it was
+    taken on the same SELL shape, else None."""
+    vals = {}
     try:
         for line in TRAFFIC_FILE.read_text().splitlines():
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(float(line.split(":", 1)[1]))
+            if ":" in line and not line.startswith(("-", "|", "#", "`")):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    key = f"traffic_bytes_per_launch_lanes{lanes}"
+    return int(float(vals[key])) if key in vals else None
 
 
 def main():
@@ -248,6 +275,9 @@ def main():
     ap.add_argument("--no-dense", action="store_true", help="skip the dense Kerr leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--dense-tf", type=float, default=None,
+                    help="end the Kerr solve early (profiling runs; default t=30)")
+    ap.add_argument("--dense-reps", type=int, default=2)
     ap.add_argument("--pmc", action="store_true",
                     help="only run the launches the HBM-traffic counters are read on")
     args = ap.parse_args()
@@ -292,20 +322,24 @@ def main():
     value = args.steps / total
     log(f"fused: {value:.2f} solves/s, err {err:.2e}, stats {stats}")
 
-    # roofline of the stage kernel (HIP events on the engine's stream)
+    # roofline of the stage kernel: HIP events on the engine's stream around
+    # back-to-back rounds of the stage launches in solve order
     lanes = solver._integrator._engine.lanes
     eng = RKEngine(L, "vern7", 1e-8, 1e-10, lanes=lanes)
-    ms = np.zeros(16)
-    _lib.call("qa_rk_profile_stages", eng.handle, 20, ms.ctypes.data_as(C.POINTER(C.c_double)))
+    rho0 = np.zeros(n, dtype=np.complex128)
+    rho0[0] = 1.0
+    eng.set_state(0.0, rho0)
+    t_step, t_dense = C.c_double(), C.c_double()
+    _lib.call("qa_rk_profile_sequence", eng.handle, 20, C.byref(t_step), C.byref(t_dense))
     byts = stage_bytes(eng, nnz, n, False)
-    cnt = np.zeros(16)
-    cnt[:10] = stats["attempts"]
-    cnt[10:] = stats["dense_preps"]
-    avg_ms = float((cnt * ms).sum() / cnt.sum())
-    avg_bytes = float((cnt * byts).sum() / cnt.sum())
+    n_step = 10 * stats["attempts"]
+    n_dense = 6 * stats["dense_preps"]
+    avg_ms = (n_step * t_step.value + n_dense * t_dense.value) / (n_step + n_dense)
+    avg_bytes = float((stats["attempts"] * byts[:10].sum() + stats["dense_preps"] * byts[10:].sum())
+                      / (n_step + n_dense))
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    log(f"stage kernel avg {avg_ms * 1e3:.2f} us, {avg_bytes / 1e6:.2f} MB -> {achieved:.0f} GB/s;"
-        f" per stage us {np.round(ms * 1e3, 2).tolist()}")
+    log(f"stage kernel avg {avg_ms * 1e3:.2f} us (step stages {t_step.value * 1e3:.2f}, dense "
+        f"{t_dense.value * 1e3:.2f}), {avg_bytes / 1e6:.2f} MB -> {achieved:.0f} GB/s")
 
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,
@@ -323,7 +357,7 @@ def main():
         "err_vs_fp64": float(err),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic(), "kernel": f"rk_stage_kernel<double2, {lanes}>",
+                     "traffic": read_traffic(lanes), "kernel": f"rk_stage_kernel<double2, {lanes}>",
                      "avg_launch_us": round(avg_ms * 1e3, 3),
                      "algorithmic_bytes_per_launch": int(avg_bytes)},
     }
@@ -342,9 +376,9 @@ def main():
                             "rk_attempts": fsolver._integrator.stats()["attempts"]}
     log(f"fp32 operator: {1 / tf32:.2f} solves/s, err {ferr:.2e}")
     if not args.no_mc:
-        out["mcsolve"] = bench_mcsolve(qa, systems, log)
+        out["mcsolve"] = bench_mcsolve(qa, systems, log, cpu=not args.no_cpu)
     if not args.no_dense:
-        out["dense_kerr"] = bench_dense(qa, systems, log)
+        out["dense_kerr"] = bench_dense(qa, systems, log, reps=args.dense_reps, tf=args.dense_tf)
     if not args.no_serve:
         out["serving"] = bench_serving(systems, log, nreq=args.serve_requests)
     if not args.no_plain:

@@ -228,6 +228,13 @@ int qa_rk_stats(qa_rk_solver *s, int64_t *stats);
  * qa_rk_set_state afterwards. */
 int qa_rk_profile_stages(qa_rk_solver *s, int reps, double *ms_per_stage);
 
+/* Average duration (ms, one pair of HIP events on the engine stream around
+ * `reps` back-to-back rounds) of one stage-kernel launch, over the step stages
+ * in solve order (*ms_step) and over the dense-output stages (*ms_dense, 0
+ * without dense output).  Clobbers the integration state like
+ * qa_rk_profile_stages.  No reference counterpart (measurement). */
+int qa_rk_profile_sequence(qa_rk_solver *s, int reps, double *ms_step, double *ms_dense);
+
 /* Enqueue `count` loop iterations without the graph and wait (profiling). */
 int qa_rk_iterate(qa_rk_solver *s, int count);
 

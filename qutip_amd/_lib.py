@@ -77,6 +77,7 @@ SIGNATURES = {
     "qa_rk_stats": [vp, P_i64],
     "qa_rk_iterate": [vp, i32],
     "qa_rk_profile_stages": [vp, i32, P_f64],
+    "qa_rk_profile_sequence": [vp, i32, P_f64, P_f64],
     "qa_mc_tableau_size": [],
     "qa_mc_pack_tableau": [vp, i32, i32, i32, i32, P_f64, P_f64, P_f64, P_f64, P_f64, i32],
     "qa_mc_run": [i32, i32, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp,

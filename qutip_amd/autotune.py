@@ -4,7 +4,7 @@ The tile shape is the number of lanes per row L (1, 2, 4, 8; C = 64 / L
 rows per wavefront).  Few lanes per row keep the padding low, more lanes per
 row give more wavefronts in flight (latency hiding for the column gathers).
 The choice is measured: for each candidate a throw-away engine times its
-stage kernels with HIP events (qa_rk_profile_stages), and the fastest
+stage kernels with HIP events (qa_rk_profile_sequence), and the fastest
 average wins.  Results are cached per (dimension bucket, non-zeros per row
 bucket, value precision, batch); the cache lives in memory and, when
 QUTIP_AMD_TUNE_CACHE names a file, in that JSON file across processes.
@@ -52,24 +52,26 @@ def heuristic_lanes(nnz_row):
     return L
 
 
-def measure(A, fp32=False, candidates=CANDIDATES, reps=3, rounds=3):
+def measure(A, fp32=False, candidates=CANDIDATES, reps=8, rounds=4):
     """Average stage-kernel time (ms) per candidate lane count.  All candidate
     engines are built first, one untimed round warms the device, then the
     candidates are timed in interleaved rounds and the fastest round counts
-    (so clock ramps and cache state do not favour whichever runs last)."""
+    (so clock ramps and cache state do not favour whichever runs last).
+    Each round times ``reps`` back-to-back passes over the step stages in
+    solve order (qa_rk_profile_sequence)."""
     import ctypes as C
 
     from . import _lib
     from .solver.hip_rk import RKEngine
     engines = {L: RKEngine(A, "vern7", 1e-6, 1e-8, fp32=fp32, lanes=L) for L in candidates}
     best = {L: float("inf") for L in candidates}
-    ms = np.zeros(16)
+    t_step, t_dense = C.c_double(), C.c_double()
     for rnd in range(rounds + 1):
         for L, eng in engines.items():
-            _lib.call("qa_rk_profile_stages", eng.handle, reps,
-                      ms.ctypes.data_as(C.POINTER(C.c_double)))
+            _lib.call("qa_rk_profile_sequence", eng.handle, reps, C.byref(t_step),
+                      C.byref(t_dense))
             if rnd > 0:
-                best[L] = min(best[L], float(ms.mean()))
+                best[L] = min(best[L], t_step.value)
     return best
 
 

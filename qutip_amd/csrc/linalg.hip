@@ -129,7 +129,6 @@ __global__ __launch_bounds__(256) void axpby_kernel(int64_t n, double2 alpha,
 struct RedScratch {
     double *partials;  // 2 * maxblocks
     double *res;       // 2
-    unsigned int *counter;
 };
 
 __global__ __launch_bounds__(256) void reduce_kernel(int kind, int64_t n, const double2 *__restrict__ x,
@@ -161,14 +160,23 @@ __global__ __launch_bounds__(256) void reduce_kernel(int kind, int64_t n, const 
         S.partials[2 * blockIdx.x] = sre;
         S.partials[2 * blockIdx.x + 1] = sim;
     }
-    if (last_block_done(S.counter) && threadIdx.x == 0) {
-        double tre = 0.0, tim = 0.0;
-        for (unsigned b = 0; b < gridDim.x; ++b) {
-            tre += S.partials[2 * b];
-            tim += S.partials[2 * b + 1];
-        }
-        if (kind == 0) tre = sqrt(tre / (double)n);
-        S.res[0] = tre;
+}
+
+// Adds the per-workgroup partial sums in a fixed order (one workgroup; the
+// launch boundary orders it after the partial sums).  mean_sqrt: the result
+// is sqrt(sum / n) (the weighted RMS norm).
+__global__ __launch_bounds__(256) void reduce_finish_kernel(int nblocks, int mean_sqrt,
+                                                            int64_t n, RedScratch S) {
+    __shared__ double smem[8];
+    double tre = 0.0, tim = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        tre += S.partials[2 * b];
+        tim += S.partials[2 * b + 1];
+    }
+    tre = block_sum(tre, smem);
+    tim = block_sum(tim, smem);
+    if (threadIdx.x == 0) {
+        S.res[0] = mean_sqrt ? sqrt(tre / (double)n) : tre;
         S.res[1] = tim;
     }
 }
@@ -206,15 +214,6 @@ __global__ __launch_bounds__(256) void expect_csr_kernel(int kind, int nrows,
         S.partials[2 * blockIdx.x] = sre;
         S.partials[2 * blockIdx.x + 1] = sim;
     }
-    if (last_block_done(S.counter) && threadIdx.x == 0) {
-        double tre = 0.0, tim = 0.0;
-        for (unsigned b = 0; b < gridDim.x; ++b) {
-            tre += S.partials[2 * b];
-            tim += S.partials[2 * b + 1];
-        }
-        S.res[0] = tre;
-        S.res[1] = tim;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -241,7 +240,6 @@ int red_scratch(RedScratch *S) {
     char *b = (char *)st.buf;
     S->partials = (double *)b;
     S->res = (double *)(b + 2 * kMaxRedBlocks * sizeof(double));
-    S->counter = (unsigned int *)(b + (2 * kMaxRedBlocks + 2) * sizeof(double));
     return QA_OK;
 }
 
@@ -252,7 +250,11 @@ unsigned red_blocks(int64_t n) {
     return (unsigned)b;
 }
 
-int finish_reduction(RedScratch &S, double *result, int ncomp, hipStream_t s) {
+int finish_reduction(RedScratch &S, unsigned nblocks, int mean_sqrt, int64_t n, double *result,
+                     int ncomp, hipStream_t s) {
+    QA_CHECK_LAUNCH();
+    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(256), 0, s, (int)nblocks, mean_sqrt, n,
+                       S);
     QA_CHECK_LAUNCH();
     QA_HIP(hipMemcpyAsync(result, S.res, ncomp * sizeof(double), hipMemcpyDeviceToHost, s));
     QA_HIP(hipStreamSynchronize(s));
@@ -421,9 +423,10 @@ static int run_reduce(int kind, int64_t n, const void *x, const void *y, double 
     int rc = red_scratch(&S);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(reduce_kernel, dim3(red_blocks(n)), dim3(256), 0, s, kind, n,
-                       (const double2 *)x, (const double2 *)y, atol, rtol, stride, S);
-    return finish_reduction(S, result, ncomp, s);
+    const unsigned nb = red_blocks(n);
+    hipLaunchKernelGGL(reduce_kernel, dim3(nb), dim3(256), 0, s, kind, n, (const double2 *)x,
+                       (const double2 *)y, atol, rtol, stride, S);
+    return finish_reduction(S, nb, kind == 0, n, result, ncomp, s);
 }
 
 int qa_wrmn_error_z(int64_t n, const void *diff, const void *state, double atol, double rtol,
@@ -457,13 +460,14 @@ static int run_expect(int kind, int nrows, const int *rowptr, const int *col, co
     int rc = red_scratch(&S);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
+    const unsigned nb = red_blocks(nrows);
     if (vals_fp32)
-        hipLaunchKernelGGL(expect_csr_kernel<float2>, dim3(red_blocks(nrows)), dim3(256), 0, s,
-                           kind, nrows, rowptr, col, (const float2 *)vals, (const double2 *)st, S);
+        hipLaunchKernelGGL(expect_csr_kernel<float2>, dim3(nb), dim3(256), 0, s, kind, nrows,
+                           rowptr, col, (const float2 *)vals, (const double2 *)st, S);
     else
-        hipLaunchKernelGGL(expect_csr_kernel<double2>, dim3(red_blocks(nrows)), dim3(256), 0, s,
-                           kind, nrows, rowptr, col, (const double2 *)vals, (const double2 *)st, S);
-    return finish_reduction(S, result2, 2, s);
+        hipLaunchKernelGGL(expect_csr_kernel<double2>, dim3(nb), dim3(256), 0, s, kind, nrows,
+                           rowptr, col, (const double2 *)vals, (const double2 *)st, S);
+    return finish_reduction(S, nb, 0, nrows, result2, 2, s);
 }
 
 int qa_expect_csr_ket_z(int n, const int *rowptr, const int *col, const void *vals,

@@ -70,28 +70,6 @@ __device__ __forceinline__ double2 block_sum2(double2 v, double *smem) {
     return make_double2(re, im);
 }
 
-// "Last workgroup done" handshake: every workgroup publishes its partial
-// results, then increments `counter`; the workgroup that observes
-// gridDim.x - 1 finishes the reduction (classic threadfence reduction).  The
-// device-scope fences write the partials back past the per-XCD L2 and make
-// the last workgroup re-read them from memory.  The counter is reset by the
-// last workgroup, so the next kernel that uses it starts from zero.
-__device__ __forceinline__ bool last_block_done(unsigned int *counter) {
-    __shared__ int is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int prev = atomicAdd(counter, 1u);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
-        if (threadIdx.x == 0) atomicExch(counter, 0u);
-    }
-    return is_last != 0;
-}
-
 // SELL-C row dot product with L lanes per row (C = 64 / L rows per slice;
 // one wavefront per slice).  Entry j of row C s + r is stored at
 // slice_ptr[s] + 64 (j / L) + L r + (j % L), so each load instruction of the

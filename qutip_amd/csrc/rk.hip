@@ -61,7 +61,6 @@ struct RKCtl {
     int ntimes, phase, cur, out_idx, nsteps_left, nsteps_call, dense_ready, status;
     int order, max_numsteps, interpolate, adaptive, single_step, nfun;
     long long n_attempts, n_accepted, n_dense, n_outputs;
-    unsigned int counter;
 };
 
 // ---- step control (executed by one thread) ---------------------------------
@@ -197,9 +196,12 @@ struct EpiArgs {
     double *partials;
     RKCtl *ctl;
     int stage, run_phase, input, epi, nterms;
-    double coef_self, ecoef_self;  // coef[stage], ecoef[stage]
-    double coef[kMaxStages];
-    double ecoef[kMaxStages];
+    double coef_self, ecoef_self;  // coefficients of this stage's own k
+    // the earlier stages with a non-zero coefficient, compacted: k_{tj[t]}
+    // enters the next input / y_front with tc[t] and the error with te[t]
+    int nt;
+    int tj[kMaxStages];
+    double tc[kMaxStages], te[kMaxStages];
 };
 
 template <typename V>
@@ -229,24 +231,46 @@ __device__ __forceinline__ StageVecs stage_vecs(const EpiArgs &P, const RKCtl *c
 }
 
 // Epilogue, part 1 (before the product is known): o = y_prev + dt sum_{j <
-// stage} coef_j k_j and er = dt sum_{j < stage} ecoef_j k_j for element r.
+// stage} coef_j k_j and er = dt sum_{j < stage} ecoef_j k_j for element r,
+// over the compacted non-zero terms.  The term count is uniform; it selects
+// a body of exactly NT unconditional loads, so all of them are in flight at
+// once (a load behind a branch would wait for its own data before the next).
+template <int NT>
+__device__ __forceinline__ void epi_terms(const EpiArgs &P, double dt, int64_t r, double2 &o,
+                                          double2 &er) {
+    double2 kv[NT > 0 ? NT : 1];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) kv[t] = P.K[(int64_t)P.tj[t] * P.npad + r];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        if (P.tc[t] != 0.0) raxpy(o, dt * P.tc[t], kv[t]);
+        if (P.te[t] != 0.0) raxpy(er, dt * P.te[t], kv[t]);
+    }
+}
+
 __device__ __forceinline__ void epi_pre(const EpiArgs &P, const StageVecs &v, int64_t r,
                                         double2 &o, double2 &er) {
     o = make_double2(0.0, 0.0);
     er = make_double2(0.0, 0.0);
     if (P.epi == EPI_NONE) return;
     o = v.yprev[r];
-#pragma unroll
-    for (int j = 0; j < kMaxStages; ++j) {
-        if (j < P.stage) {
-            const double fb = v.dt * P.coef[j];
-            const double fe = v.dt * P.ecoef[j];
-            if (fb != 0.0 || fe != 0.0) {
-                const double2 kj = P.K[(int64_t)j * P.npad + r];
-                if (fb != 0.0) raxpy(o, fb, kj);
-                if (fe != 0.0) raxpy(er, fe, kj);
-            }
-        }
+    switch (P.nt) {
+        case 0: break;
+        case 1: epi_terms<1>(P, v.dt, r, o, er); break;
+        case 2: epi_terms<2>(P, v.dt, r, o, er); break;
+        case 3: epi_terms<3>(P, v.dt, r, o, er); break;
+        case 4: epi_terms<4>(P, v.dt, r, o, er); break;
+        case 5: epi_terms<5>(P, v.dt, r, o, er); break;
+        case 6: epi_terms<6>(P, v.dt, r, o, er); break;
+        case 7: epi_terms<7>(P, v.dt, r, o, er); break;
+        case 8: epi_terms<8>(P, v.dt, r, o, er); break;
+        case 9: epi_terms<9>(P, v.dt, r, o, er); break;
+        case 10: epi_terms<10>(P, v.dt, r, o, er); break;
+        case 11: epi_terms<11>(P, v.dt, r, o, er); break;
+        case 12: epi_terms<12>(P, v.dt, r, o, er); break;
+        case 13: epi_terms<13>(P, v.dt, r, o, er); break;
+        case 14: epi_terms<14>(P, v.dt, r, o, er); break;
+        default: epi_terms<15>(P, v.dt, r, o, er); break;
     }
 }
 
@@ -593,39 +617,46 @@ struct OutArgs {
 // front[q] = 1 when the output time equals t_front (the state is y_front).
 __device__ int pending_outputs(const RKCtl *c, const OutArgs &A, double (*w)[kMaxStages],
                                int *front, int *nq_s) {
-    if (threadIdx.x == 0) {
-        int nq = 0;
-        while (nq < kMaxPending && c->out_idx + nq < c->ntimes &&
-               c->times[c->out_idx + nq] <= c->t_front)
-            ++nq;
-        *nq_s = nq;
-    }
-    __syncthreads();
-    const int nq = *nq_s;
+    // Thread (q, i) reads output time q itself, so all the reads are one
+    // round trip; the count is the number of leading times inside the step.
+    __shared__ int okq[kMaxPending];
     const int t = threadIdx.x;
     if (t < kMaxPending * kMaxStages) {
         const int q = t / kMaxStages, i = t % kMaxStages;
+        const int idx = c->out_idx + q;
+        const double t_front = c->t_front, t_prev = c->t_prev, dt_int = c->dt_int;
+        const bool in_range = idx < c->ntimes;
+        const double tq = in_range ? c->times[idx] : 0.0;
+        const bool ok = in_range && tq <= t_front;
         double v = 0.0;
-        if (q < nq) {
-            const double tq = c->times[c->out_idx + q];
-            if (i == 0) front[q] = (tq == c->t_front);
-            if (tq != c->t_front && A.has_bi && i < A.nk) {
-                const double tau = (tq - c->t_prev) / c->dt_int;
-                for (int j = A.dorder - 1; j >= 0; --j) {
-                    v += A.bi[i * A.dorder + j];
-                    v *= tau;
-                }
-                v *= c->dt_int;
+        if (ok && tq != t_front && A.has_bi && i < A.nk) {
+            const double tau = (tq - t_prev) / dt_int;
+            for (int j = A.dorder - 1; j >= 0; --j) {
+                v += A.bi[i * A.dorder + j];
+                v *= tau;
             }
+            v *= dt_int;
         }
         w[q][i] = v;
+        if (i == 0) {
+            okq[q] = ok;
+            front[q] = ok && tq == t_front;
+        }
     }
     __syncthreads();
-    return nq;
+    if (t == 0) {
+        int nq = 0;
+        while (nq < kMaxPending && okq[nq]) ++nq;
+        *nq_s = nq;
+    }
+    __syncthreads();
+    return *nq_s;
 }
 
-// y(t_q)[idx]: y_front, or y_prev + sum_i w_i k_i[idx] with every load in
-// flight at once (the table is fully unrolled).
+// y(t_q)[idx]: y_front, or y_prev + sum_i w_i k_i[idx].  Every stage load is
+// unconditional (the stage index is clamped into [0, nk), so no load sits
+// behind a branch and all of them are in flight at once); stages without a
+// weight are dropped by a select, so what an unused slot holds never enters.
 __device__ __forceinline__ double2 state_at(const OutArgs &A, const double2 *yprev,
                                             const double2 *yfront, const double *w, int front,
                                             int64_t idx) {
@@ -633,11 +664,16 @@ __device__ __forceinline__ double2 state_at(const OutArgs &A, const double2 *ypr
     double2 o = yprev[idx];
     double2 kv[kMaxStages];
 #pragma unroll
-    for (int i = 0; i < kMaxStages; ++i)
-        kv[i] = (i < A.nk && w[i] != 0.0) ? A.K[(int64_t)i * A.npad + idx] : make_double2(0.0, 0.0);
+    for (int i = 0; i < kMaxStages; ++i) {
+        const int ii = i < A.nk ? i : A.nk - 1;
+        kv[i] = A.K[(int64_t)ii * A.npad + idx];
+    }
 #pragma unroll
-    for (int i = 0; i < kMaxStages; ++i)
-        if (i < A.nk && w[i] != 0.0) raxpy(o, w[i], kv[i]);
+    for (int i = 0; i < kMaxStages; ++i) {
+        const double wi = i < A.nk ? w[i] : 0.0;
+        const double2 kk = wi != 0.0 ? kv[i] : make_double2(0.0, 0.0);
+        raxpy(o, wi, kk);
+    }
     return o;
 }
 
@@ -673,10 +709,15 @@ __global__ __launch_bounds__(256) void rk_store_kernel(OutArgs A) {
 // Functionals of the pending output states: linear f = sum w y[col] (a
 // vectorised density matrix: tr(O rho), tr(rho)) or quadratic
 // f = sum w conj(y[row]) y[col] (a ket: <psi|O|psi>, ||psi||^2).  Grid
-// (nchunks, nfun); the last workgroup reduces in a fixed order, stores the
-// values and advances the output index.
+// (nchunks, nfun); each workgroup writes the partial sum of its chunk, which
+// rk_output_finish_kernel adds up.
 __global__ __launch_bounds__(256) void rk_output_kernel(OutArgs A) {
     RKCtl *c = A.ctl;
+    // the functional's range does not depend on the control block: it is
+    // read in the same round trip
+    const int f = blockIdx.y;
+    const int fbeg = A.fptr[f], fend = A.fptr[f + 1];
+    const int quad = A.fkind[f];
     const int ph = c->phase;
     if (ph != PH_DENSE && ph != PH_OUTPUT) return;
     __shared__ double w[kMaxPending][kMaxStages];
@@ -686,10 +727,8 @@ __global__ __launch_bounds__(256) void rk_output_kernel(OutArgs A) {
     const int nq = pending_outputs(c, A, w, front, &nq_s);
     const double2 *yprev = c->cur ? A.y1 : A.y0;
     const double2 *yfront = c->cur ? A.y0 : A.y1;
-    const int f = blockIdx.y;
-    const int beg = A.fptr[f] + blockIdx.x * kOutChunk;
-    const int end = min(A.fptr[f + 1], beg + kOutChunk);
-    const int quad = A.fkind[f];
+    const int beg = fbeg + blockIdx.x * kOutChunk;
+    const int end = min(fend, beg + kOutChunk);
     for (int q = 0; q < nq; ++q) {
         double2 acc = make_double2(0.0, 0.0);
         for (int e = beg + threadIdx.x; e < end; e += blockDim.x) {
@@ -700,36 +739,39 @@ __global__ __launch_bounds__(256) void rk_output_kernel(OutArgs A) {
             }
             acc = cadd(acc, v);
         }
-        const double2 s = block_sum2(acc, smem);
+        const double2 sq = block_sum2(acc, smem);
         if (threadIdx.x == 0)
-            A.partials[((int64_t)q * A.nfun + f) * A.nchunks + blockIdx.x] = s;
+            A.partials[((int64_t)q * A.nfun + f) * A.nchunks + blockIdx.x] = sq;
     }
-    // last-block handshake over the 2-D grid
-    __shared__ int is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned total = gridDim.x * gridDim.y;
-        is_last = (atomicAdd(&c->counter, 1u) == total - 1);
+}
+
+// Completes an output launch: sums the partial sums of every (output time,
+// functional) pair in a fixed order, stores them and advances the control
+// block.  A separate one-workgroup kernel (the launch boundary orders it
+// after the partial sums) instead of a last-block handshake, whose
+// device-scope fence and atomic cost more than a launch.
+__global__ __launch_bounds__(256) void rk_output_finish_kernel(OutArgs A) {
+    RKCtl *c = A.ctl;
+    const int ph = c->phase;
+    if (ph != PH_DENSE && ph != PH_OUTPUT) return;
+    __shared__ double w[kMaxPending][kMaxStages];
+    __shared__ int front[kMaxPending];
+    __shared__ int nq_s;
+    const int nq = pending_outputs(c, A, w, front, &nq_s);
+    for (int p = threadIdx.x; p < nq * A.nfun; p += blockDim.x) {
+        const int q = p / A.nfun, ff = p % A.nfun;
+        double2 t = make_double2(0.0, 0.0);
+        for (int k = 0; k < A.nchunks; ++k)
+            t = cadd(t, A.partials[((int64_t)q * A.nfun + ff) * A.nchunks + k]);
+        c->fun_out[(int64_t)(c->out_idx + q) * A.nfun + ff] = t;
     }
     __syncthreads();
-    if (!is_last) return;
-    __threadfence();
     if (threadIdx.x != 0) return;
-    atomicExch(&c->counter, 0u);
     if (nq == 0) {
         // a non-finite step left no output time inside [t_prev, t_front]
         c->status = ST_OUTSIDE_RANGE;
         c->phase = PH_ERROR;
         return;
-    }
-    for (int q = 0; q < nq; ++q) {
-        for (int ff = 0; ff < A.nfun; ++ff) {
-            double2 t = make_double2(0.0, 0.0);
-            for (int k = 0; k < A.nchunks; ++k)
-                t = cadd(t, A.partials[((int64_t)q * A.nfun + ff) * A.nchunks + k]);
-            c->fun_out[(int64_t)(c->out_idx + q) * A.nfun + ff] = t;
-        }
     }
     if (ph == PH_DENSE) {
         c->dense_ready = 1;
@@ -834,7 +876,8 @@ void fill_epi(EpiArgs &P, qa_rk_solver *S, int i) {
     P.partials = S->partials;
     P.ctl = S->ctl;
     P.stage = i;
-    for (int j = 0; j < kMaxStages; ++j) P.coef[j] = P.ecoef[j] = 0.0;
+    double coef[kMaxStages], ecoef[kMaxStages];
+    for (int j = 0; j < kMaxStages; ++j) coef[j] = ecoef[j] = 0.0;
     const int ne = S->rk_extra;
     if (i < S->rk_step) {
         P.run_phase = PH_STEP;
@@ -842,13 +885,13 @@ void fill_epi(EpiArgs &P, qa_rk_solver *S, int i) {
         if (i < S->rk_step - 1) {
             P.epi = EPI_NEXT;
             P.nterms = i + 1;
-            for (int j = 0; j <= i; ++j) P.coef[j] = S->a[(size_t)(i + 1) * ne + j];
+            for (int j = 0; j <= i; ++j) coef[j] = S->a[(size_t)(i + 1) * ne + j];
         } else {
             P.epi = EPI_FINAL;
             P.nterms = S->rk_step;
             for (int j = 0; j < S->rk_step; ++j) {
-                P.coef[j] = S->b[j];
-                P.ecoef[j] = S->adaptive ? S->e[j] : 0.0;
+                coef[j] = S->b[j];
+                ecoef[j] = S->adaptive ? S->e[j] : 0.0;
             }
         }
     } else {
@@ -857,14 +900,29 @@ void fill_epi(EpiArgs &P, qa_rk_solver *S, int i) {
         if (i < S->rk_extra - 1) {
             P.epi = EPI_NEXT;
             P.nterms = i + 1;
-            for (int j = 0; j <= i; ++j) P.coef[j] = S->a[(size_t)(i + 1) * ne + j];
+            for (int j = 0; j <= i; ++j) coef[j] = S->a[(size_t)(i + 1) * ne + j];
         } else {
             P.epi = EPI_NONE;
             P.nterms = 0;
         }
     }
-    P.coef_self = P.coef[i];
-    P.ecoef_self = P.ecoef[i];
+    P.coef_self = coef[i];
+    P.ecoef_self = ecoef[i];
+    P.nt = 0;
+    for (int j = 0; j < kMaxStages; ++j) {
+        P.tj[j] = 0;
+        P.tc[j] = P.te[j] = 0.0;
+    }
+    if (P.epi != EPI_NONE) {
+        for (int j = 0; j < i; ++j) {
+            if (coef[j] != 0.0 || ecoef[j] != 0.0) {
+                P.tj[P.nt] = j;
+                P.tc[P.nt] = coef[j];
+                P.te[P.nt] = ecoef[j];
+                ++P.nt;
+            }
+        }
+    }
 }
 
 template <typename V>
@@ -995,6 +1053,8 @@ int enqueue_iteration(qa_rk_solver *S) {
     QA_CHECK_LAUNCH();
     hipLaunchKernelGGL(rk_output_kernel, dim3(S->nchunks, std::max(S->nfun, 1)), dim3(256), 0,
                        S->stream, A);
+    QA_CHECK_LAUNCH();
+    hipLaunchKernelGGL(rk_output_finish_kernel, dim3(1), dim3(256), 0, S->stream, A);
     QA_CHECK_LAUNCH();
     return QA_OK;
 }
@@ -1582,56 +1642,123 @@ int qa_rk_stats(qa_rk_solver *S, int64_t *stats) {
 // qa_rk_set_state afterwards.  The control block is saved and restored; while
 // profiling, it points at a private far-future output time so the step
 // controller of the last stage only touches the control block.
-int qa_rk_profile_stages(qa_rk_solver *S, int reps, double *ms_per_stage) {
-    QA_REQUIRE(S != nullptr && reps > 0 && ms_per_stage != nullptr,
-               "qa_rk_profile_stages: bad arguments");
+// Profiling runs stage kernels outside a solve: the control block is saved,
+// given an output time that is never reached, and restored afterwards.
+static int profile_enter(qa_rk_solver *S, RKCtl **backup, double **far_time) {
     hipStream_t s = S->stream;
     QA_HIP(hipStreamSynchronize(s));
-    RKCtl *backup = nullptr;
-    double *far_time = nullptr;
-    QA_HIP(hipMalloc((void **)&backup, sizeof(RKCtl)));
-    QA_HIP(hipMalloc((void **)&far_time, sizeof(double)));
-    QA_HIP(hipMemcpyAsync(backup, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToDevice, s));
+    QA_HIP(hipMalloc((void **)backup, sizeof(RKCtl)));
+    QA_HIP(hipMalloc((void **)far_time, sizeof(double)));
+    QA_HIP(hipMemcpyAsync(*backup, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToDevice, s));
     QA_HIP(hipMemcpyAsync(S->ctl_host, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToHost, s));
     QA_HIP(hipStreamSynchronize(s));
     const double big = 1e300;
-    QA_HIP(hipMemcpy(far_time, &big, sizeof(double), hipMemcpyHostToDevice));
+    QA_HIP(hipMemcpy(*far_time, &big, sizeof(double), hipMemcpyHostToDevice));
     RKCtl prof = *S->ctl_host;
-    prof.times = far_time;
+    prof.times = *far_time;
     prof.ntimes = 1;
     prof.out_idx = 0;
     prof.nsteps_left = 1 << 30;
     prof.nsteps_call = 0;
-    prof.counter = 0;
     QA_HIP(hipMemcpy(S->ctl, &prof, sizeof(RKCtl), hipMemcpyHostToDevice));
-    hipEvent_t e0, e1;
-    QA_HIP(hipEventCreate(&e0));
-    QA_HIP(hipEventCreate(&e1));
+    return QA_OK;
+}
+
+static int profile_leave(qa_rk_solver *S, RKCtl *backup, double *far_time) {
+    hipStream_t s = S->stream;
+    QA_HIP(hipMemcpyAsync(S->ctl, backup, sizeof(RKCtl), hipMemcpyDeviceToDevice, s));
+    QA_HIP(hipStreamSynchronize(s));
+    (void)hipFree(backup);
+    (void)hipFree(far_time);
+    S->state_set = 0;
+    return QA_OK;
+}
+
+int qa_rk_profile_stages(qa_rk_solver *S, int reps, double *ms_per_stage) {
+    QA_REQUIRE(S != nullptr && reps > 0 && ms_per_stage != nullptr,
+               "qa_rk_profile_stages: bad arguments");
+    hipStream_t s = S->stream;
+    RKCtl *backup = nullptr;
+    double *far_time = nullptr;
+    {
+        const int rc0 = profile_enter(S, &backup, &far_time);
+        if (rc0 != QA_OK) return rc0;
+    }
     const int nk = S->interpolate ? S->rk_extra : S->rk_step;
     int rc = QA_OK;
+    // The stages are launched in solve order (0, 1, ..., nk - 1, 0, 1, ...),
+    // so each launch finds the caches as a solve leaves them, and every
+    // launch is bracketed by its own pair of events (recorded without host
+    // synchronisation; read at the end).
+    std::vector<hipEvent_t> ev((size_t)2 * nk * reps, nullptr);
+    for (auto &e : ev)
+        if (rc == QA_OK && hipEventCreate(&e) != hipSuccess) rc = QA_ERR_HIP;
+    for (int r = 0; r < reps && rc == QA_OK; ++r) {
+        for (int i = 0; i < nk && rc == QA_OK; ++i) {
+            if (i == 0 || i == S->rk_step)
+                hipLaunchKernelGGL(rk_set_phase_kernel, dim3(1), dim3(1), 0, s, S->ctl,
+                                   i < S->rk_step ? PH_STEP : PH_DENSE);
+            const size_t k = (size_t)2 * (r * nk + i);
+            if (hipEventRecord(ev[k], s) != hipSuccess) rc = QA_ERR_HIP;
+            if (rc == QA_OK) rc = launch_stage(S, i);
+            if (rc == QA_OK && hipEventRecord(ev[k + 1], s) != hipSuccess) rc = QA_ERR_HIP;
+        }
+    }
+    if (rc == QA_OK && hipStreamSynchronize(s) != hipSuccess) rc = QA_ERR_HIP;
     for (int i = 0; i < nk && rc == QA_OK; ++i) {
-        const int phase = i < S->rk_step ? PH_STEP : PH_DENSE;
         double tot = 0.0;
         for (int r = 0; r < reps && rc == QA_OK; ++r) {
-            hipLaunchKernelGGL(rk_set_phase_kernel, dim3(1), dim3(1), 0, s, S->ctl, phase);
-            QA_HIP(hipEventRecord(e0, s));
-            rc = launch_stage(S, i);
-            QA_HIP(hipEventRecord(e1, s));
-            QA_HIP(hipEventSynchronize(e1));
+            const size_t k = (size_t)2 * (r * nk + i);
             float ms = 0.f;
-            QA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (hipEventElapsedTime(&ms, ev[k], ev[k + 1]) != hipSuccess) rc = QA_ERR_HIP;
             tot += ms;
         }
         ms_per_stage[i] = tot / reps;
     }
-    QA_HIP(hipMemcpyAsync(S->ctl, backup, sizeof(RKCtl), hipMemcpyDeviceToDevice, s));
-    QA_HIP(hipStreamSynchronize(s));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(backup);
-    (void)hipFree(far_time);
-    S->state_set = 0;
-    return rc;
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc == QA_ERR_HIP) qa::set_error("qa_rk_profile_stages: event failure");
+    const int rc2 = profile_leave(S, backup, far_time);
+    return rc != QA_OK ? rc : rc2;
+}
+
+int qa_rk_profile_sequence(qa_rk_solver *S, int reps, double *ms_step, double *ms_dense) {
+    QA_REQUIRE(S != nullptr && reps > 0 && ms_step != nullptr && ms_dense != nullptr,
+               "qa_rk_profile_sequence: bad arguments");
+    hipStream_t s = S->stream;
+    RKCtl *backup = nullptr;
+    double *far_time = nullptr;
+    {
+        const int rc0 = profile_enter(S, &backup, &far_time);
+        if (rc0 != QA_OK) return rc0;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = QA_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = QA_ERR_HIP;
+    // part 0: the step stages, part 1: the dense-output stages
+    for (int part = 0; part < 2 && rc == QA_OK; ++part) {
+        const int i0 = part == 0 ? 0 : S->rk_step;
+        const int i1 = part == 0 ? S->rk_step : (S->interpolate ? S->rk_extra : S->rk_step);
+        double *out = part == 0 ? ms_step : ms_dense;
+        *out = 0.0;
+        if (i1 <= i0) continue;
+        hipLaunchKernelGGL(rk_set_phase_kernel, dim3(1), dim3(1), 0, s, S->ctl,
+                           part == 0 ? PH_STEP : PH_DENSE);
+        if (hipEventRecord(e0, s) != hipSuccess) rc = QA_ERR_HIP;
+        for (int r = 0; r < reps && rc == QA_OK; ++r)
+            for (int i = i0; i < i1 && rc == QA_OK; ++i) rc = launch_stage(S, i);
+        float ms = 0.f;
+        if (rc == QA_OK && (hipEventRecord(e1, s) != hipSuccess ||
+                            hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = QA_ERR_HIP;
+        *out = (double)ms / ((double)reps * (i1 - i0));
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc == QA_ERR_HIP) qa::set_error("qa_rk_profile_sequence: event failure");
+    const int rc2 = profile_leave(S, backup, far_time);
+    return rc != QA_OK ? rc : rc2;
 }
 
 // Launch one loop iteration directly (no graph) -- used to time kernels

@@ -136,11 +136,11 @@ def _csr_block(ops, d, fp32):
             _data.DeviceBuffer.from_array(vl))
 
 
-def _ell_block(A, d, fp32):
+def _ell_host(A, d, fp32):
     """Column-major ELL of a d x d operator for the trajectory kernel: width W
     = the longest row rounded up to a multiple of 4; entry u of row r at
-    [u * d + r]; shorter rows padded with value 0 and their own row index as
-    the column."""
+    [u, r]; shorter rows padded with value 0 and their own row index as the
+    column.  Returns (W, col [max(W,1), d] int32, val [max(W,1), d])."""
     A = sp.csr_matrix(A, dtype=np.complex128)
     A.sort_indices()
     if A.shape != (d, d):
@@ -153,7 +153,12 @@ def _ell_block(A, d, fp32):
     slot = np.arange(A.nnz) - np.repeat(A.indptr[:-1], lens)
     col[slot, rows] = A.indices
     val[slot, rows] = A.data
-    val = val.astype(np.complex64 if fp32 else np.complex128)
+    return W, col, val.astype(np.complex64 if fp32 else np.complex128)
+
+
+def _ell_block(A, d, fp32):
+    """_ell_host uploaded: (W, column buffer, value buffer)."""
+    W, col, val = _ell_host(A, d, fp32)
     return (W, _data.DeviceBuffer.from_array(np.ascontiguousarray(col)),
             _data.DeviceBuffer.from_array(np.ascontiguousarray(val)))
 

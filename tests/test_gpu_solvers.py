@@ -146,3 +146,33 @@ def test_integrator_protocol_qobjevo(fused):
     t_in, st = integ.mcstep(t_target)
     assert t_in == t_target
     assert_allclose(st.to_array().ravel(), analytical(t_in), atol=5e-5)
+
+
+def test_profiling_leaves_the_engine_reusable():
+    """The measurement entry points run stage kernels outside a solve; after
+    a new set_state the engine must integrate exactly as a fresh one."""
+    import ctypes as C
+
+    from qutip_amd import _lib
+    from qutip_amd.solver.hip_rk import RKEngine
+    m = models.jaynes_cummings()
+    Lm = models.liouvillian(m["H"], m["c_ops"])
+    rho0 = models.vec(models.ket2dm(m["psi0"]))
+    times = m["tlist"][1:20]
+
+    def run(profile):
+        eng = RKEngine(Lm, "vern7", 1e-8, 1e-10)
+        eng.set_state(0.0, rho0)
+        if profile:
+            ms = np.zeros(16)
+            _lib.call("qa_rk_profile_stages", eng.handle, 2,
+                      ms.ctypes.data_as(C.POINTER(C.c_double)))
+            t_step, t_dense = C.c_double(), C.c_double()
+            _lib.call("qa_rk_profile_sequence", eng.handle, 3, C.byref(t_step), C.byref(t_dense))
+            assert np.all(ms > 0) and t_step.value > 0 and t_dense.value > 0
+            eng.set_state(0.0, rho0)
+        eng.run(times, states_out=None)
+        _, y = eng.get_state()
+        return y.to_array()
+
+    assert np.array_equal(run(False), run(True))

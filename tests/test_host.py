@@ -115,3 +115,20 @@ def test_qutip_plugin_imports_without_qutip():
     if not qutip_plugin.available():
         with pytest.raises(ImportError):
             qutip_plugin.register()
+
+
+@pytest.mark.parametrize("d,density", [(1, 1.0), (7, 0.3), (64, 0.05), (130, 0.0), (33, 1.0)])
+def test_ell_layout_reproduces_csr(d, density):
+    """The trajectory kernel's column-major ELL: entry u of row r at u d + r,
+    width a multiple of 4, padding with value 0 and an in-range column."""
+    from qutip_amd.solver.mcsolve import _ell_host
+    rng = np.random.default_rng(d)
+    A = sp.random(d, d, density=density, random_state=rng, format="csr") * (1 - 0.25j)
+    W, col, val = _ell_host(A, d, False)
+    assert W % 4 == 0 and W >= np.diff(A.indptr).max()
+    assert col.shape == (max(W, 1), d) and col.min() >= 0 and col.max() < d
+    x = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+    y = (val[:W] * x[col[:W]]).sum(axis=0) if W else np.zeros(d, complex)
+    assert_allclose(y, A @ x, atol=1e-13)
+    _, _, val32 = _ell_host(A, d, True)
+    assert val32.dtype == np.complex64
