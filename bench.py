@@ -133,6 +133,45 @@ def bench_mcsolve(qa, systems, log, ntraj=2048, reps=3, cpu=True):
     return out
 
 
+def bench_jc(qa, systems, log, reps=20, cpu=True):
+    """The CPU-runnable plain mesolve: Jaynes-Cummings N=10 x qubit (d=20,
+    Liouvillian 400 x 400, 1543 non-zeros), t in [0,25] with 100 outputs,
+    vern7 rtol=1e-8 atol=1e-10; fused hip_rk (a launch-bound size) beside
+    the C restatement on one core."""
+    from qutip_amd import _lib
+    s = systems.jaynes_cummings()
+    g = np.load(ROOT / "tests" / "golden" / "jc_mesolve.npz")
+    solver = qa.MESolver(s["H"], s["c_ops"], options={"rtol": 1e-8, "atol": 1e-10})
+    res = solver.run(s["psi0"], g["tlist"], e_ops=s["e_ops"])
+    _lib.call("qa_device_sync")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = solver.run(s["psi0"], g["tlist"], e_ops=s["e_ops"])
+    _lib.call("qa_device_sync")
+    dt = (time.perf_counter() - t0) / reps
+    err = max(np.abs(np.asarray(res.expect[k]) - g["expect"][k]).max() for k in range(2))
+    out = {"workload": "mesolve Jaynes-Cummings N=10 x qubit (Liouvillian 400^2, 1543 nnz), "
+                       "t in [0,25], 100 outputs, vern7 rtol=1e-8 atol=1e-10",
+           "solves_per_s": round(1 / dt, 2), "err_vs_fp64": float(err),
+           "rk_attempts": solver._integrator.stats()["attempts"]}
+    if cpu:
+        from oracle import cbind, models, tableaus
+        m = models.jaynes_cummings()
+        Lm = models.liouvillian(m["H"], m["c_ops"])
+        fun = cbind.functionals_from_ops(m["e_ops"], 20)
+        rho0 = models.vec(models.ket2dm(m["psi0"]))
+        n = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < 2.0:
+            cbind.rk_solve(Lm, rho0, m["tlist"], tableaus.vern7, fun, rtol=1e-8, atol=1e-10)
+            n += 1
+        tc = (time.perf_counter() - t0) / n
+        out["cpu_baseline"] = {"value": round(1 / tc, 2), "unit": "solves/s", "cores": 1,
+                               "kind": "port", "sample": f"{n} full solves of the same workload"}
+    log(f"jc: {out}")
+    return out
+
+
 def bench_dense(qa, systems, log, reps=2, tf=None):
     """Matrix-form mesolve of the driven-dissipative Kerr oscillator, N=256
     (dense rho 256 x 256), t in [0, 30] with 150 outputs, e_ops a^dag a and
@@ -375,6 +414,7 @@ def main():
     out["fp32_operator"] = {"solves_per_s": round(1 / tf32, 4), "err_vs_fp64": float(ferr),
                             "rk_attempts": fsolver._integrator.stats()["attempts"]}
     log(f"fp32 operator: {1 / tf32:.2f} solves/s, err {ferr:.2e}")
+    out["jc_mesolve"] = bench_jc(qa, systems, log, cpu=not args.no_cpu)
     if not args.no_mc:
         out["mcsolve"] = bench_mcsolve(qa, systems, log, cpu=not args.no_cpu)
     if not args.no_dense:

@@ -19,6 +19,13 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 
 #define QA_CHECK_LAUNCH() QA_HIP(hipGetLastError())
 
+// propagate a non-zero status code of a library call
+#define QA_CHECK(expr)                                                        \
+    do {                                                                      \
+        const int _rc = (expr);                                               \
+        if (_rc != QA_OK) return _rc;                                         \
+    } while (0)
+
 #define QA_REQUIRE(cond, ...)                                                 \
     do {                                                                      \
         if (!(cond)) {                                                        \

@@ -352,20 +352,34 @@ struct DenseOp {
 __device__ __forceinline__ float2 tof(double2 a) { return make_float2((float)a.x, (float)a.y); }
 __device__ __forceinline__ float2 tof(float2 a) { return a; }
 
-// re + i im += A[i0:i0+16, :] B[:, j0:j0+16]  (FP32 MFMA, K = N)
+// re + i im += A[i0:i0+16, k0:k1] B[k0:k1, j0:j0+16]  (FP32 MFMA).
+// MFMA u of a 16-wide K block takes k = kk + 4 kq + u on lane (r, kq), so
+// each lane reads four consecutive rows of its B column (one contiguous
+// 32- or 64-byte piece) and the 16 lanes of a row group read 16 consecutive
+// A rows.  The operands of the next K block are loaded before the MFMAs of
+// the current one (register double buffering), so one memory latency is
+// exposed per product instead of one per block.  The prefetch index is
+// clamped to the last block, so the loads need no branch.
 template <typename TA, typename TB>
 __device__ __forceinline__ void zmma_f32(v4f &re, v4f &im, const TA *__restrict__ A,
                                          const TB *__restrict__ B, int N, int i0, int j0,
                                          int lane, int k0, int k1) {
     const int r = lane & 15, kq = lane >> 4;
-    const TA *pa = A + (size_t)kq * N + i0 + r;
-    const TB *pb = B + (size_t)(j0 + r) * N + kq;
+    const TA *pa = A + (size_t)(4 * kq) * N + i0 + r;
+    const TB *pb = B + (size_t)(j0 + r) * N + 4 * kq;
+    float2 a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        a[u] = tof(pa[(size_t)(k0 + u) * N]);
+        b[u] = tof(pb[k0 + u]);
+    }
     for (int kk = k0; kk < k1; kk += 16) {
-        float2 a[4], b[4];
+        const int kn = min(kk + 16, k1 - 16);
+        float2 an[4], bn[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            a[u] = tof(pa[(size_t)(kk + 4 * u) * N]);
-            b[u] = tof(pb[kk + 4 * u]);
+            an[u] = tof(pa[(size_t)(kn + u) * N]);
+            bn[u] = tof(pb[kn + u]);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -373,6 +387,11 @@ __device__ __forceinline__ void zmma_f32(v4f &re, v4f &im, const TA *__restrict_
             re = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[u].y, b[u].y, re, 0, 0, 0);
             im = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].y, im, 0, 0, 0);
             im = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[u].x, im, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] = an[u];
+            b[u] = bn[u];
         }
     }
 }
@@ -382,32 +401,46 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
     lo = (__bf16)(x - (float)hi);
 }
 
-// Same product with bf16 x3 splitting (K = N, N a multiple of 32).
+// Same product with bf16 x3 splitting (32-wide K blocks; lane (r, kq)
+// holds k = kk + 8 kq + u, u < 8, so its B piece is contiguous).  The next
+// block's operands are prefetched as in zmma_f32.
 template <typename TA, typename TB>
 __device__ __forceinline__ void zmma_bf16x3(v4f &re, v4f &im, const TA *__restrict__ A,
                                             const TB *__restrict__ B, int N, int i0, int j0,
                                             int lane, int k0, int k1) {
     const int r = lane & 15, kq = lane >> 4;
+    const TA *pa = A + (size_t)(8 * kq) * N + i0 + r;
+    const TB *pb = B + (size_t)(j0 + r) * N + 8 * kq;
+    float2 a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        a[u] = tof(pa[(size_t)(k0 + u) * N]);
+        b[u] = tof(pb[k0 + u]);
+    }
     for (int kk = k0; kk < k1; kk += 32) {
+        const int kn = min(kk + 32, k1 - 32);
+        float2 an[8], bn[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            an[u] = tof(pa[(size_t)(kn + u) * N]);
+            bn[u] = tof(pb[kn + u]);
+        }
         v8bf arh, arl, aih, ail, naih, nail, brh, brl, bih, bil;
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int k = kk + 8 * kq + u;
-            const float2 a = tof(A[(size_t)k * N + i0 + r]);
-            const float2 b = tof(B[(size_t)(j0 + r) * N + k]);
             __bf16 h, l;
-            split_bf16(a.x, h, l);
+            split_bf16(a[u].x, h, l);
             arh[u] = h;
             arl[u] = l;
-            split_bf16(a.y, h, l);
+            split_bf16(a[u].y, h, l);
             aih[u] = h;
             ail[u] = l;
             naih[u] = -h;
             nail[u] = -l;
-            split_bf16(b.x, h, l);
+            split_bf16(b[u].x, h, l);
             brh[u] = h;
             brl[u] = l;
-            split_bf16(b.y, h, l);
+            split_bf16(b[u].y, h, l);
             bih[u] = h;
             bil[u] = l;
         }
@@ -424,6 +457,11 @@ __device__ __forceinline__ void zmma_bf16x3(v4f &re, v4f &im, const TA *__restri
         im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brh, im, 0, 0, 0);
         im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brl, im, 0, 0, 0);
         im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, brh, im, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a[u] = an[u];
+            b[u] = bn[u];
+        }
     }
 }
 
@@ -452,7 +490,7 @@ __device__ __forceinline__ int64_t tile_index(int N, int i0, int j0, int lane, i
     return (int64_t)(j0 + (lane & 15)) * N + i0 + 4 * (lane >> 4) + v;
 }
 
-constexpr int kMaxSplit = 4;
+constexpr int kMaxSplit = 8;
 
 // Sum the per-wave partial tiles into wave 0 (fixed order), via LDS.
 __device__ __forceinline__ void reduce_split(v4f &re, v4f &im, int lane, int w, int nsplit,
@@ -512,13 +550,13 @@ __device__ __forceinline__ void dense_k_tile(const DenseOp &D, const double2 *X,
     reduce_split(re, im, lane, w, nsplit, lds);
 }
 
-__global__ __launch_bounds__(256) void rk_dense_t_kernel(EpiArgs P, DenseOp D) {
+__global__ __launch_bounds__(64 * kMaxSplit) void rk_dense_t_kernel(EpiArgs P, DenseOp D) {
     if (P.ctl->phase != P.run_phase) return;
     const StageVecs v = stage_vecs(P, P.ctl);
     dense_t_tile(D, v.xin);
 }
 
-__global__ __launch_bounds__(256) void rk_dense_stage_kernel(EpiArgs P, DenseOp D) {
+__global__ __launch_bounds__(64 * kMaxSplit) void rk_dense_stage_kernel(EpiArgs P, DenseOp D) {
     RKCtl *ctl = P.ctl;
     if (ctl->phase != P.run_phase) return;
     const StageVecs v = stage_vecs(P, ctl);
@@ -544,11 +582,11 @@ __global__ __launch_bounds__(256) void rk_dense_stage_kernel(EpiArgs P, DenseOp 
 }
 
 // Ungated versions: out = L(X) (the data-layer product of the matrix form).
-__global__ __launch_bounds__(256) void dense_t_kernel(DenseOp D, const double2 *X) {
+__global__ __launch_bounds__(64 * kMaxSplit) void dense_t_kernel(DenseOp D, const double2 *X) {
     dense_t_tile(D, X);
 }
 
-__global__ __launch_bounds__(256) void dense_rhs_kernel(DenseOp D, const double2 *X, double2 *out) {
+__global__ __launch_bounds__(64 * kMaxSplit) void dense_rhs_kernel(DenseOp D, const double2 *X, double2 *out) {
     v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
     dense_k_tile(D, X, re, im);
     if ((threadIdx.x >> 6) != 0) return;
@@ -816,6 +854,9 @@ __global__ void rk_set_phase_kernel(RKCtl *c, int phase) { c->phase = phase; }
 // ---------------------------------------------------------------------------
 struct qa_rk_solver {
     int kind = 0;  // 0: sparse SELL-C generator, 1: dense Lindblad (matrix form)
+    // buffers replaced while the engine lives; freed with it (hipFree waits
+    // for the whole device, which would stall other engines' streams)
+    std::vector<void *> retired;
     int n = 0, nslices = 0, vals_fp32 = 0, lanes = 1;
     int nparts = 0;      // per-workgroup error partials of the last stage
     DenseOp dop = {};    // kind 1: operators in HBM (complex64)
@@ -1083,6 +1124,16 @@ int rebuild_graph(qa_rk_solver *S) {
     return QA_OK;
 }
 
+// Retired buffers are freed in batches of kRetiredBatch (one device-wide
+// wait per batch instead of one per buffer).
+constexpr size_t kRetiredBatch = 64;
+int release_retired(qa_rk_solver *S) {
+    if (S->retired.size() < kRetiredBatch) return QA_OK;
+    for (void *p : S->retired) QA_HIP(hipFree(p));
+    S->retired.clear();
+    return QA_OK;
+}
+
 void free_all(qa_rk_solver *S) {
     if (S->exec) (void)hipGraphExecDestroy(S->exec);
     if (S->graph) (void)hipGraphDestroy(S->graph);
@@ -1091,15 +1142,13 @@ void free_all(qa_rk_solver *S) {
                     S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (void *p : S->retired) (void)hipFree(p);
+    S->retired.clear();
     if (S->ctl_host) (void)hipHostFree(S->ctl_host);
     if (S->stream) (void)hipStreamDestroy(S->stream);
 }
 
 }  // namespace
-
-extern "C" {
-
-}  // extern "C"
 
 namespace {
 
@@ -1298,7 +1347,7 @@ int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, co
     S->dop.N = N;
     S->dop.nc = nc;
     S->dop.prec = precision;
-    // split K over up to 4 waves while every wave keeps whole K steps
+    // split K over up to kMaxSplit waves while every wave keeps whole K steps
     // (16 for FP32, 32 for bf16 x3)
     {
         const int kstep = precision == 1 ? 32 : 16;
@@ -1359,8 +1408,10 @@ int qa_rk_set_functionals(qa_rk_solver *S, int nfun, const int *kinds, const int
     }
     QA_HIP(hipStreamSynchronize(S->stream));
     void *olds[] = {S->fptr_dev, S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev};
+    QA_HIP(hipStreamSynchronize(S->stream));  // no launch of this engine still reads them
     for (void *p : olds)
-        if (p) QA_HIP(hipFree(p));
+        if (p) S->retired.push_back(p);
+    QA_CHECK(release_retired(S));
     S->fptr_dev = S->fkind_dev = S->frow_dev = S->fcol_dev = nullptr;
     S->fw_dev = S->fpart_dev = nullptr;
     S->nfun = nfun;
@@ -1461,14 +1512,16 @@ int qa_rk_begin(qa_rk_solver *S, int ntimes, const double *times, int single_ste
     // the previous run may still be draining its last chunk
     QA_HIP(hipStreamSynchronize(s));
     if (ntimes > S->times_cap) {
-        if (S->times_dev) QA_HIP(hipFree(S->times_dev));
+        if (S->times_dev) S->retired.push_back(S->times_dev);
+        QA_CHECK(release_retired(S));
         S->times_dev = nullptr;
         S->times_cap = std::max(ntimes, 2 * S->times_cap);
         QA_HIP(hipMalloc((void **)&S->times_dev, sizeof(double) * S->times_cap));
     }
     const int64_t need = (int64_t)ntimes * std::max(S->nfun, 1);
     if (need > S->fun_out_cap) {
-        if (S->fun_out_dev) QA_HIP(hipFree(S->fun_out_dev));
+        if (S->fun_out_dev) S->retired.push_back(S->fun_out_dev);
+        QA_CHECK(release_retired(S));
         S->fun_out_dev = nullptr;
         S->fun_out_cap = std::max(need, 2 * S->fun_out_cap);
         QA_HIP(hipMalloc((void **)&S->fun_out_dev, sizeof(double2) * S->fun_out_cap));

@@ -25,13 +25,76 @@ import scipy.sparse as sp
 
 from . import _lib
 
-__all__ = ["HipDense", "HipCSR", "DeviceBuffer", "matmul", "add", "iadd", "mul", "imul",
+__all__ = ["HipDense", "HipCSR", "DeviceBuffer", "deferred_release", "release_deferred",
+           "matmul", "add", "iadd", "mul", "imul",
            "expect", "inner", "norm_l2", "frobenius", "wrmn_error", "zeros", "trace",
            "column_stack", "column_unstack", "to_hip", "to_numpy"]
 
 
+# Deferred release.  hipFree waits for the whole device, so a host loop that
+# frees while other work is in flight (the solve server) serialises on every
+# release.  Inside ``deferred_release()`` the buffers and engines that die
+# are queued instead, and ``release_deferred()`` frees the queue after one
+# device synchronisation.
+_deferred = None
+_deferred_bytes = 0
+
+
+def _defer(kind, obj, nbytes=0):
+    """Queue a release when deferral is on; True when queued."""
+    global _deferred_bytes
+    if _deferred is None:
+        return False
+    _deferred.append((kind, obj))
+    _deferred_bytes += nbytes
+    return True
+
+
+class deferred_release:
+    """Context manager: releases inside it are queued, then done at exit
+    (or earlier, by ``release_deferred()``)."""
+
+    def __enter__(self):
+        global _deferred
+        if _deferred is None:
+            _deferred = []
+            self._owner = True
+        else:
+            self._owner = False
+        return self
+
+    def __exit__(self, *exc):
+        global _deferred
+        if self._owner:
+            release_deferred()
+            _deferred = None
+        return False
+
+
+def pending_releases():
+    """(queued releases, queued buffer bytes)."""
+    return (0 if _deferred is None else len(_deferred)), _deferred_bytes
+
+
+def release_deferred():
+    """Synchronise the device once and free everything queued."""
+    global _deferred_bytes
+    if not _deferred or _lib._lib is None:
+        return
+    L = _lib._lib
+    L.qa_device_sync()
+    _deferred_bytes = 0
+    while _deferred:
+        kind, obj = _deferred.pop()
+        if kind == "buffer":
+            L.qa_free(C.c_void_p(obj))
+        else:
+            L.qa_rk_destroy(obj)
+
+
 class DeviceBuffer:
-    """A block of HBM owned by this object (freed on garbage collection)."""
+    """A block of HBM owned by this object (freed on garbage collection, or
+    queued for release inside ``deferred_release()``)."""
     __slots__ = ("ptr", "nbytes", "__weakref__")
 
     def __init__(self, nbytes):
@@ -44,7 +107,8 @@ class DeviceBuffer:
     def __del__(self):
         p = getattr(self, "ptr", None)
         if p and _lib._lib is not None:
-            _lib._lib.qa_free(C.c_void_p(p))
+            if not _defer("buffer", p, self.nbytes):
+                _lib._lib.qa_free(C.c_void_p(p))
             self.ptr = None
 
     def upload(self, arr):

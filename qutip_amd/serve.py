@@ -239,28 +239,40 @@ class SolveServer:
         self.pending = keep
         return admitted
 
+    # queued releases are done (one device synchronisation) past these
+    release_count = 4096
+    release_bytes = 8 << 30
+
     def run(self, requests=None, realtime=True, clock=None):
         """Serve ``requests`` (arrival times in seconds from the start) until
-        all have finished; returns them with result / t_done filled in."""
+        all have finished; returns them with result / t_done filled in.
+
+        Device memory of retired requests is released in batches: hipFree
+        waits for the whole device, so freeing each request's buffers as it
+        retires would stall every request in flight."""
         for r in requests or []:
             self.submit(r)
         self.pending.sort(key=lambda r: r.arrival)
         t0 = time.perf_counter()
         now = lambda: (time.perf_counter() - t0) if realtime else float("inf")
-        while self.pending or self.running:
-            self._admit(now())
-            self.peak_in_flight = max(self.peak_in_flight, len(self.running))
-            still = []
-            for run in self.running:
-                if run.poll():
-                    run.req.t_done = time.perf_counter() - t0
-                    self.in_use -= run.req._bytes
-                    self.finished.append(run.req)
-                else:
-                    still.append(run)
-            self.running = still
-            if not self.running and self.pending and realtime:
-                wait = self.pending[0].arrival - now()
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
+        with _data.deferred_release():
+            while self.pending or self.running:
+                self._admit(now())
+                self.peak_in_flight = max(self.peak_in_flight, len(self.running))
+                still = []
+                for run in self.running:
+                    if run.poll():
+                        run.req.t_done = time.perf_counter() - t0
+                        self.in_use -= run.req._bytes
+                        self.finished.append(run.req)
+                    else:
+                        still.append(run)
+                self.running = still
+                count, nbytes = _data.pending_releases()
+                if count > self.release_count or nbytes > self.release_bytes:
+                    _data.release_deferred()
+                if not self.running and self.pending and realtime:
+                    wait = self.pending[0].arrival - now()
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
         return self.finished

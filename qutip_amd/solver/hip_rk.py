@@ -118,7 +118,8 @@ class RKEngine:
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value and _lib._lib is not None:
-            _lib._lib.qa_rk_destroy(h)
+            if not _data._defer("engine", h):
+                _lib._lib.qa_rk_destroy(h)
             self.handle = None
 
     def set_functionals(self, kinds, fptr, rows, cols, w):

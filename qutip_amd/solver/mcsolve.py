@@ -229,7 +229,13 @@ class MCSolver:
         self._n_ops = []
         for c in c_ops:
             cc = c.to_csr()
-            n = (cc.conj().T @ cc).tocsr()
+            if cc.nnz > 0.05 * self.d * self.d:
+                # a dense collapse operator: the product by BLAS (a sparse
+                # product of dense factors costs orders of magnitude more)
+                cd = cc.toarray()
+                n = sp.csr_matrix(cd.conj().T @ cd)
+            else:
+                n = (cc.conj().T @ cc).tocsr()
             self._n_ops.append(n)
             Heff = Heff - 0.5 * n
         Heff = sp.csr_matrix(Heff)
