@@ -20,9 +20,11 @@
 // streamed in; a trajectory that uses up its draws stops with a status the
 // host acts on.
 //
-// Layout: each thread owns one or two rows of the state (d <= 1024); the RK
-// stage vectors k_0..k_9 stay in registers and k_10..k_15 in LDS for the
-// whole trajectory; the vector an RHS product reads is staged in LDS.  H_eff is
+// Layout: each thread owns one or two rows of the state (d <= 1024).  The
+// RK stage vectors stay on the chip for the whole trajectory: all sixteen in
+// LDS when they take at most half of it (two workgroups per CU), else
+// k_0..k_9 in registers and k_10..k_15 in LDS.  The vector an RHS product
+// reads is staged in LDS.  H_eff is
 // a column-major ELL in HBM (entry u of row r at u d + r), so the 64 lanes of
 // a wave, which own 64 consecutive rows, load 64 consecutive columns and
 // values per instruction; c_ops and e_ops (used at jumps and outputs only)
@@ -125,7 +127,12 @@ __device__ __forceinline__ double nrm2(C2<R> v) {
     return (double)v.x * (double)v.x + (double)v.y * (double)v.y;
 }
 
-constexpr int kRegStages = 10;  // k_0..k_9 in registers, the rest in LDS
+// RS (a template parameter): stages k_0..k_{RS-1} live in registers, the
+// rest in LDS.  RS = 10 keeps the step stages of vern7 in registers (512
+// threads, up to 256 VGPRs each); RS = 0 keeps every stage in LDS, which
+// frees the registers for 1024-thread workgroups (more waves in flight) when
+// the LDS holds all 18 vectors.
+constexpr int kRegStagesMax = 10;
 
 // Per-workgroup LDS (dynamic): the vector an RHS product reads, the stage
 // vectors k_10.. (the dense-output stages of vern7) and the FSAL slot, one
@@ -133,7 +140,7 @@ constexpr int kRegStages = 10;  // k_0..k_9 in registers, the rest in LDS
 template <typename R>
 struct McLds {
     C2<R> *x;      // [npad]
-    C2<R> *kd;     // [kMcStages - kRegStages][npad]
+    C2<R> *kd;     // [kMcStages - RS][npad]
     C2<R> *kfsal;  // [npad]
     double *red;   // [threads / 64]
     double *probs; // [kMcMaxChan]
@@ -141,8 +148,8 @@ struct McLds {
 };
 
 template <typename R>
-inline size_t mc_lds_bytes(int npad, int threads) {
-    return sizeof(C2<R>) * (size_t)npad * (2 + kMcStages - kRegStages) +
+inline size_t mc_lds_bytes(int npad, int threads, int RS) {
+    return sizeof(C2<R>) * (size_t)npad * (2 + kMcStages - RS) +
            sizeof(double) * ((threads / 64) + kMcMaxChan) + 64;
 }
 
@@ -162,9 +169,9 @@ __device__ __forceinline__ V<R, NR> vzero() {
 
 // Per-trajectory integrator state.  Vectors are per thread; scalars are
 // uniform (every thread holds the same value).
-template <typename R, int NR>
+template <typename R, int NR, int RS>
 struct Traj {
-    V<R, NR> k[kRegStages];
+    V<R, NR> k[RS > 0 ? RS : 1];
     V<R, NR> yprev, yfront, y;
     double t, t_prev, t_front, dt_safe, dt_int;
     int status;    // trajectory status (McStatus)
@@ -172,7 +179,7 @@ struct Traj {
     long long attempts;
 };
 
-template <typename R, int NR>
+template <typename R, int NR, int RS>
 struct Ctx {
     const McArgs &A;
     McLds<R> &S;
@@ -184,27 +191,27 @@ struct Ctx {
 
 // Stage vector j; j is a compile-time constant after unrolling, so the
 // register / LDS choice costs nothing.
-template <typename R, int NR>
-__device__ __forceinline__ V<R, NR> kget(Ctx<R, NR> &c, Traj<R, NR> &s, int j) {
-    if (j < kRegStages) return s.k[j < kRegStages ? j : 0];
+template <typename R, int NR, int RS>
+__device__ __forceinline__ V<R, NR> kget(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, int j) {
+    if (j < RS) return s.k[j < RS ? j : 0];
     V<R, NR> o;
 #pragma unroll
     for (int q = 0; q < NR; ++q)
-        o.v[q] = c.S.kd[(size_t)(j - kRegStages) * c.S.npad + c.row[q]];
+        o.v[q] = c.S.kd[(size_t)(j - RS) * c.S.npad + c.row[q]];
     return o;
 }
-template <typename R, int NR>
-__device__ __forceinline__ void kset(Ctx<R, NR> &c, Traj<R, NR> &s, int j, const V<R, NR> &x) {
-    if (j < kRegStages) {
-        s.k[j < kRegStages ? j : 0] = x;
+template <typename R, int NR, int RS>
+__device__ __forceinline__ void kset(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, int j, const V<R, NR> &x) {
+    if (j < RS) {
+        s.k[j < RS ? j : 0] = x;
     } else {
 #pragma unroll
-        for (int q = 0; q < NR; ++q) c.S.kd[(size_t)(j - kRegStages) * c.S.npad + c.row[q]] = x.v[q];
+        for (int q = 0; q < NR; ++q) c.S.kd[(size_t)(j - RS) * c.S.npad + c.row[q]] = x.v[q];
     }
 }
 
-template <typename R, int NR>
-__device__ __forceinline__ double bsum(Ctx<R, NR> &c, double v) {
+template <typename R, int NR, int RS>
+__device__ __forceinline__ double bsum(Ctx<R, NR, RS> &c, double v) {
     const int lane = threadIdx.x & (QA_WAVE - 1), wid = threadIdx.x / QA_WAVE;
     const int nw = blockDim.x / QA_WAVE;
     v = wave_sum(v);
@@ -216,8 +223,8 @@ __device__ __forceinline__ double bsum(Ctx<R, NR> &c, double v) {
     return t;
 }
 
-template <typename R, int NR>
-__device__ __forceinline__ void stage_in(Ctx<R, NR> &c, const V<R, NR> &x) {
+template <typename R, int NR, int RS>
+__device__ __forceinline__ void stage_in(Ctx<R, NR, RS> &c, const V<R, NR> &x) {
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NR; ++q) c.S.x[c.row[q]] = x.v[q];
@@ -226,8 +233,8 @@ __device__ __forceinline__ void stage_in(Ctx<R, NR> &c, const V<R, NR> &x) {
 
 // M ELL entries (u0 .. u0 + M - 1) of this thread's rows: all column and
 // value loads first, then the LDS gathers and multiply-adds.
-template <int M, typename R, int NR>
-__device__ __forceinline__ void ell_chunk(Ctx<R, NR> &c, int u0, V<R, NR> &acc) {
+template <int M, typename R, int NR, int RS>
+__device__ __forceinline__ void ell_chunk(Ctx<R, NR, RS> &c, int u0, V<R, NR> &acc) {
     const C2<R> *hv = (const C2<R> *)c.A.h_vals;
     const size_t d = (size_t)c.A.d;
     int cc[NR][M];
@@ -250,8 +257,8 @@ __device__ __forceinline__ void ell_chunk(Ctx<R, NR> &c, int u0, V<R, NR> &acc) 
 // width is a multiple of 4 (the host pads it), so entries go in chunks of 8
 // (4 for two complex128 rows per thread, which leaves room for the stage
 // vectors in registers) and at most one chunk of 4; no load is predicated.
-template <typename R, int NR>
-__device__ __forceinline__ V<R, NR> rhs(Ctx<R, NR> &c, const V<R, NR> &x) {
+template <typename R, int NR, int RS>
+__device__ __forceinline__ V<R, NR> rhs(Ctx<R, NR, RS> &c, const V<R, NR> &x) {
     constexpr int K = (NR == 2 && sizeof(R) == 8) ? 4 : 8;
     stage_in(c, x);
     V<R, NR> acc = vzero<R, NR>();
@@ -266,8 +273,8 @@ __device__ __forceinline__ V<R, NR> rhs(Ctx<R, NR> &c, const V<R, NR> &x) {
 }
 
 // (O x)_r for a CSR operator (c_ops / e_ops block); x already in LDS.
-template <typename R, int NR>
-__device__ __forceinline__ V<R, NR> op_rows(Ctx<R, NR> &c, const int *rowptr, const int *col,
+template <typename R, int NR, int RS>
+__device__ __forceinline__ V<R, NR> op_rows(Ctx<R, NR, RS> &c, const int *rowptr, const int *col,
                                             const void *vals) {
     V<R, NR> acc = vzero<R, NR>();
     const C2<R> *v = (const C2<R> *)vals;
@@ -279,8 +286,8 @@ __device__ __forceinline__ V<R, NR> op_rows(Ctx<R, NR> &c, const int *rowptr, co
     return acc;
 }
 
-template <typename R, int NR>
-__device__ __forceinline__ double norm2(Ctx<R, NR> &c, const V<R, NR> &x) {
+template <typename R, int NR, int RS>
+__device__ __forceinline__ double norm2(Ctx<R, NR, RS> &c, const V<R, NR> &x) {
     double s = 0.0;
 #pragma unroll
     for (int q = 0; q < NR; ++q) s += nrm2<R>(x.v[q]);  // inactive rows hold zeros
@@ -288,8 +295,8 @@ __device__ __forceinline__ double norm2(Ctx<R, NR> &c, const V<R, NR> &x) {
 }
 
 // base + dt * sum_{j < n} coef[j] k_j (zero terms skipped like iadd_data)
-template <typename R, int NR>
-__device__ __forceinline__ V<R, NR> combine(Ctx<R, NR> &c, Traj<R, NR> &s, const V<R, NR> &base,
+template <typename R, int NR, int RS>
+__device__ __forceinline__ V<R, NR> combine(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, const V<R, NR> &base,
                                             const double *coef, int n, double dt) {
     V<R, NR> o = base;
 #pragma unroll
@@ -307,15 +314,17 @@ __device__ __forceinline__ V<R, NR> combine(Ctx<R, NR> &c, Traj<R, NR> &s, const
 }
 
 // _compute_step (explicit_rk.pyx:358-389) + _error (:391-397)
-template <typename R, int NR>
-__device__ __forceinline__ double compute_step(Ctx<R, NR> &c, Traj<R, NR> &s, double dt) {
+template <typename R, int NR, int RS>
+__device__ __forceinline__ double compute_step(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, double dt) {
     const McTableau &T = c.T;
     s.attempts++;
     if (T.fsal) {
+        V<R, NR> k0;
 #pragma unroll
-        for (int q = 0; q < NR; ++q) s.k[0].v[q] = c.S.kfsal[c.row[q]];
+        for (int q = 0; q < NR; ++q) k0.v[q] = c.S.kfsal[c.row[q]];
+        kset(c, s, 0, k0);
     } else {
-        s.k[0] = rhs(c, s.yprev);
+        kset(c, s, 0, rhs(c, s.yprev));
     }
 #pragma unroll
     for (int i = 1; i < kMcStages; ++i) {
@@ -340,8 +349,8 @@ __device__ __forceinline__ double compute_step(Ctx<R, NR> &c, Traj<R, NR> &s, do
     return sqrt(bsum(c, qsum) / (double)c.A.d);
 }
 
-template <typename R, int NR>
-__device__ __forceinline__ void prep_dense_out(Ctx<R, NR> &c, Traj<R, NR> &s) {
+template <typename R, int NR, int RS>
+__device__ __forceinline__ void prep_dense_out(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s) {
     const McTableau &T = c.T;
 #pragma unroll
     for (int i = 0; i < kMcStages; ++i) {
@@ -352,8 +361,8 @@ __device__ __forceinline__ void prep_dense_out(Ctx<R, NR> &c, Traj<R, NR> &s) {
     }
 }
 
-template <typename R, int NR>
-__device__ __forceinline__ V<R, NR> interpolate(Ctx<R, NR> &c, Traj<R, NR> &s, double t) {
+template <typename R, int NR, int RS>
+__device__ __forceinline__ V<R, NR> interpolate(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, double t) {
     const McTableau &T = c.T;
     const double tau = (t - s.t_prev) / s.dt_int;
     double bf[kMcStages];
@@ -371,21 +380,24 @@ __device__ __forceinline__ V<R, NR> interpolate(Ctx<R, NR> &c, Traj<R, NR> &s, d
     return combine(c, s, s.yprev, bf, T.rk_extra, s.dt_int);
 }
 
-template <typename R, int NR>
-__device__ __forceinline__ double estimate_first_step(Ctx<R, NR> &c, Traj<R, NR> &s,
+template <typename R, int NR, int RS>
+__device__ __forceinline__ double estimate_first_step(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s,
                                                       const V<R, NR> &y0) {
     const McTableau &T = c.T;
     if (!T.adaptive) return 0.0;
     double norm = sqrt(norm2(c, y0));
     const double tol = c.A.atol + norm * c.A.rtol;
+    // f(y0) and f(y0 + dt1/100 f(y0)) are local temporaries (the reference
+    // borrows k[0] and k[1] for them; the next step overwrites both anyway)
+    V<R, NR> f0;
     if (T.fsal) {
 #pragma unroll
-        for (int q = 0; q < NR; ++q) s.k[0].v[q] = c.S.kfsal[c.row[q]];
+        for (int q = 0; q < NR; ++q) f0.v[q] = c.S.kfsal[c.row[q]];
     } else {
-        s.k[0] = rhs(c, y0);
+        f0 = rhs(c, y0);
     }
     if (norm <= c.A.atol) norm = 1.0;
-    double tmp = sqrt(norm2(c, s.k[0]));
+    double tmp = sqrt(norm2(c, f0));
     double factorial = 1.0;
     for (int i = 1; i <= T.order; ++i) factorial *= i;
     double dt1, dt2;
@@ -395,9 +407,9 @@ __device__ __forceinline__ double estimate_first_step(Ctx<R, NR> &c, Traj<R, NR>
         dt1 = pow(tol * factorial, 1.0 / (T.order + 1)) * norm * 0.5;
     V<R, NR> yt = y0;
 #pragma unroll
-    for (int q = 0; q < NR; ++q) raxpyR<R>(yt.v[q], (R)(dt1 / 100), s.k[0].v[q]);
-    s.k[1] = rhs(c, yt);
-    tmp = sqrt(norm2(c, s.k[1]));
+    for (int q = 0; q < NR; ++q) raxpyR<R>(yt.v[q], (R)(dt1 / 100), f0.v[q]);
+    const V<R, NR> f1 = rhs(c, yt);
+    tmp = sqrt(norm2(c, f1));
     if (tmp >= c.A.atol * 1e-6)
         dt2 = pow(tol * factorial * pow(norm, T.order), 1.0 / (T.order + 1)) / tmp;
     else
@@ -409,8 +421,8 @@ __device__ __forceinline__ double estimate_first_step(Ctx<R, NR> &c, Traj<R, NR>
 }
 
 // set_initial_value (explicit_rk.pyx:204-230)
-template <typename R, int NR>
-__device__ __forceinline__ void set_initial_value(Ctx<R, NR> &c, Traj<R, NR> &s,
+template <typename R, int NR, int RS>
+__device__ __forceinline__ void set_initial_value(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s,
                                                   const V<R, NR> &y0, double t0) {
     s.t = s.t_prev = s.t_front = t0;
     s.dt_int = 0.0;
@@ -426,8 +438,8 @@ __device__ __forceinline__ void set_initial_value(Ctx<R, NR> &c, Traj<R, NR> &s,
     s.dt_safe = c.A.first_step != 0.0 ? c.A.first_step : estimate_first_step(c, s, y0);
 }
 
-template <typename R, int NR>
-__device__ __forceinline__ void recompute_safe_step(Ctx<R, NR> &c, Traj<R, NR> &s, double err,
+template <typename R, int NR, int RS>
+__device__ __forceinline__ void recompute_safe_step(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, double err,
                                                     double dt) {
     if (!c.T.adaptive) return;
     double f;
@@ -443,8 +455,8 @@ __device__ __forceinline__ void recompute_safe_step(Ctx<R, NR> &c, Traj<R, NR> &
     if (c.A.min_step != 0.0) s.dt_safe = fmax(c.A.min_step, s.dt_safe);
 }
 
-template <typename R, int NR>
-__device__ __forceinline__ double get_timestep(Ctx<R, NR> &c, Traj<R, NR> &s, double t) {
+template <typename R, int NR, int RS>
+__device__ __forceinline__ double get_timestep(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, double t) {
     const double dt_needed = t - s.t_prev;
     if (!c.T.adaptive) return dt_needed;
     if (c.T.interpolate) return s.dt_safe;
@@ -453,8 +465,8 @@ __device__ __forceinline__ double get_timestep(Ctx<R, NR> &c, Traj<R, NR> &s, do
 }
 
 // _step_in_err (explicit_rk.pyx:333-356); returns attempts, sets status
-template <typename R, int NR>
-__device__ __forceinline__ int step_in_err(Ctx<R, NR> &c, Traj<R, NR> &s, double t, int max_step) {
+template <typename R, int NR, int RS>
+__device__ __forceinline__ int step_in_err(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, double t, int max_step) {
     double error = 1.0;
     int n = 0;
     while (error >= 1.0) {
@@ -486,8 +498,8 @@ __device__ __forceinline__ int step_in_err(Ctx<R, NR> &c, Traj<R, NR> &s, double
 }
 
 // integrate(t, step=True): explicit_rk.pyx:278-331 with step=True.
-template <typename R, int NR>
-__device__ __forceinline__ void mcstep(Ctx<R, NR> &c, Traj<R, NR> &s, double t) {
+template <typename R, int NR, int RS>
+__device__ __forceinline__ void mcstep(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, double t) {
     int nsteps_left = c.A.nsteps;
     if (t == s.t) return;
     if (t < s.t_prev) {
@@ -534,8 +546,8 @@ __device__ __forceinline__ bool draw(Rng &g, double &u) {
     return true;
 }
 
-template <typename R, int NR>
-__device__ __forceinline__ void write_expect(Ctx<R, NR> &c, const V<R, NR> &y, int traj, int k) {
+template <typename R, int NR, int RS>
+__device__ __forceinline__ void write_expect(Ctx<R, NR, RS> &c, const V<R, NR> &y, int traj, int k) {
     const double n2 = norm2(c, y);
     stage_in(c, y);
     for (int e = 0; e < c.A.neops; ++e) {
@@ -555,8 +567,8 @@ __device__ __forceinline__ void write_expect(Ctx<R, NR> &c, const V<R, NR> &y, i
     }
 }
 
-template <typename R, int NR>
-__global__ __launch_bounds__(kMcThreads) void mc_traj_kernel(McArgs A) {
+template <typename R, int NR, int RS>
+__global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj_kernel(McArgs A) {
     extern __shared__ __align__(16) unsigned char mc_dyn_lds[];
     __shared__ McTableau T;
     McLds<R> S;
@@ -568,7 +580,7 @@ __global__ __launch_bounds__(kMcThreads) void mc_traj_kernel(McArgs A) {
         S.x = (C2<R> *)p;
         p += sizeof(C2<R>) * npad;
         S.kd = (C2<R> *)p;
-        p += sizeof(C2<R>) * npad * (kMcStages - kRegStages);
+        p += sizeof(C2<R>) * npad * (kMcStages - RS);
         S.kfsal = (C2<R> *)p;
         p += sizeof(C2<R>) * npad;
         S.red = (double *)p;
@@ -581,10 +593,10 @@ __global__ __launch_bounds__(kMcThreads) void mc_traj_kernel(McArgs A) {
         const int nd = (int)(sizeof(McTableau) / sizeof(double));
         for (int i = threadIdx.x; i < nd; i += nt) dst[i] = src[i];
     }
-    for (int i = threadIdx.x; i < (kMcStages - kRegStages + 2) * npad; i += nt)
+    for (int i = threadIdx.x; i < (kMcStages - RS + 2) * npad; i += nt)
         S.x[i] = mk<R>(0, 0);  // x, k_10.. and the FSAL slot start at zero
     __syncthreads();
-    Ctx<R, NR> c{A, S, T};
+    Ctx<R, NR, RS> c{A, S, T};
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
         c.row[q] = threadIdx.x + q * nt;
@@ -593,7 +605,7 @@ __global__ __launch_bounds__(kMcThreads) void mc_traj_kernel(McArgs A) {
     }
 
     for (int traj = blockIdx.x; traj < A.ntraj; traj += gridDim.x) {
-        Traj<R, NR> s;
+        Traj<R, NR, RS> s;
         s.status = MC_OK;
         s.attempts = 0;
         Rng g{A.draws + (size_t)traj * A.max_draws, A.max_draws, 0};
@@ -730,32 +742,37 @@ __global__ __launch_bounds__(kMcThreads) void mc_traj_kernel(McArgs A) {
     }
 }
 
-template <typename R, int NR>
-int launch_mc_nr(const McArgs &A, int threads, int blocks, hipStream_t s) {
-    const size_t lds = mc_lds_bytes<R>(threads * NR, threads);
-    QA_HIP(hipFuncSetAttribute((const void *)mc_traj_kernel<R, NR>,
+template <typename R, int NR, int RS>
+int launch_mc_cfg(const McArgs &A, int threads, int blocks, hipStream_t s) {
+    const size_t lds = mc_lds_bytes<R>(threads * NR, threads, RS);
+    QA_HIP(hipFuncSetAttribute((const void *)mc_traj_kernel<R, NR, RS>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((mc_traj_kernel<R, NR>), dim3(blocks), dim3(threads), lds, s, A);
+    hipLaunchKernelGGL((mc_traj_kernel<R, NR, RS>), dim3(blocks), dim3(threads), lds, s, A);
     QA_CHECK_LAUNCH();
     return QA_OK;
 }
 
-// Rows per thread: NR = 1 up to 512 rows (one wave per 64 rows), then 2
-// rows per thread on at most 512 threads, which leaves each lane 256 VGPRs.
-inline void mc_shape(int d, int *nr, int *threads) {
-    int n = 1;
-    while (n < 2 && (d + n - 1) / n > kMcThreads) n *= 2;
-    *nr = n;
-    const int rows = (d + n - 1) / n;
-    *threads = ((rows + QA_WAVE - 1) / QA_WAVE) * QA_WAVE;
-}
+constexpr size_t kLdsBudget = 160 * 1024;  // LDS of one gfx950 CU
 
+// Launch shape (measured on the TFIM trajectories at d = 512 and 1024).
+// Every stage in LDS, one row per thread, when the 18 vectors take at most
+// half of the LDS: two workgroups then share a CU, which beats keeping the
+// step stages in registers.  Otherwise the step stages stay in registers,
+// one row per thread up to 512 rows and two rows per thread above.
 template <typename R>
 int launch_mc(const McArgs &A, int blocks, hipStream_t s) {
-    int nr, threads;
-    mc_shape(A.d, &nr, &threads);
-    if (nr == 1) return launch_mc_nr<R, 1>(A, threads, blocks, s);
-    return launch_mc_nr<R, 2>(A, threads, blocks, s);
+    const int t1 = ((A.d + QA_WAVE - 1) / QA_WAVE) * QA_WAVE;
+    if (t1 <= 2 * kMcThreads && mc_lds_bytes<R>(t1, t1, 0) <= kLdsBudget / 2)
+        return launch_mc_cfg<R, 1, 0>(A, t1, blocks, s);
+    if (t1 <= kMcThreads && mc_lds_bytes<R>(t1, t1, kRegStagesMax) <= kLdsBudget)
+        return launch_mc_cfg<R, 1, kRegStagesMax>(A, t1, blocks, s);
+    const int rows = (A.d + 1) / 2;
+    const int t2 = ((rows + QA_WAVE - 1) / QA_WAVE) * QA_WAVE;
+    if (t2 > kMcThreads || mc_lds_bytes<R>(2 * t2, t2, kRegStagesMax) > kLdsBudget) {
+        qa::set_error("qa_mc_run: d = %d does not fit the trajectory kernel", A.d);
+        return QA_ERR_ARG;
+    }
+    return launch_mc_cfg<R, 2, kRegStagesMax>(A, t2, blocks, s);
 }
 
 }  // namespace
