@@ -153,7 +153,32 @@ def bench_jc(qa, systems, log, reps=20, cpu=True):
     out = {"workload": "mesolve Jaynes-Cummings N=10 x qubit (Liouvillian 400^2, 1543 nnz), "
                        "t in [0,25], 100 outputs, vern7 rtol=1e-8 atol=1e-10",
            "solves_per_s": round(1 / dt, 2), "err_vs_fp64": float(err),
-           "rk_attempts": solver._integrator.stats()["attempts"]}
+           "rk_attempts": solver._integrator.stats()["attempts"],
+           "path": solver._integrator.stats().get("path", "fused engine")}
+    # a sweep: 2048 couplings g in [0.02, 0.09] x 2 pi, one launch, one
+    # workgroup per solve
+    from oracle import models as _m
+    from qutip_amd.solver.small import SmallSolve
+    gs = 2 * np.pi * np.linspace(0.02, 0.09, 2048)
+    Ls = []
+    for gg in gs:
+        mm = _m.jaynes_cummings(g=gg)
+        Ls.append(_m.liouvillian(mm["H"], mm["c_ops"]))
+    mm = _m.jaynes_cummings()
+    ss = SmallSolve(Ls, "vern7", rtol=1e-8, atol=1e-10)
+    from oracle import cbind as _cb
+    fptr, fidx, fw = _cb.functionals_from_ops(mm["e_ops"], 20)
+    ss.set_functionals(np.zeros(len(fptr) - 1, np.int32), fptr, np.zeros(len(fidx), np.int32),
+                       fidx, fw.view(np.complex128))
+    rho0 = np.tile(_m.vec(_m.ket2dm(mm["psi0"])), (len(gs), 1))
+    ss.run(0.0, rho0[:256], g["tlist"][1:])  # warm-up
+    t0 = time.perf_counter()
+    vals, _, info = ss.run(0.0, rho0, g["tlist"][1:])
+    dts = time.perf_counter() - t0
+    out["sweep_2048_couplings"] = {
+        "solves_per_s": round(len(gs) / dts, 1), "s": round(dts, 4),
+        "all_ok": bool((info[:, 0] >= 0).all()),
+        "rk_attempts_total": int(info[:, 3].sum())}
     if cpu:
         from oracle import cbind, models, tableaus
         m = models.jaynes_cummings()

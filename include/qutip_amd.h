@@ -280,6 +280,30 @@ int qa_mc_run(int d, int ntraj, int ntimes, const double *times_dev, const void 
               double *col_times_dev, int *col_which_dev, int max_jumps, int *info_dev,
               int grid, void *stream);
 
+/* ---- whole solves of small systems ----------------------------------------
+ * Replaces, for a constant generator with n <= 1024, the output loop of
+ * Solver.run (qutip/solver/solver_base.py:81-157) around
+ * Explicit_RungeKutta.integrate(t) (qutip/solver/integrator/explicit_rk.pyx:
+ * 278-331) and the e_ops evaluation of Result.add (result.py:303-340): one
+ * workgroup runs the whole integration of one system on the device.  nsys
+ * independent systems (initial states y0_dev [nsys][n] complex128, and
+ * operators h_stride ELL entries apart, 0 = one shared operator) run in one
+ * launch; the batch has no reference counterpart.  The generator A (dy/dt =
+ * A y) is a column-major ELL as for qa_mc_run.  Functionals as for
+ * qa_rk_set_functionals, on device.  Integrates from t0 through
+ * times_dev[0 .. ntimes-1]; fun_out_dev [nsys][ntimes][nfun] complex128;
+ * y_final_dev [nsys][n] (or NULL); info_dev [nsys][4]: status (the RK codes),
+ * outputs written, dense-output preparations, RK attempts.  The tableau is packed by
+ * qa_mc_pack_tableau.  fp32 != 0: complex64 operator, state and stages with
+ * FP64 error norm and functionals. */
+int qa_rk_small_run(int n, int nsys, double t0, int ntimes, const double *times_dev,
+                    const void *y0_dev, int h_width, const int *h_col, const void *h_vals,
+                    long long h_stride, int nfun, const int *fkind, const int *fptr,
+                    const int *frows, const int *fcols, const void *fw, const void *tableau_dev,
+                    double rtol, double atol, int nsteps, double first_step, double min_step,
+                    double max_step, int fp32, void *fun_out_dev, void *y_final_dev,
+                    int *info_dev, int grid, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

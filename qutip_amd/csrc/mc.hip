@@ -87,6 +87,16 @@ struct McArgs {
     double *col_times;    // [ntraj][max_jumps]
     int *col_which;
     int *info;  // [ntraj][4]: status, jumps, draws used, RK attempts
+    // whole-solve mode (rk_small_kernel): per-system operator stride
+    // (entries; 0 = one shared operator), start time, linear / quadratic
+    // functionals of the state, outputs
+    long long h_stride;
+    double t0;
+    int nfun;
+    const int *f_kind, *f_ptr, *f_row, *f_col;
+    const double2 *f_w;
+    double2 *fun_out;  // [nsys][ntimes][nfun]
+    double2 *y_final;  // [nsys][d] or null
 };
 
 template <typename R>
@@ -177,6 +187,7 @@ struct Traj {
     int status;    // trajectory status (McStatus)
     int rk_state;  // last RK status (INTERPOLATED marks a prepared dense output)
     long long attempts;
+    int ndense;  // dense-output preparations
 };
 
 template <typename R, int NR, int RS>
@@ -187,6 +198,8 @@ struct Ctx {
     int row[NR];
     bool act[NR];
     int rowc[NR];  // the row clamped into [0, d) for the ELL loads
+    const int *hcol;    // this system's H_eff ELL
+    const void *hval;
 };
 
 // Stage vector j; j is a compile-time constant after unrolling, so the
@@ -235,14 +248,14 @@ __device__ __forceinline__ void stage_in(Ctx<R, NR, RS> &c, const V<R, NR> &x) {
 // value loads first, then the LDS gathers and multiply-adds.
 template <int M, typename R, int NR, int RS>
 __device__ __forceinline__ void ell_chunk(Ctx<R, NR, RS> &c, int u0, V<R, NR> &acc) {
-    const C2<R> *hv = (const C2<R> *)c.A.h_vals;
+    const C2<R> *hv = (const C2<R> *)c.hval;
     const size_t d = (size_t)c.A.d;
     int cc[NR][M];
     C2<R> vv[NR][M];
 #pragma unroll
     for (int q = 0; q < NR; ++q)
 #pragma unroll
-        for (int w = 0; w < M; ++w) cc[q][w] = c.A.h_col[(size_t)(u0 + w) * d + c.rowc[q]];
+        for (int w = 0; w < M; ++w) cc[q][w] = c.hcol[(size_t)(u0 + w) * d + c.rowc[q]];
 #pragma unroll
     for (int q = 0; q < NR; ++q)
 #pragma unroll
@@ -352,6 +365,7 @@ __device__ __forceinline__ double compute_step(Ctx<R, NR, RS> &c, Traj<R, NR, RS
 template <typename R, int NR, int RS>
 __device__ __forceinline__ void prep_dense_out(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s) {
     const McTableau &T = c.T;
+    s.ndense++;
 #pragma unroll
     for (int i = 0; i < kMcStages; ++i) {
         if (i >= T.rk_step && i < T.rk_extra) {
@@ -567,6 +581,118 @@ __device__ __forceinline__ void write_expect(Ctx<R, NR, RS> &c, const V<R, NR> &
     }
 }
 
+// LDS carving, tableau copy and zeroing shared by the kernels of this file.
+template <typename R, int NR, int RS>
+__device__ __forceinline__ void kernel_prologue(const McArgs &A, unsigned char *lds,
+                                                McTableau &T, McLds<R> &S) {
+    const int nt = blockDim.x;
+    const int npad = nt * NR;
+    {
+        unsigned char *p = lds;
+        S.npad = npad;
+        S.x = (C2<R> *)p;
+        p += sizeof(C2<R>) * npad;
+        S.kd = (C2<R> *)p;
+        p += sizeof(C2<R>) * npad * (kMcStages - RS);
+        S.kfsal = (C2<R> *)p;
+        p += sizeof(C2<R>) * npad;
+        S.red = (double *)p;
+        p += sizeof(double) * (nt / QA_WAVE);
+        S.probs = (double *)p;
+    }
+    {
+        const double *src = (const double *)A.tab;
+        double *dst = (double *)&T;
+        const int nd = (int)(sizeof(McTableau) / sizeof(double));
+        for (int i = threadIdx.x; i < nd; i += nt) dst[i] = src[i];
+    }
+    for (int i = threadIdx.x; i < (kMcStages - RS + 2) * npad; i += nt)
+        S.x[i] = mk<R>(0, 0);  // x, the LDS stages and the FSAL slot start at zero
+    __syncthreads();
+}
+
+template <typename R, int NR, int RS>
+__device__ __forceinline__ void ctx_rows(Ctx<R, NR, RS> &c, const int *hcol, const void *hval) {
+    c.hcol = hcol;
+    c.hval = hval;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        c.row[q] = threadIdx.x + q * blockDim.x;
+        c.act[q] = c.row[q] < c.A.d;
+        c.rowc[q] = c.act[q] ? c.row[q] : 0;
+    }
+}
+
+// integrate(t, step=False) (explicit_rk.pyx:278-331): steps until t_front
+// reaches t within one budget of nsteps attempts, then interpolates (or
+// takes y_front).
+template <typename R, int NR, int RS>
+__device__ __forceinline__ void integrate_to(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, double t) {
+    int nsteps_left = c.A.nsteps;
+    if (t == s.t) return;
+    if (t < s.t_prev) {
+        s.status = MC_OUTSIDE_RANGE;
+        return;
+    }
+    if (c.T.interpolate && t < s.t_front) {
+        if (s.rk_state != RK_INTERPOLATED) {
+            prep_dense_out(c, s);
+            s.rk_state = RK_INTERPOLATED;
+        }
+        s.y = interpolate(c, s, t);
+        s.t = t;
+        return;
+    }
+    s.rk_state = RK_NORMAL;
+    while (s.t_front < t && s.status >= 0) {
+        s.yprev = s.yfront;
+        s.t_prev = s.t_front;
+        nsteps_left -= step_in_err(c, s, t, nsteps_left);
+    }
+    if (s.status < 0) return;
+    if (s.t_front > t) {
+        prep_dense_out(c, s);
+        s.rk_state = RK_INTERPOLATED;
+        s.t = t;
+        s.y = interpolate(c, s, t);
+    } else {
+        s.rk_state = RK_AT_FRONT;
+        s.t = s.t_front;
+        s.y = s.yfront;
+    }
+}
+
+// Functionals of the state y (linear: sum w y[col]; quadratic:
+// sum w conj(y[row]) y[col]), accumulated in FP64 in a fixed order, into
+// out[0 .. nfun-1] (thread 0 writes).
+template <typename R, int NR, int RS>
+__device__ __forceinline__ void write_functionals(Ctx<R, NR, RS> &c, const V<R, NR> &y,
+                                                  double2 *out) {
+    stage_in(c, y);
+    for (int f = 0; f < c.A.nfun; ++f) {
+        const int quad = c.A.f_kind[f];
+        double re = 0.0, im = 0.0;
+        for (int e = c.A.f_ptr[f] + threadIdx.x; e < c.A.f_ptr[f + 1]; e += blockDim.x) {
+            const C2<R> xc = c.S.x[c.A.f_col[e]];
+            const double2 w = c.A.f_w[e];
+            double vr = w.x * (double)xc.x - w.y * (double)xc.y;
+            double vi = w.x * (double)xc.y + w.y * (double)xc.x;
+            if (quad) {
+                const C2<R> xr = c.S.x[c.A.f_row[e]];
+                const double lr = (double)xr.x, li = -(double)xr.y;
+                const double tr = lr * vr - li * vi;
+                vi = lr * vi + li * vr;
+                vr = tr;
+            }
+            re += vr;
+            im += vi;
+        }
+        re = bsum(c, re);
+        im = bsum(c, im);
+        if (threadIdx.x == 0) out[f] = make_double2(re, im);
+    }
+}
+
 template <typename R, int NR, int RS>
 __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj_kernel(McArgs A) {
     extern __shared__ __align__(16) unsigned char mc_dyn_lds[];
@@ -594,9 +720,13 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj
         for (int i = threadIdx.x; i < nd; i += nt) dst[i] = src[i];
     }
     for (int i = threadIdx.x; i < (kMcStages - RS + 2) * npad; i += nt)
-        S.x[i] = mk<R>(0, 0);  // x, k_10.. and the FSAL slot start at zero
+        S.x[i] = mk<R>(0, 0);  // x, the LDS stages and the FSAL slot start at zero
     __syncthreads();
+    // (written out here rather than through kernel_prologue / ctx_rows: the
+    // shared helpers cost this kernel 100 B/lane more register spill)
     Ctx<R, NR, RS> c{A, S, T};
+    c.hcol = A.h_col;
+    c.hval = A.h_vals;
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
         c.row[q] = threadIdx.x + q * nt;
@@ -608,6 +738,7 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj
         Traj<R, NR, RS> s;
         s.status = MC_OK;
         s.attempts = 0;
+        s.ndense = 0;
         Rng g{A.draws + (size_t)traj * A.max_draws, A.max_draws, 0};
         int jumps = 0;
         double target = 0.0;
@@ -742,12 +873,68 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj
     }
 }
 
+// Whole solves of small systems (sesolve / mesolve with n <= 1024): one
+// workgroup integrates one system from t0 through every output time (the
+// integrate(t) loop of the reference with interpolation), evaluating the
+// functionals at each time, with no host round trip.  Systems are
+// independent initial states (and optionally operators, h_stride apart),
+// picked up grid-stride.
 template <typename R, int NR, int RS>
+__global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void rk_small_kernel(McArgs A) {
+    extern __shared__ __align__(16) unsigned char mc_dyn_lds[];
+    __shared__ McTableau T;
+    McLds<R> S;
+    kernel_prologue<R, NR, RS>(A, mc_dyn_lds, T, S);
+    Ctx<R, NR, RS> c{A, S, T};
+    for (int sys = blockIdx.x; sys < A.ntraj; sys += gridDim.x) {
+        const size_t hoff = (size_t)sys * (size_t)A.h_stride;
+        ctx_rows(c, A.h_col + hoff,
+                 (const void *)((const C2<R> *)A.h_vals + hoff));
+        Traj<R, NR, RS> s;
+        s.status = MC_OK;
+        s.attempts = 0;
+        s.ndense = 0;
+        V<R, NR> y0;
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+            const double2 p0 =
+                c.act[q] ? A.psi0[(size_t)sys * A.d + c.row[q]] : make_double2(0.0, 0.0);
+            y0.v[q] = mk<R>((R)p0.x, (R)p0.y);
+        }
+        set_initial_value(c, s, y0, A.t0);
+        int k = 0;
+        for (; k < A.ntimes && s.status >= 0; ++k) {
+            integrate_to(c, s, A.times[k]);
+            if (s.status < 0) break;
+            write_functionals(c, s.y, A.fun_out + ((size_t)sys * A.ntimes + k) * A.nfun);
+        }
+        if (A.y_final) {
+#pragma unroll
+            for (int q = 0; q < NR; ++q)
+                if (c.act[q])
+                    A.y_final[(size_t)sys * A.d + c.row[q]] =
+                        make_double2((double)s.y.v[q].x, (double)s.y.v[q].y);
+        }
+        if (threadIdx.x == 0) {
+            A.info[(size_t)sys * 4 + 0] = s.status;
+            A.info[(size_t)sys * 4 + 1] = k;  // outputs written
+            A.info[(size_t)sys * 4 + 2] = s.ndense;
+            A.info[(size_t)sys * 4 + 3] = (int)s.attempts;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename R, int NR, int RS, bool SMALL>
 int launch_mc_cfg(const McArgs &A, int threads, int blocks, hipStream_t s) {
     const size_t lds = mc_lds_bytes<R>(threads * NR, threads, RS);
-    QA_HIP(hipFuncSetAttribute((const void *)mc_traj_kernel<R, NR, RS>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((mc_traj_kernel<R, NR, RS>), dim3(blocks), dim3(threads), lds, s, A);
+    const void *fn = SMALL ? (const void *)rk_small_kernel<R, NR, RS>
+                           : (const void *)mc_traj_kernel<R, NR, RS>;
+    QA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (SMALL)
+        hipLaunchKernelGGL((rk_small_kernel<R, NR, RS>), dim3(blocks), dim3(threads), lds, s, A);
+    else
+        hipLaunchKernelGGL((mc_traj_kernel<R, NR, RS>), dim3(blocks), dim3(threads), lds, s, A);
     QA_CHECK_LAUNCH();
     return QA_OK;
 }
@@ -759,20 +946,20 @@ constexpr size_t kLdsBudget = 160 * 1024;  // LDS of one gfx950 CU
 // half of the LDS: two workgroups then share a CU, which beats keeping the
 // step stages in registers.  Otherwise the step stages stay in registers,
 // one row per thread up to 512 rows and two rows per thread above.
-template <typename R>
+template <typename R, bool SMALL = false>
 int launch_mc(const McArgs &A, int blocks, hipStream_t s) {
     const int t1 = ((A.d + QA_WAVE - 1) / QA_WAVE) * QA_WAVE;
     if (t1 <= 2 * kMcThreads && mc_lds_bytes<R>(t1, t1, 0) <= kLdsBudget / 2)
-        return launch_mc_cfg<R, 1, 0>(A, t1, blocks, s);
+        return launch_mc_cfg<R, 1, 0, SMALL>(A, t1, blocks, s);
     if (t1 <= kMcThreads && mc_lds_bytes<R>(t1, t1, kRegStagesMax) <= kLdsBudget)
-        return launch_mc_cfg<R, 1, kRegStagesMax>(A, t1, blocks, s);
+        return launch_mc_cfg<R, 1, kRegStagesMax, SMALL>(A, t1, blocks, s);
     const int rows = (A.d + 1) / 2;
     const int t2 = ((rows + QA_WAVE - 1) / QA_WAVE) * QA_WAVE;
     if (t2 > kMcThreads || mc_lds_bytes<R>(2 * t2, t2, kRegStagesMax) > kLdsBudget) {
         qa::set_error("qa_mc_run: d = %d does not fit the trajectory kernel", A.d);
         return QA_ERR_ARG;
     }
-    return launch_mc_cfg<R, 2, kRegStagesMax>(A, t2, blocks, s);
+    return launch_mc_cfg<R, 2, kRegStagesMax, SMALL>(A, t2, blocks, s);
 }
 
 }  // namespace
@@ -883,6 +1070,57 @@ int qa_mc_run(int d, int ntraj, int ntimes, const double *times_dev, const void 
     if (grid > ntraj) grid = ntraj;
     hipStream_t s = (hipStream_t)stream;
     return fp32 ? launch_mc<float>(A, grid, s) : launch_mc<double>(A, grid, s);
+}
+
+
+int qa_rk_small_run(int n, int nsys, double t0, int ntimes, const double *times_dev,
+                    const void *y0_dev, int h_width, const int *h_col, const void *h_vals,
+                    long long h_stride, int nfun, const int *fkind, const int *fptr,
+                    const int *frows, const int *fcols, const void *fw, const void *tableau_dev,
+                    double rtol, double atol, int nsteps, double first_step, double min_step,
+                    double max_step, int fp32, void *fun_out_dev, void *y_final_dev,
+                    int *info_dev, int grid, void *stream) {
+    QA_REQUIRE(n >= 1 && n <= 2 * kMcThreads, "qa_rk_small_run: 1 <= n <= %d", 2 * kMcThreads);
+    QA_REQUIRE(nsys >= 0 && ntimes >= 0 && nfun >= 0, "qa_rk_small_run: negative size");
+    QA_REQUIRE(h_width >= 0 && h_width % 4 == 0 && h_width <= n + 3 && h_stride >= 0,
+               "qa_rk_small_run: h_width must be a multiple of 4, at most n + 3");
+    QA_REQUIRE(y0_dev && tableau_dev && info_dev && (ntimes == 0 || times_dev) &&
+                   (h_width == 0 || (h_col && h_vals)) &&
+                   (nfun == 0 || (fkind && fptr && frows && fcols && fw && fun_out_dev)),
+               "qa_rk_small_run: null pointer");
+    QA_REQUIRE(nsteps >= 0, "qa_rk_small_run: nsteps >= 0");
+    if (nsys == 0) return QA_OK;
+    McArgs A;
+    memset(&A, 0, sizeof(A));
+    A.d = n;
+    A.ntraj = nsys;
+    A.ntimes = ntimes;
+    A.h_width = h_width;
+    A.h_col = h_col;
+    A.h_vals = h_vals;
+    A.h_stride = h_stride;
+    A.tab = (const McTableau *)tableau_dev;
+    A.times = times_dev;
+    A.t0 = t0;
+    A.psi0 = (const double2 *)y0_dev;
+    A.rtol = rtol;
+    A.atol = atol;
+    A.first_step = first_step;
+    A.min_step = min_step;
+    A.max_step = max_step;
+    A.nsteps = nsteps;
+    A.nfun = nfun;
+    A.f_kind = fkind;
+    A.f_ptr = fptr;
+    A.f_row = frows;
+    A.f_col = fcols;
+    A.f_w = (const double2 *)fw;
+    A.fun_out = (double2 *)fun_out_dev;
+    A.y_final = (double2 *)y_final_dev;
+    A.info = info_dev;
+    if (grid <= 0 || grid > nsys) grid = nsys;
+    hipStream_t s = (hipStream_t)stream;
+    return fp32 ? launch_mc<float, true>(A, grid, s) : launch_mc<double, true>(A, grid, s);
 }
 
 }  // extern "C"

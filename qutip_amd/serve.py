@@ -6,8 +6,9 @@ every admitted request owns its device-resident operators, state and work
 vectors and runs on its own HIP stream, so the work of concurrent requests
 overlaps on the device:
 
-* sesolve: the fused RK engine on the sparse -iH (device step control,
-  expectation values evaluated on the device at every output time);
+* sesolve: the whole solve in one workgroup (d <= 1024: steps, step
+  control, dense output and the expectation values at every output time in
+  one kernel), else the fused RK engine on the sparse -iH;
 * mesolve: the fused RK engine on the matrix-form Lindblad RHS (MFMA
   products; dense collapse operators make the superoperator infeasible:
   d^4 non-zeros);
@@ -32,6 +33,7 @@ import scipy.sparse as sp
 from . import _lib
 from . import data as _data
 from .core import Qobj
+from .solver import small as _small
 from .solver.hip_rk import RKEngine
 from .solver.mcsolve import MCSolver, _read_seeds, draws_for
 
@@ -89,20 +91,30 @@ class _Running:
         self.kind = req.kind
         o = req.options
         tl = req.tlist
-        if req.kind in ("se", "me"):
+        if req.kind == "se" and req.d <= _small.MAX_N:
+            # the whole solve in one workgroup, on the request's own stream
+            A = -1j * req.H.to_csr()
+            self.small = _small.SmallSolve(A, "vern7", o["rtol"], o["atol"], o["nsteps"])
+            psi = req.psi0.full().ravel()
+            kinds, fptr, rows, cols, w = self._quadratic(req, psi.size)
+            cat = lambda xs, dt: np.concatenate(xs).astype(dt)
+            self.small.set_functionals(np.asarray(kinds, np.int32), np.asarray(fptr, np.int32),
+                                       cat(rows, np.int32), cat(cols, np.int32),
+                                       cat(w, np.complex128))
+            self._first = self._functionals_host(psi, req)
+            self.kind = "se_small"
+            self.stream = C.c_void_p()
+            _lib.call("qa_stream_create", C.byref(self.stream))
+            self.event = C.c_void_p()
+            _lib.call("qa_event_create", C.byref(self.event))
+            self._h = self.small.launch(tl[0], psi, tl[1:], stream=self.stream)
+            _lib.call("qa_event_record", self.event, self.stream)
+        elif req.kind in ("se", "me"):
             if req.kind == "se":
                 A = -1j * req.H.to_csr()
                 self.eng = RKEngine(A, "vern7", o["rtol"], o["atol"], o["nsteps"])
                 psi = req.psi0.full().ravel()
-                n = psi.size
-                kinds, fptr, rows, cols, w = [], [0], [], [], []
-                for O in req.e_ops + [None]:
-                    M = sp.coo_matrix(O.to_csr()) if O is not None else sp.identity(n, format="coo")
-                    kinds.append(1)
-                    rows.append(M.row)
-                    cols.append(M.col)
-                    w.append(M.data.astype(np.complex128))
-                    fptr.append(fptr[-1] + M.nnz)
+                kinds, fptr, rows, cols, w = self._quadratic(req, psi.size)
                 y0 = psi
             else:
                 cs = [c.full() for c in req.c_ops]
@@ -148,6 +160,19 @@ class _Running:
             _lib.call("qa_event_record", self.event, self.stream)
 
     @staticmethod
+    def _quadratic(req, n):
+        """<psi|O|psi> for every e_op and ||psi||^2 as quadratic functionals."""
+        kinds, fptr, rows, cols, w = [], [0], [], [], []
+        for O in req.e_ops + [None]:
+            M = sp.coo_matrix(O.to_csr()) if O is not None else sp.identity(n, format="coo")
+            kinds.append(1)
+            rows.append(M.row)
+            cols.append(M.col)
+            w.append(M.data.astype(np.complex128))
+            fptr.append(fptr[-1] + M.nnz)
+        return kinds, fptr, rows, cols, w
+
+    @staticmethod
     def _functionals_host(y0, req):
         vals = []
         if req.kind == "se":
@@ -165,6 +190,24 @@ class _Running:
     def poll(self):
         """Advance without blocking; True when the request has finished."""
         if self.done:
+            return True
+        if self.kind == "se_small":
+            ok = C.c_int(0)
+            _lib.call("qa_event_query", self.event, C.byref(ok))
+            if not ok.value:
+                return False
+            vals, _, info = self.small.collect(self._h)
+            self._h = None
+            if info[0, 0] < 0:
+                self.req.error = f"integration failed (status {int(info[0, 0])})"
+            else:
+                v = vals[0]
+                vals = v[:, :-1] / v[:, -1:].real
+                first = np.asarray(self._first, dtype=np.complex128)[None, :]
+                self.req.result = np.vstack([first, vals]).T
+            _lib.call("qa_event_destroy", self.event)
+            _lib.call("qa_stream_destroy", self.stream)
+            self.done = True
             return True
         if self.kind in ("se", "me"):
             st = C.c_int(0)

@@ -25,6 +25,7 @@ import scipy.sparse as sp
 
 from .. import _lib, autotune
 from .. import data as _data
+from . import small as _small
 from .integrator import Integrator, IntegratorException
 from .tableaus import TABLEAUS
 
@@ -412,6 +413,10 @@ class IntegratorHipRK(Integrator):
         o = self.options
         self._engine = None
         self._plain = None
+        self._small = None
+        self._small_start = None   # (t0, device state) of the last set_state
+        self._small_final = None   # (t, device state) after a whole-solve run
+        self._small_stats = None
         owner = getattr(self.derivative, "__self__", None)
         dense = getattr(owner, "dense_operators", None) if o["fused"] else None
         op = constant_operator(self.derivative) if (o["fused"] and dense is None) else None
@@ -426,6 +431,12 @@ class IntegratorHipRK(Integrator):
                                     o["first_step"], o["min_step"], o["max_step"],
                                     o["interpolate"], fp32=(o["operator_precision"] == "fp32"))
             self.name = "hip_rk (fused)"
+            if op.shape[0] <= _small.MAX_N:
+                # whole solves (Solver.run) of a small system in one workgroup
+                self._small = _small.SmallSolve(
+                    op, o["tableau"], o["rtol"], o["atol"], o["nsteps"], o["first_step"],
+                    o["min_step"], o["max_step"], o["interpolate"],
+                    fp32=(o["operator_precision"] == "fp32"))
         else:
             self._plain = PlainRK(self.derivative, o["tableau"], o["rtol"], o["atol"],
                                   o["nsteps"], o["first_step"], o["min_step"], o["max_step"],
@@ -458,9 +469,21 @@ class IntegratorHipRK(Integrator):
             self._engine.set_state(t, y.reshape((y.size, 1)))
         else:
             self._plain.set_initial_value(y.copy(), t)
+        self._small_start = (t, y.copy()) if self._engine is not None else None
+        self._small_final = None
         self._is_set = True
 
+    def _leave_small(self):
+        """Continue step-wise from where a whole-solve run ended."""
+        if self._small_final is not None:
+            t, y = self._small_final
+            self._small_final = None
+            self._engine.set_state(t, y.reshape((y.size, 1)))
+
     def get_state(self, copy=True):
+        if self._small_final is not None:
+            t, y = self._small_final
+            return t, (y.copy() if copy else y).reshape(self._shape)
         if self._engine is not None:
             t, y = self._engine.get_state(copy)
             return t, y.reshape(self._shape)
@@ -474,6 +497,7 @@ class IntegratorHipRK(Integrator):
 
     def integrate(self, t, copy=True):
         if self._engine is not None:
+            self._leave_small()
             self._engine.run([t])
         else:
             self._plain.integrate(t, step=False)
@@ -482,6 +506,7 @@ class IntegratorHipRK(Integrator):
 
     def mcstep(self, t, copy=True):
         if self._engine is not None:
+            self._leave_small()
             self._engine.run([t], single_step=True)
         else:
             self._plain.integrate(t, step=True)
@@ -493,11 +518,32 @@ class IntegratorHipRK(Integrator):
         ``tlist[1:]`` evaluated on the device.  Returns [len(tlist)-1, nfun]."""
         if self._engine is None:
             raise IntegratorException("run_functionals needs the fused engine")
+        times = np.asarray(tlist[1:], dtype=np.float64)
+        if self._small is not None and self._small_start is not None:
+            t0, y0 = self._small_start
+            self._small.set_functionals(kinds, fptr, rows, cols, w)
+            vals, yfin, info = self._small.run(t0, y0, times)
+            st = int(info[0, 0])
+            if st < 0:
+                raise IntegratorException(STATUS_MESSAGES.get(st, f"status {st}"))
+            yf = _data.HipDense((self._small.n, 1))
+            _lib.call("qa_memcpy_d2d", C.c_void_p(yf.ptr), C.c_void_p(yfin.ptr),
+                      C.c_size_t(16 * self._small.n), None)
+            self._small_final = (float(times[-1]) if len(times) else t0, yf)
+            self._small_stats = {"attempts": int(info[0, 3]), "dense_preps": int(info[0, 2]),
+                                 "rhs_evals": self._small.rhs_evals(info[0]),
+                                 "outputs": int(info[0, 1]), "status": st,
+                                 "path": "whole solve in one workgroup"}
+            return vals[0]
+        self._small_stats = None
+        self._leave_small()
         self._engine.set_functionals(kinds, fptr, rows, cols, w)
-        vals, _, _ = self._engine.run(np.asarray(tlist[1:], dtype=np.float64))
+        vals, _, _ = self._engine.run(times)
         return vals
 
     def stats(self):
+        if self._small_stats is not None:
+            return dict(self._small_stats)
         return self._engine.stats() if self._engine is not None else {}
 
     @property
