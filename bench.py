@@ -121,10 +121,17 @@ def bench_mcsolve(qa, systems, log, ntraj=2048, reps=3, cpu=True):
             mean, sig = v.mean(0), v.std(0) / np.sqrt(ntraj)
             dev = np.abs(mean - golden["expect"][k])
             worst_sigma = max(worst_sigma, float((dev[1:] / sig[1:]).max()))
+        # operator bytes the trajectories read from L2: at least 10 RHS per
+        # attempt, each reading the whole ELL (width x d columns and values)
+        width = solver._device_ops(prec == "fp32")[1][0]
+        vb = 8 if prec == "fp32" else 16
+        op_bytes = 10 * float(info[:, 3].sum()) * width * s["H"].shape[0] * (vb + 4)
         out[prec] = {"traj_per_s": round(ntraj / best, 1), "s_per_batch": round(best, 4),
                      "all_ok": ok, "jumps": int(info[:, 1].sum()),
                      "rk_attempts": int(info[:, 3].sum()),
-                     "max_dev_over_sigma_vs_mesolve": round(worst_sigma, 3)}
+                     "max_dev_over_sigma_vs_mesolve": round(worst_sigma, 3),
+                     "ell_width": int(width),
+                     "operator_l2_read_gbs_lower_bound": round(op_bytes / best / 1e9, 1)}
         log(f"mcsolve {prec}: {ntraj / best:.0f} traj/s ({best * 1e3:.1f} ms), worst "
             f"{worst_sigma:.2f} sigma, jumps {int(info[:, 1].sum())}")
     if cpu:
