@@ -345,8 +345,10 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 struct DenseOp {
     int N, nc, prec;  // prec 0: FP32 MFMA, 1: bf16 x3 split
     int nsplit;       // waves per 16 x 16 tile (split K)
+    int tiled;        // 1: 32 x 32 workgroup tiles staged through LDS (large N)
     const float2 *Hm, *Hp, *C, *Cd;  // N x N each; C / Cd: nc blocks
     float2 *T;                        // nc blocks of scratch
+    float2 *Xf;                       // the stage input in complex64 (tiled path)
 };
 
 __device__ __forceinline__ float2 tof(double2 a) { return make_float2((float)a.x, (float)a.y); }
@@ -367,6 +369,9 @@ __device__ __forceinline__ void zmma_f32(v4f &re, v4f &im, const TA *__restrict_
     const int r = lane & 15, kq = lane >> 4;
     const TA *pa = A + (size_t)(4 * kq) * N + i0 + r;
     const TB *pb = B + (size_t)(j0 + r) * N + 4 * kq;
+    // two accumulator chains per part (four independent MFMA chains), added
+    // at the end: back-to-back MFMAs on one accumulator wait for each other
+    v4f re2 = {0.f, 0.f, 0.f, 0.f}, im2 = {0.f, 0.f, 0.f, 0.f};
     float2 a[4], b[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -384,9 +389,9 @@ __device__ __forceinline__ void zmma_f32(v4f &re, v4f &im, const TA *__restrict_
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             re = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].x, re, 0, 0, 0);
-            re = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[u].y, b[u].y, re, 0, 0, 0);
+            re2 = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[u].y, b[u].y, re2, 0, 0, 0);
             im = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].y, im, 0, 0, 0);
-            im = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[u].x, im, 0, 0, 0);
+            im2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[u].x, im2, 0, 0, 0);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -394,6 +399,8 @@ __device__ __forceinline__ void zmma_f32(v4f &re, v4f &im, const TA *__restrict_
             b[u] = bn[u];
         }
     }
+    re += re2;
+    im += im2;
 }
 
 __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
@@ -411,6 +418,7 @@ __device__ __forceinline__ void zmma_bf16x3(v4f &re, v4f &im, const TA *__restri
     const int r = lane & 15, kq = lane >> 4;
     const TA *pa = A + (size_t)(8 * kq) * N + i0 + r;
     const TB *pb = B + (size_t)(j0 + r) * N + 8 * kq;
+    v4f re2 = {0.f, 0.f, 0.f, 0.f}, im2 = {0.f, 0.f, 0.f, 0.f};
     float2 a[8], b[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -448,21 +456,23 @@ __device__ __forceinline__ void zmma_bf16x3(v4f &re, v4f &im, const TA *__restri
         re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brh, re, 0, 0, 0);
         re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brl, re, 0, 0, 0);
         re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, brh, re, 0, 0, 0);
-        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bih, re, 0, 0, 0);
-        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bil, re, 0, 0, 0);
-        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail, bih, re, 0, 0, 0);
+        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bih, re2, 0, 0, 0);
+        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bil, re2, 0, 0, 0);
+        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail, bih, re2, 0, 0, 0);
         im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bih, im, 0, 0, 0);
         im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bil, im, 0, 0, 0);
         im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, bih, im, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brh, im, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brl, im, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, brh, im, 0, 0, 0);
+        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brh, im2, 0, 0, 0);
+        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brl, im2, 0, 0, 0);
+        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, brh, im2, 0, 0, 0);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             a[u] = an[u];
             b[u] = bn[u];
         }
     }
+    re += re2;
+    im += im2;
 }
 
 // The K range of wave w of a workgroup of `nsplit` waves: the tile's sum is
@@ -579,6 +589,277 @@ __global__ __launch_bounds__(64 * kMaxSplit) void rk_dense_stage_kernel(EpiArgs 
         part = wave_sum(part);
         if (lane == 0) P.partials[blockIdx.y * gridDim.x + blockIdx.x] = part;
     }
+}
+
+// ---- large N: 32 x 32 workgroup tiles through LDS ------------------------
+// A workgroup of 4 waves owns a 32 x 32 output tile (each wave one 16 x 16
+// MFMA tile).  K goes in chunks of 64: the 256 threads load the A chunk
+// (32 rows x 64 k, one 256-byte column piece per k) and the B chunk (64 k x
+// 32 columns, one 512-byte piece per column) into LDS, the next chunk's loads are issued before the current
+// chunk's MFMAs (one chunk of register prefetch, double-buffered LDS), and
+// every A and B element loaded serves two waves.  All products of a stage
+// run through one flattened chunk loop, so one memory latency is exposed
+// per stage.  Used for N >= kTiledN, where the one-tile-per-workgroup
+// kernels above re-read every operand row and column 32 times from L2.
+constexpr int kGT = 32;      // output tile edge per workgroup
+constexpr int kGK = 64;      // K chunk
+constexpr int kGP = 2;       // LDS row padding (float2) against bank conflicts
+constexpr int kTiledN = 384;
+
+struct GemmTile {
+    float2 a[2][kGK][kGT + kGP];  // a[buf][k][i]
+    float2 b[2][kGT][kGK + kGP];  // b[buf][j][k]
+};
+
+constexpr int kGQ = kGT * kGK / 256;  // chunk elements per thread, per operand
+
+template <typename TA, typename TB>
+__device__ __forceinline__ void gt_fetch(const TA *__restrict__ A, const TB *__restrict__ B,
+                                         int N, int i0, int j0, int k0, int tid,
+                                         float2 (&ra)[kGQ], float2 (&rb)[kGQ]) {
+#pragma unroll
+    for (int q = 0; q < kGQ; ++q) {
+        const int e = tid + 256 * q;
+        ra[q] = tof(A[(size_t)(k0 + e / kGT) * N + i0 + e % kGT]);
+        rb[q] = tof(B[(size_t)(j0 + e / kGK) * N + k0 + e % kGK]);
+    }
+}
+
+__device__ __forceinline__ void gt_store(GemmTile &L, int buf, int tid, const float2 (&ra)[kGQ],
+                                         const float2 (&rb)[kGQ]) {
+#pragma unroll
+    for (int q = 0; q < kGQ; ++q) {
+        const int e = tid + 256 * q;
+        L.a[buf][e / kGT][e % kGT] = ra[q];
+        L.b[buf][e / kGK][e % kGK] = rb[q];
+    }
+}
+
+// The wave's 16 x 16 tile += A_chunk B_chunk from LDS buffer `buf`.
+__device__ __forceinline__ void gt_mma(const GemmTile &L, int buf, int prec, int wi, int wj,
+                                       int lane, v4f &re, v4f &im) {
+    const int r = lane & 15, kq = lane >> 4;
+    v4f re2 = {0.f, 0.f, 0.f, 0.f}, im2 = {0.f, 0.f, 0.f, 0.f};
+    if (prec == 1) {
+#pragma unroll
+        for (int st = 0; st < kGK / 32; ++st) {
+        v8bf arh, arl, aih, ail, naih, nail, brh, brl, bih, bil;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const float2 a = L.a[buf][32 * st + 8 * kq + u][16 * wi + r];
+            const float2 b = L.b[buf][16 * wj + r][32 * st + 8 * kq + u];
+            __bf16 h, l;
+            split_bf16(a.x, h, l);
+            arh[u] = h;
+            arl[u] = l;
+            split_bf16(a.y, h, l);
+            aih[u] = h;
+            ail[u] = l;
+            naih[u] = -h;
+            nail[u] = -l;
+            split_bf16(b.x, h, l);
+            brh[u] = h;
+            brl[u] = l;
+            split_bf16(b.y, h, l);
+            bih[u] = h;
+            bil[u] = l;
+        }
+        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brh, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brl, re, 0, 0, 0);
+        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, brh, re, 0, 0, 0);
+        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bih, re2, 0, 0, 0);
+        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bil, re2, 0, 0, 0);
+        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail, bih, re2, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bih, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bil, im, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, bih, im, 0, 0, 0);
+        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brh, im2, 0, 0, 0);
+        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brl, im2, 0, 0, 0);
+        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, brh, im2, 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+        for (int st = 0; st < kGK / 4; ++st) {
+            const float2 a = L.a[buf][4 * st + kq][16 * wi + r];
+            const float2 b = L.b[buf][16 * wj + r][4 * st + kq];
+            re = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, re, 0, 0, 0);
+            re2 = __builtin_amdgcn_mfma_f32_16x16x4f32(-a.y, b.y, re2, 0, 0, 0);
+            im = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, im, 0, 0, 0);
+            im2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, im2, 0, 0, 0);
+        }
+    }
+    re += re2;
+    im += im2;
+}
+
+// Operand pair of product p of a stage: Hm Xf, Xf Hp, then T_c Cd_c.
+__device__ __forceinline__ void gt_operands(const DenseOp &D, int p, const float2 *&A,
+                                            const float2 *&B) {
+    const size_t NN = (size_t)D.N * D.N;
+    if (p == 0) {
+        A = D.Hm;
+        B = D.Xf;
+    } else if (p == 1) {
+        A = D.Xf;
+        B = D.Hp;
+    } else {
+        A = D.T + (size_t)(p - 2) * NN;
+        B = D.Cd + (size_t)(p - 2) * NN;
+    }
+}
+
+// The chunks of one output tile are split over the two halves of a
+// 512-thread workgroup (chunk 2 s + h to half h, each half with its own LDS
+// pipeline), which puts two waves on every SIMD; half 1's partial tile is
+// then added to half 0's through LDS (fixed order).  Both halves run the
+// same number of loop trips (a half past the end recomputes its last chunk
+// and discards it), so every barrier is reached by all threads.
+constexpr int kGHalves = 2;
+
+struct GemmShared {
+    GemmTile L[kGHalves];
+};
+
+// Product p's operands: p < 0 selects the T kernel's single product C_c X.
+template <typename TB>
+__device__ __forceinline__ void gt_run(const DenseOp &D, GemmShared &G, int np,
+                                       const float2 *A0, const TB *B0, int i0, int j0,
+                                       v4f &re, v4f &im) {
+    const int h = threadIdx.x >> 8, tid = threadIdx.x & 255;
+    const int lane = threadIdx.x & 63, w = tid >> 6, wi = w & 1, wj = w >> 1;
+    const int nch = D.N / kGK, total = np * nch;
+    const int trips = (total + kGHalves - 1) / kGHalves;
+    GemmTile &L = G.L[h];
+    auto operands = [&](int idx, const float2 *&A, const TB *&B, int &k0) {
+        const int p = idx / nch;
+        k0 = (idx % nch) * kGK;
+        if (A0) {
+            A = A0;
+            B = B0;
+        } else {
+            const float2 *Bf;
+            gt_operands(D, p, A, Bf);
+            B = (const TB *)Bf;
+        }
+    };
+    float2 ra[kGQ], rb[kGQ];
+    const float2 *A;
+    const TB *B;
+    int k0;
+    operands(min(h, total - 1), A, B, k0);
+    gt_fetch(A, B, D.N, i0, j0, k0, tid, ra, rb);
+    gt_store(L, 0, tid, ra, rb);
+    __syncthreads();
+    for (int it = 0; it < trips; ++it) {
+        const int cur = kGHalves * it + h;
+        const int nx = min(cur + kGHalves, total - 1);
+        operands(nx, A, B, k0);
+        gt_fetch(A, B, D.N, i0, j0, k0, tid, ra, rb);
+        if (cur < total) gt_mma(L, it & 1, D.prec, wi, wj, lane, re, im);
+        gt_store(L, (it + 1) & 1, tid, ra, rb);
+        __syncthreads();
+    }
+    // half 1 hands its partial tile to half 0 through its own LDS tile
+    float *red = (float *)&G.L[1];
+    if (h == 1) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            red[(w * 8 + v) * 64 + lane] = re[v];
+            red[(w * 8 + 4 + v) * 64 + lane] = im[v];
+        }
+    }
+    __syncthreads();
+    if (h == 0) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            re[v] += red[(w * 8 + v) * 64 + lane];
+            im[v] += red[(w * 8 + 4 + v) * 64 + lane];
+        }
+    }
+}
+
+// T_c = C_c X for the workgroup's tile (grid N/32, N/32, max(nc, 1)); the
+// blockIdx.z == 0 workgroups also write Xf = complex64(X) for their tile.
+__device__ __forceinline__ void dense_t_tiled(const DenseOp &D, const double2 *X) {
+    __shared__ GemmShared G;
+    const int i0 = blockIdx.x * kGT, j0 = blockIdx.y * kGT, c = blockIdx.z;
+    const size_t NN = (size_t)D.N * D.N;
+    if (c == 0) {
+        for (int e = threadIdx.x; e < kGT * kGT; e += blockDim.x) {
+            const size_t idx = (size_t)(j0 + e / kGT) * D.N + i0 + e % kGT;
+            D.Xf[idx] = tof(X[idx]);
+        }
+    }
+    if (c >= D.nc) return;  // uniform over the workgroup
+    v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
+    gt_run<double2>(D, G, 1, D.C + (size_t)c * NN, X, i0, j0, re, im);
+    if ((threadIdx.x >> 8) != 0) return;
+    const int lane = threadIdx.x & 63, w = (threadIdx.x & 255) >> 6;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        D.T[c * NN + tile_index(D.N, i0 + 16 * (w & 1), j0 + 16 * (w >> 1), lane, v)] =
+            make_float2(re[v], im[v]);
+}
+
+// k tile = Hm Xf + Xf Hp + sum_c T_c Cd_c; complete in half 0.
+__device__ __forceinline__ void dense_k_tiled(const DenseOp &D, GemmShared &G, v4f &re,
+                                              v4f &im) {
+    gt_run<float2>(D, G, 2 + D.nc, nullptr, (const float2 *)nullptr, blockIdx.x * kGT,
+                   blockIdx.y * kGT, re, im);
+}
+
+__global__ __launch_bounds__(256 * kGHalves) void rk_dense_t_tiled_kernel(EpiArgs P, DenseOp D) {
+    if (P.ctl->phase != P.run_phase) return;
+    const StageVecs v = stage_vecs(P, P.ctl);
+    dense_t_tiled(D, v.xin);
+}
+
+__global__ __launch_bounds__(256 * kGHalves) void rk_dense_stage_tiled_kernel(EpiArgs P,
+                                                                             DenseOp D) {
+    __shared__ GemmShared G;
+    __shared__ double smem[256 * kGHalves / QA_WAVE];
+    RKCtl *ctl = P.ctl;
+    if (ctl->phase != P.run_phase) return;
+    const StageVecs v = stage_vecs(P, ctl);
+    const int h = threadIdx.x >> 8, lane = threadIdx.x & 63, w = (threadIdx.x & 255) >> 6;
+    const int i0 = blockIdx.x * kGT + 16 * (w & 1), j0 = blockIdx.y * kGT + 16 * (w >> 1);
+    double2 o[4], er[4];
+    if (h == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) epi_pre(P, v, tile_index(D.N, i0, j0, lane, q), o[q], er[q]);
+    }
+    v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
+    dense_k_tiled(D, G, re, im);
+    double part = 0.0;
+    if (h == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            part += epi_post(P, ctl, v, tile_index(D.N, i0, j0, lane, q),
+                             make_double2((double)re[q], (double)im[q]), o[q], er[q]);
+    }
+    if (P.epi == EPI_FINAL) {
+        const double sum = block_sum(part, smem);
+        if (threadIdx.x == 0) P.partials[blockIdx.y * gridDim.x + blockIdx.x] = sum;
+    }
+}
+
+// Ungated tiled versions (qa_rk_apply and the first-step estimate).
+__global__ __launch_bounds__(256 * kGHalves) void dense_t_tiled_kernel(DenseOp D,
+                                                                      const double2 *X) {
+    dense_t_tiled(D, X);
+}
+
+__global__ __launch_bounds__(256 * kGHalves) void dense_rhs_tiled_kernel(DenseOp D,
+                                                                        double2 *out) {
+    __shared__ GemmShared G;
+    v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
+    dense_k_tiled(D, G, re, im);
+    if ((threadIdx.x >> 8) != 0) return;
+    const int lane = threadIdx.x & 63, w = (threadIdx.x & 255) >> 6;
+    const int i0 = blockIdx.x * kGT + 16 * (w & 1), j0 = blockIdx.y * kGT + 16 * (w >> 1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        out[tile_index(D.N, i0, j0, lane, q)] = make_double2((double)re[q], (double)im[q]);
 }
 
 // Ungated versions: out = L(X) (the data-layer product of the matrix form).
@@ -1002,6 +1283,16 @@ int launch_sparse_stage(qa_rk_solver *S, int i) {
 int launch_dense_stage(qa_rk_solver *S, int i) {
     EpiArgs P;
     fill_epi(P, S, i);
+    if (S->dop.tiled) {
+        const int nt = S->dop.N / kGT;
+        hipLaunchKernelGGL(rk_dense_t_tiled_kernel, dim3(nt, nt, std::max(S->dop.nc, 1)),
+                           dim3(256 * kGHalves), 0, S->stream, P, S->dop);
+        QA_CHECK_LAUNCH();
+        hipLaunchKernelGGL(rk_dense_stage_tiled_kernel, dim3(nt, nt), dim3(256 * kGHalves), 0,
+                           S->stream, P, S->dop);
+        QA_CHECK_LAUNCH();
+        return QA_OK;
+    }
     const int nt = S->dop.N / 16;
     const dim3 block(64 * S->dop.nsplit);
     if (S->dop.nc > 0) {
@@ -1021,6 +1312,16 @@ int launch_stage(qa_rk_solver *S, int i) {
 
 // out = A x (ungated): the derivative for the first-step estimate.
 int engine_rhs(qa_rk_solver *S, const double2 *x, double2 *out) {
+    if (S->kind == 1 && S->dop.tiled) {
+        const int nt = S->dop.N / kGT;
+        hipLaunchKernelGGL(dense_t_tiled_kernel, dim3(nt, nt, std::max(S->dop.nc, 1)),
+                           dim3(256 * kGHalves), 0, S->stream, S->dop, x);
+        QA_CHECK_LAUNCH();
+        hipLaunchKernelGGL(dense_rhs_tiled_kernel, dim3(nt, nt), dim3(256 * kGHalves), 0,
+                           S->stream, S->dop, out);
+        QA_CHECK_LAUNCH();
+        return QA_OK;
+    }
     if (S->kind == 1) {
         const int nt = S->dop.N / 16;
         const dim3 block(64 * S->dop.nsplit);
@@ -1335,7 +1636,7 @@ int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, co
             }
     hipError_t he = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
     if (he == hipSuccess)
-        he = hipMalloc(&S->dense_block, sizeof(float2) * ((2 + 3 * (size_t)nc) * NN + 1));
+        he = hipMalloc(&S->dense_block, sizeof(float2) * ((3 + 3 * (size_t)nc) * NN + 1));
     if (he == hipSuccess)
         he = hipMemcpy(S->dense_block, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice);
     if (he != hipSuccess) {
@@ -1360,6 +1661,9 @@ int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, co
     S->dop.C = d + 2 * NN;
     S->dop.Cd = d + (2 + (size_t)nc) * NN;
     S->dop.T = d + (2 + 2 * (size_t)nc) * NN;
+    S->dop.Xf = d + (2 + 3 * (size_t)nc) * NN;
+    S->dop.tiled = (N >= kTiledN && N % kGK == 0 && N % kGT == 0) ? 1 : 0;
+    if (S->dop.tiled) S->nparts = (N / kGT) * (N / kGT);
     rc = rk_finish_create(S, order, rk_step, rk_extra, dorder, a, b, c, e, bi, rtol, atol, nsteps,
                           first_step, min_step, max_step, interpolate);
     if (rc) return rc;

@@ -23,9 +23,10 @@ def _dense_rhs(H, cs, rho):
     return out
 
 
-@pytest.mark.parametrize("N", [16, 32, 64, 128])
+@pytest.mark.parametrize("N", [16, 32, 64, 128, 384, 512])
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_matrix_form_rhs_vs_numpy(N, prec):
+    # N >= 384 runs the 32 x 32 LDS-tiled kernels, smaller N the one-tile kernels
     if prec == "bf16x3" and N % 32:
         pytest.skip("bf16x3 needs N % 32 == 0")
     print(f"dense rhs N={N} prec={prec}", flush=True)
@@ -73,3 +74,21 @@ def test_matrix_form_matches_superoperator_small():
     err = np.abs(np.asarray(got.expect[0]) - np.asarray(ref.expect[0])).max()
     print(f"matrix form vs superoperator: {err:.2e}", flush=True)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_matrix_form_decay_closed_form_large(prec):
+    """The whole tiled stage path (N = 384: 32 x 32 LDS tiles) on a solve with
+    a closed form: H = 0, c = sqrt(g) a, rho0 = |5><5| gives
+    <a^dag a>(t) = 5 exp(-g t) (qutip/tests/solver/test_mesolve.py:84-95
+    uses the same decay)."""
+    N, g = 384, 0.4
+    a = qa.destroy(N)
+    H = qa.Qobj(np.zeros((N, N), dtype=complex))
+    tl = np.linspace(0, 4, 9)
+    res = qa.mesolve(H, qa.basis(N, 5), tl, [np.sqrt(g) * a], e_ops=[a.dag() @ a],
+                     options={"matrix_form": True, "dense_precision": prec, "rtol": 1e-8,
+                              "atol": 1e-10})
+    err = np.abs(np.asarray(res.expect[0]) - 5 * np.exp(-g * tl)).max()
+    print(f"decay N={N} prec={prec}: max err {err:.2e}", flush=True)
+    assert err < (1e-5 if prec == "fp32" else 5e-4)
