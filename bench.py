@@ -332,6 +332,30 @@ def pmc_workload():
         f"lanes 1, 2, 4, 8 (nnz {L.nnz})")
 
 
+def dense_rhs_probe(N, nc, prec="fp32"):
+    """Time the matrix-form Lindblad RHS alone (stage kernels in solve order,
+    qa_rk_profile_sequence) for a random N x N system with nc dense collapse
+    operators (seeded); prints us per RHS and TFLOP/s."""
+    from qutip_amd import _lib
+    from qutip_amd.solver.hip_rk import RKEngine
+    rng = np.random.default_rng(0)
+    H = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    H = H + H.conj().T
+    cs = [0.1 * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))
+          for _ in range(nc)]
+    Hnh = H - 0.5j * sum(c.conj().T @ c for c in cs)
+    eng = RKEngine(None, "vern7", 1e-6, 1e-8, dense=(Hnh, cs), dense_precision=prec)
+    eng.set_state(0.0, (np.eye(N, dtype=complex) / N).ravel(order="F"))
+    ts, td = C.c_double(), C.c_double()
+    _lib.call("qa_rk_profile_sequence", eng.handle, 10, C.byref(ts), C.byref(td))
+    us = ts.value * 1e3
+    flop = (2 + 2 * nc) * 8 * N ** 3
+    print(json.dumps({"dense_rhs_probe": {"N": N, "nc": nc, "precision": prec,
+                                          "us_per_rhs": round(us, 2),
+                                          "tflops": round(flop / (us * 1e-6) / 1e12, 2)}}),
+          flush=True)
+
+
 def read_traffic(lanes):
     """HBM bytes per stage-kernel launch from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE scaled by the copy calibration) when it was
@@ -364,9 +388,14 @@ def main():
     ap.add_argument("--dense-reps", type=int, default=2)
     ap.add_argument("--pmc", action="store_true",
                     help="only run the launches the HBM-traffic counters are read on")
+    ap.add_argument("--dense-rhs-probe", type=int, nargs=2, metavar=("N", "NC"),
+                    help="only time the matrix-form RHS at size N with NC collapse operators")
     args = ap.parse_args()
     if args.pmc:
         pmc_workload()
+        return
+    if args.dense_rhs_probe:
+        dense_rhs_probe(*args.dense_rhs_probe)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the "
