@@ -13,7 +13,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 INCLUDE = HERE.parent / "include"
 LIB = HERE / "libqutip_amd.so"
-SOURCES = ["runtime.hip", "linalg.hip", "rk.hip", "mc.hip", "dense.hip"]
+SOURCES = ["runtime.hip", "linalg.hip", "rk.hip", "mc.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -32,7 +32,10 @@ def hipcc():
 
 
 def sources():
-    return [CSRC / s for s in SOURCES if (CSRC / s).exists()]
+    missing = [s for s in SOURCES if not (CSRC / s).exists()]
+    if missing:
+        raise RuntimeError(f"missing HIP sources: {missing}")
+    return [CSRC / s for s in SOURCES]
 
 
 def build(verbose=False, jobs=None):
