@@ -264,7 +264,7 @@ def bench_dense(qa, systems, log, reps=2, tf=None):
     return out
 
 
-def bench_serving(systems, log, nreq=400, warm=40):
+def bench_serving(systems, log, nreq=400, warm=40, policy="fifo"):
     """The serving mix: Poisson arrivals at 200 req/s (seed 7), d uniform in
     {16..512}, 40% sesolve / 40% mesolve / 20% mcsolve, admitted by HBM
     budget and slots, overlapped on per-request streams.  Latency =
@@ -276,7 +276,7 @@ def bench_serving(systems, log, nreq=400, warm=40):
     tbuild = time.perf_counter() - t0
     # warm the kernels and the autotuner cache on a short prefix
     SolveServer(max_slots=64).run(systems.serving_trace(12), realtime=False)
-    srv = SolveServer(max_slots=64)
+    srv = SolveServer(max_slots=64, policy=policy)
     t0 = time.perf_counter()
     done = srv.run(reqs, realtime=True)
     wall = time.perf_counter() - t0
@@ -286,6 +286,7 @@ def bench_serving(systems, log, nreq=400, warm=40):
     kinds = {k: sum(1 for r in done if r.kind == k) for k in ("se", "me", "mc")}
     out = {"workload": f"{nreq} requests, Poisson 200 req/s seed 7, d in {{16..512}}, "
                        f"40/40/20 se/me/mc, t_final 10, 50 outputs",
+           "admission": policy,
            "solves_per_s": round(len(done) / wall, 2), "wall_s": round(wall, 3),
            "arrival_span_s": round(reqs[-1].arrival, 3),
            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
@@ -495,6 +496,8 @@ def main():
         out["dense_kerr"] = bench_dense(qa, systems, log, reps=args.dense_reps, tf=args.dense_tf)
     if not args.no_serve:
         out["serving"] = bench_serving(systems, log, nreq=args.serve_requests)
+        out["serving_shortest_first"] = bench_serving(systems, log, nreq=args.serve_requests,
+                                                      policy="sjf")
     if not args.no_plain:
         psolver = qa.MESolver(s["H"], s["c_ops"], options={**opts, "fused": False})
         t0 = time.perf_counter()

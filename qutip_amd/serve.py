@@ -81,6 +81,16 @@ class SolveRequest:
     def latency(self):
         return None if self.t_done is None else self.t_done - self.arrival
 
+    def work_estimate(self):
+        """Relative device work of the request, for shortest-first
+        admission: flops of one RHS times the number of states it carries."""
+        d = self.d
+        if self.kind == "se":
+            return 8.0 * self.H.to_csr().nnz
+        if self.kind == "me":
+            return (2 + 2 * len(self.c_ops)) * 8.0 * d ** 3
+        return 8.0 * d * d * max(self.ntraj, 1)
+
 
 class _Running:
     """Device-side state of one admitted request."""
@@ -247,7 +257,14 @@ class SolveServer:
     """Admit by HBM budget and slot count, overlap in-flight requests on
     their own streams, retire and admit continuously."""
 
-    def __init__(self, hbm_budget=64 << 30, max_slots=32):
+    def __init__(self, hbm_budget=64 << 30, max_slots=32, policy="fifo"):
+        """policy: 'fifo' admits arrived requests in arrival order; 'sjf'
+        admits the arrived request with the least estimated work first
+        (lower median latency under overload, longer waits for the
+        largest requests)."""
+        if policy not in ("fifo", "sjf"):
+            raise ValueError("policy must be 'fifo' or 'sjf'")
+        self.policy = policy
         self.hbm_budget = int(hbm_budget)
         self.max_slots = int(max_slots)
         self.pending = []
@@ -262,18 +279,29 @@ class SolveServer:
     def _admit(self, now):
         admitted = 0
         keep = []
-        for req in self.pending:
+        order = self.pending
+        if self.policy == "sjf":
+            arrived = [r for r in self.pending if r.arrival <= now]
+            later = [r for r in self.pending if r.arrival > now]
+            order = sorted(arrived, key=lambda r: (r.work_estimate(), r.arrival)) + later
+        for req in order:
             need = req.device_bytes()
+            if req.arrival <= now and need > self.hbm_budget:
+                # can never fit: fail it rather than wait forever
+                req.error = f"needs {need} bytes, above the HBM budget {self.hbm_budget}"
+                req.t_admit = req.t_done = self._clock()
+                self.finished.append(req)
+                continue
             if (req.arrival <= now and len(self.running) < self.max_slots
                     and self.in_use + need <= self.hbm_budget):
-                req.t_admit = now
+                req.t_admit = self._clock()
                 req._bytes = need
                 self.in_use += need
                 try:
                     self.running.append(_Running(req))
                 except Exception as e:  # a bad request fails alone
                     req.error = repr(e)
-                    req.t_done = now
+                    req.t_done = self._clock()
                     self.in_use -= need
                     self.finished.append(req)
                 admitted += 1
@@ -297,6 +325,7 @@ class SolveServer:
             self.submit(r)
         self.pending.sort(key=lambda r: r.arrival)
         t0 = time.perf_counter()
+        self._clock = lambda: time.perf_counter() - t0
         now = lambda: (time.perf_counter() - t0) if realtime else float("inf")
         with _data.deferred_release():
             while self.pending or self.running:
@@ -315,7 +344,7 @@ class SolveServer:
                 if count > self.release_count or nbytes > self.release_bytes:
                     _data.release_deferred()
                 if not self.running and self.pending and realtime:
-                    wait = self.pending[0].arrival - now()
+                    wait = min(r.arrival for r in self.pending) - now()
                     if wait > 0:
                         time.sleep(min(wait, 0.01))
         return self.finished

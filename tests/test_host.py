@@ -132,3 +132,25 @@ def test_ell_layout_reproduces_csr(d, density):
     assert_allclose(y, A @ x, atol=1e-13)
     _, _, val32 = _ell_host(A, d, True)
     assert val32.dtype == np.complex64
+
+
+def test_server_fails_a_request_above_the_budget():
+    """A request that can never fit the HBM budget fails at once instead of
+    waiting forever (no device work is started)."""
+    from qutip_amd.serve import SolveRequest, SolveServer
+    req = SolveRequest("se", qa.sigmax(), qa.basis(2, 0), [0.0, 1.0], e_ops=[qa.sigmaz()])
+    done = SolveServer(hbm_budget=1).run([req], realtime=False)
+    assert done[0].error and "HBM budget" in done[0].error
+    assert done[0].latency() is not None and np.isfinite(done[0].latency())
+
+
+def test_shortest_work_first_estimates():
+    from qutip_amd.serve import SolveServer
+    reqs = systems.serving_trace(30)
+    w = {r.id: r.work_estimate() for r in reqs}
+    me512 = [r for r in reqs if r.kind == "me" and r.d == 512]
+    se16 = [r for r in reqs if r.kind == "se" and r.d == 16]
+    if me512 and se16:
+        assert w[me512[0].id] > w[se16[0].id]
+    with pytest.raises(ValueError):
+        SolveServer(policy="lifo")
