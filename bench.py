@@ -35,15 +35,20 @@ def log(*a):
 
 
 def stage_bytes(engine, nnz, n, fp32):
-    """Algorithmic HBM bytes of each stage-kernel launch: operator values and
-    columns once, the input vector once, k_i written, plus the epilogue's
-    reads of y_prev and of every k_j with a non-zero coefficient and its
-    write of the next stage input (or y_front)."""
+    """Algorithmic HBM bytes of each stage-kernel launch: the operator once
+    (plain: a value and a column per non-zero; coded: a 2-byte code per
+    non-zero and the two 256-entry tables), the input vector once, k_i
+    written, plus the epilogue's reads of y_prev and of every k_j with a
+    non-zero coefficient and its write of the next stage input (or
+    y_front)."""
     tab = engine.tableau
     a, b, e = tab["a"], tab["b"], tab["e"]
     s, se = len(b), len(tab["c"])
-    vb = 8 if fp32 else 16
-    base = nnz * (vb + 4) + (engine.sell.nslices + 1) * 4 + n * 16 + n * 16
+    if engine.coded:
+        op = nnz * 2 + 256 * (16 + 4)
+    else:
+        op = nnz * ((8 if fp32 else 16) + 4)
+    base = op + (engine.sell.nslices + 1) * 4 + n * 16 + n * 16
     out = []
     for i in range(se):
         if i < s - 1 or (s <= i < se - 1):
@@ -306,9 +311,10 @@ def pmc_workload():
     """The launches the HBM-traffic counters are read on (run under
     rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE): three device-to-device
     copies of 64 MiB (the calibration: a plain copy reads and writes exactly
-    that many bytes), then, for each SELL shape (1, 2, 4, 8 lanes per row),
-    3 launches of each of the 16 stage kernels of the L=8 Ising solve in
-    solve order (direct launches, no graph)."""
+    that many bytes), then, for the plain and then the coded operator form
+    and each SELL shape (1, 2, 4, 8 lanes per row), 3 launches of each of
+    the 16 stage kernels of the L=8 Ising solve in solve order (direct
+    launches, no graph)."""
     from qutip_amd import _lib, data as qd
     from qutip_amd.solver.hip_rk import RKEngine
     from tests import systems
@@ -324,13 +330,15 @@ def pmc_workload():
     rho0 = np.zeros(L.shape[0], dtype=np.complex128)
     rho0[0] = 1.0
     ms = np.zeros(16)
-    for lanes in (1, 2, 4, 8):
-        eng = RKEngine(L, "vern7", 1e-8, 1e-10, lanes=lanes)
-        eng.set_state(0.0, qd.HipDense.from_array(rho0))
-        _lib.call("qa_rk_profile_stages", eng.handle, 3, ms.ctypes.data_as(C.POINTER(C.c_double)))
-        _lib.call("qa_device_sync")
+    for coded in (False, True):
+        for lanes in (1, 2, 4, 8):
+            eng = RKEngine(L, "vern7", 1e-8, 1e-10, lanes=lanes, coded=coded)
+            eng.set_state(0.0, qd.HipDense.from_array(rho0))
+            _lib.call("qa_rk_profile_stages", eng.handle, 3,
+                      ms.ctypes.data_as(C.POINTER(C.c_double)))
+            _lib.call("qa_device_sync")
     log(f"pmc workload: 3 copies of {COPY_BYTES} B, then 16 x 3 stage launches for each of "
-        f"lanes 1, 2, 4, 8 (nnz {L.nnz})")
+        f"lanes 1, 2, 4, 8, plain then coded (nnz {L.nnz})")
 
 
 def dense_rhs_probe(N, nc, prec="fp32"):
@@ -357,10 +365,44 @@ def dense_rhs_probe(N, nc, prec="fp32"):
           flush=True)
 
 
-def read_traffic(lanes):
+def stage_roofline(RKEngine, L, stats, lanes, coded, reps=20):
+    """Roofline entry of the stage kernel: HIP events on the engine's stream
+    around back-to-back rounds of the 16 stage launches in solve order
+    (qa_rk_profile_sequence), weighted by the solve's step and dense-output
+    stage counts; algorithmic bytes from stage_bytes."""
+    n, nnz = L.shape[0], L.nnz
+    eng = RKEngine(L, "vern7", 1e-8, 1e-10, lanes=lanes, coded=coded)
+    rho0 = np.zeros(n, dtype=np.complex128)
+    rho0[0] = 1.0
+    eng.set_state(0.0, rho0)
+    t_step, t_dense = C.c_double(), C.c_double()
+    _lib_call("qa_rk_profile_sequence", eng.handle, reps, C.byref(t_step), C.byref(t_dense))
+    byts = stage_bytes(eng, nnz, n, False)
+    n_step = 10 * stats["attempts"]
+    n_dense = 6 * stats["dense_preps"]
+    avg_ms = (n_step * t_step.value + n_dense * t_dense.value) / (n_step + n_dense)
+    avg_bytes = float((stats["attempts"] * byts[:10].sum() + stats["dense_preps"] * byts[10:].sum())
+                      / (n_step + n_dense))
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    log(f"stage kernel ({'coded' if coded else 'plain'}, lanes {lanes}) avg {avg_ms * 1e3:.2f} us "
+        f"(step stages {t_step.value * 1e3:.2f}, dense {t_dense.value * 1e3:.2f}), "
+        f"{avg_bytes / 1e6:.2f} MB -> {achieved:.0f} GB/s")
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(lanes, coded),
+            "kernel": f"rk_stage_kernel<double2, {lanes}, {'true' if coded else 'false'}>",
+            "avg_launch_us": round(avg_ms * 1e3, 3),
+            "algorithmic_bytes_per_launch": int(avg_bytes)}
+
+
+def _lib_call(*a):
+    from qutip_amd import _lib
+    _lib.call(*a)
+
+
+def read_traffic(lanes, coded=False):
     """HBM bytes per stage-kernel launch from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE scaled by the copy calibration) when it was
-    taken on the same SELL shape, else None."""
+    taken on the same operator form and SELL shape, else None."""
     vals = {}
     try:
         for line in TRAFFIC_FILE.read_text().splitlines():
@@ -369,7 +411,7 @@ def read_traffic(lanes):
                 vals[k.strip()] = v.strip()
     except OSError:
         return None
-    key = f"traffic_bytes_per_launch_lanes{lanes}"
+    key = f"traffic_bytes_per_launch_{'coded_' if coded else ''}lanes{lanes}"
     return int(float(vals[key])) if key in vals else None
 
 
@@ -436,24 +478,9 @@ def main():
     value = args.steps / total
     log(f"fused: {value:.2f} solves/s, err {err:.2e}, stats {stats}")
 
-    # roofline of the stage kernel: HIP events on the engine's stream around
-    # back-to-back rounds of the stage launches in solve order
     lanes = solver._integrator._engine.lanes
-    eng = RKEngine(L, "vern7", 1e-8, 1e-10, lanes=lanes)
-    rho0 = np.zeros(n, dtype=np.complex128)
-    rho0[0] = 1.0
-    eng.set_state(0.0, rho0)
-    t_step, t_dense = C.c_double(), C.c_double()
-    _lib.call("qa_rk_profile_sequence", eng.handle, 20, C.byref(t_step), C.byref(t_dense))
-    byts = stage_bytes(eng, nnz, n, False)
-    n_step = 10 * stats["attempts"]
-    n_dense = 6 * stats["dense_preps"]
-    avg_ms = (n_step * t_step.value + n_dense * t_dense.value) / (n_step + n_dense)
-    avg_bytes = float((stats["attempts"] * byts[:10].sum() + stats["dense_preps"] * byts[10:].sum())
-                      / (n_step + n_dense))
-    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    log(f"stage kernel avg {avg_ms * 1e3:.2f} us (step stages {t_step.value * 1e3:.2f}, dense "
-        f"{t_dense.value * 1e3:.2f}), {avg_bytes / 1e6:.2f} MB -> {achieved:.0f} GB/s")
+    coded = solver._integrator._engine.coded
+    roof = stage_roofline(RKEngine, L, stats, lanes, coded)
 
     out = {
         "metric": METRIC, "value": round(value, 4), "unit": "solves/s", "n_gpus": 1,
@@ -467,17 +494,35 @@ def main():
                         "2 e_ops; fused hip_rk (device step control, hipGraph replay)",
             "n": n, "nnz": nnz, "rk_attempts": stats["attempts"],
             "rhs_evals": stats["rhs_evals"], "sell_lanes_per_row": lanes,
+            "operator_format": "coded SELL-C" if coded else "SELL-C",
             "autotune": autotune.cache()},
         "err_vs_fp64": float(err),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic(lanes), "kernel": f"rk_stage_kernel<double2, {lanes}>",
-                     "avg_launch_us": round(avg_ms * 1e3, 3),
-                     "algorithmic_bytes_per_launch": int(avg_bytes)},
+        "roofline": roof,
     }
-    # the chosen precision form: complex64 operator values, FP64 state,
-    # accumulation, error norm and expectation values
-    fsolver = qa.MESolver(s["H"], s["c_ops"], options={**opts, "operator_precision": "fp32"})
+    # the plain SELL-C operator (a value and a column per entry) on the same
+    # solve: the same result bits, ten times the operator bytes
+    ssolver = qa.MESolver(s["H"], s["c_ops"], options={**opts, "operator_format": "sell"})
+    sres = ssolver.run(s["psi0"], tlist, e_ops=s["e_ops"])
+    _lib.call("qa_device_sync")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sres = ssolver.run(s["psi0"], tlist, e_ops=s["e_ops"])
+    _lib.call("qa_device_sync")
+    tpl = (time.perf_counter() - t0) / args.steps
+    planes = ssolver._integrator._engine.lanes
+    diff = max(float(np.abs(np.asarray(sres.expect[k]) - np.asarray(res.expect[k])).max())
+               for k in range(2))
+    # the same bits hold for the same lane count (the same summation order)
+    same = (diff == 0.0) if planes == lanes else None
+    out["sell_plain"] = {"solves_per_s": round(1 / tpl, 4), "sell_lanes_per_row": planes,
+                         "same_bits_as_coded": same, "max_abs_diff_vs_coded": diff,
+                         "roofline": stage_roofline(RKEngine, L, stats, planes, False)}
+    log(f"plain SELL-C operator: {1 / tpl:.2f} solves/s (lanes {planes}), max |diff| vs coded "
+        f"{diff:.1e}")
+    # the chosen precision form of the plain SELL-C operator: complex64
+    # values, FP64 state, accumulation, error norm and expectation values
+    fsolver = qa.MESolver(s["H"], s["c_ops"], options={**opts, "operator_precision": "fp32",
+                                                       "operator_format": "sell"})
     fres = fsolver.run(s["psi0"], tlist, e_ops=s["e_ops"])
     _lib.call("qa_device_sync")
     t0 = time.perf_counter()

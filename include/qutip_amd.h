@@ -95,6 +95,23 @@ int qa_sell_matvec_z(int nrows, int lanes, int nslices, const int *slice_ptr, co
                      const void *vals, int vals_fp32, const void *x, void *out, double scale_re,
                      double scale_im, int accumulate, void *stream);
 
+/* The same product with a coded SELL-C operator: every stored entry is a
+ * 16-bit code, low `vbits` bits (6..10) = index into `vtab` (2^vbits
+ * complex128 values), high 16 - vbits bits = index into `otab`
+ * (2^(16 - vbits) int32 column offsets; column = row + offset, clamped to
+ * [0, ncols - 1]).  Four consecutive entries of a lane are one
+ * 8-byte word: word slice_ptr[s] + 64 w + lane holds the lane's steps
+ * 4w .. 4w + 3 (slice widths are multiples of 4 lanes; slice_ptr counts
+ * words).  Same lane -> entry assignment and summation order as
+ * qa_sell_matvec_z, hence the same result bits for the same operator.
+ * Replaces the CSR product of matmul_csr_dense_dense (matmul.pyx:255-265)
+ * for operators whose distinct values and distinct diagonals fit such a
+ * split (up to 1024 values with 64 diagonals, 256 with 256, 64 with 1024). */
+int qa_sell_coded_matvec_z(int nrows, int ncols, int lanes, int nslices, const int *slice_ptr,
+                           const void *codes, const void *vtab, const int *otab, int vbits,
+                           const void *x, void *out, double scale_re, double scale_im,
+                           int accumulate, void *stream);
+
 /* out = base + sum_j coefs[j] * vecs[j] (real coefficients, nterms <= 16,
  * base may be NULL).  Replaces the copy + zaxpy chain of
  * Explicit_RungeKutta._accumulate (qutip/solver/integrator/explicit_rk.pyx:
@@ -156,6 +173,18 @@ int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *s
                  int dorder, const double *a, const double *b, const double *c,
                  const double *e, const double *bi, double rtol, double atol, int nsteps,
                  double first_step, double min_step, double max_step, int interpolate);
+/* The same engine over a coded SELL-C operator (slice_ptr, codes, vtab,
+ * otab, vbits as for qa_sell_coded_matvec_z).  The stage products are the terms of the plain
+ * SELL-C form summed in the same order, so a solve gives the same bits as
+ * qa_rk_create on the same operator, with 2 bytes per stored entry read
+ * instead of 20.  Tableau and options as for qa_rk_create
+ * (explicit_rk.pyx:117-151, qutip_integrator.py:52-60). */
+int qa_rk_create_coded(qa_rk_solver **out, int n, int lanes, int nslices, const int *slice_ptr,
+                       const void *codes, const void *vtab, const int *otab, int vbits,
+                       int order, int rk_step, int rk_extra, int dorder, const double *a,
+                       const double *b, const double *c, const double *e, const double *bi,
+                       double rtol, double atol, int nsteps, double first_step,
+                       double min_step, double max_step, int interpolate);
 /* The same engine for the Lindblad equation in matrix form (mesolve with
  * options {"matrix_form": True}: qutip/core/cy/lindblad_matrix_form.pyx:
  * 105-203): the state is rho, N x N column-major (N a multiple of 16), and

@@ -52,6 +52,8 @@ SIGNATURES = {
     "qa_csr_matvec_z": [i32, i32, vp, vp, vp, i32, i32, vp, vp, f64, f64, i32, vp],
     "qa_csr_matmat_z": [i32, i32, vp, vp, vp, i32, i32, vp, i64, vp, i64, f64, f64, i32, vp],
     "qa_sell_matvec_z": [i32, i32, i32, vp, vp, vp, i32, vp, vp, f64, f64, i32, vp],
+    "qa_sell_coded_matvec_z": [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp, f64, f64, i32,
+                               vp],
     "qa_lincomb_z": [i64, vp, i32, C.POINTER(vp), P_f64, vp, vp],
     "qa_axpby_z": [i64, f64, f64, vp, f64, f64, vp, vp, vp],
     "qa_wrmn_error_z": [i64, vp, vp, f64, f64, P_f64, vp],
@@ -62,6 +64,8 @@ SIGNATURES = {
     "qa_expect_csr_dm_z": [i32, vp, vp, vp, i32, vp, P_f64, vp],
     "qa_rk_create": [C.POINTER(vp), i32, i32, i32, vp, vp, vp, i32, i32, i32, i32, i32,
                      P_f64, P_f64, P_f64, P_f64, P_f64, f64, f64, i32, f64, f64, f64, i32],
+    "qa_rk_create_coded": [C.POINTER(vp), i32, i32, i32, vp, vp, vp, vp, i32, i32, i32, i32, i32,
+                           P_f64, P_f64, P_f64, P_f64, P_f64, f64, f64, i32, f64, f64, f64, i32],
     "qa_rk_create_dense": [C.POINTER(vp), i32, i32, P_f64, P_f64, i32, i32, i32, i32, i32,
                            P_f64, P_f64, P_f64, P_f64, P_f64, f64, f64, i32, f64, f64, f64, i32],
     "qa_rk_apply": [vp, vp, vp],

@@ -20,8 +20,9 @@ CANDIDATES = (1, 2, 4, 8)
 MIN_TUNE_ROWS = 4096  # below this the stage kernels are launch-bound
 
 
-def _key(n, nnz_row, fp32, batch=1):
-    return f"n{int(math.log2(max(n, 1)))}-r{int(round(nnz_row))}-{'c64' if fp32 else 'c128'}-b{batch}"
+def _key(n, nnz_row, fp32, batch=1, coded=False):
+    k = f"n{int(math.log2(max(n, 1)))}-r{int(round(nnz_row))}-{'c64' if fp32 else 'c128'}-b{batch}"
+    return k + "-coded" if coded else k
 
 
 def _load_file():
@@ -52,7 +53,7 @@ def heuristic_lanes(nnz_row):
     return L
 
 
-def measure(A, fp32=False, candidates=CANDIDATES, reps=8, rounds=4):
+def measure(A, fp32=False, candidates=CANDIDATES, reps=8, rounds=4, coded=False):
     """Average stage-kernel time (ms) per candidate lane count.  All candidate
     engines are built first, one untimed round warms the device, then the
     candidates are timed in interleaved rounds and the fastest round counts
@@ -63,7 +64,8 @@ def measure(A, fp32=False, candidates=CANDIDATES, reps=8, rounds=4):
 
     from . import _lib
     from .solver.hip_rk import RKEngine
-    engines = {L: RKEngine(A, "vern7", 1e-6, 1e-8, fp32=fp32, lanes=L) for L in candidates}
+    engines = {L: RKEngine(A, "vern7", 1e-6, 1e-8, fp32=fp32, lanes=L, coded=coded)
+               for L in candidates}
     best = {L: float("inf") for L in candidates}
     t_step, t_dense = C.c_double(), C.c_double()
     for rnd in range(rounds + 1):
@@ -75,16 +77,16 @@ def measure(A, fp32=False, candidates=CANDIDATES, reps=8, rounds=4):
     return best
 
 
-def lanes_for(A, fp32=False):
+def lanes_for(A, fp32=False, coded=False):
     n = A.shape[0]
     nnz_row = A.nnz / max(n, 1)
     if n < MIN_TUNE_ROWS or os.environ.get("QUTIP_AMD_NO_TUNE"):
         return heuristic_lanes(nnz_row)
     if not _CACHE:
         _load_file()
-    k = _key(n, nnz_row, fp32)
+    k = _key(n, nnz_row, fp32, coded=coded)
     if k not in _CACHE:
-        t = measure(A, fp32)
+        t = measure(A, fp32, coded=coded)
         _CACHE[k] = {"lanes": min(t, key=t.get), "ms": {str(L): v for L, v in t.items()}}
         _save_file()
     return int(_CACHE[k]["lanes"])

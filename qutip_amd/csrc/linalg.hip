@@ -88,6 +88,35 @@ __global__ __launch_bounds__(256) void sell_matvec_kernel(int nrows, int nslices
     }
 }
 
+// The same product over a coded SELL-C operator (value and column-offset
+// tables, 16-bit codes; sell_coded_row_dot).  Every workgroup (256 threads)
+// stages the two tables in LDS first.
+template <int L>
+__global__ __launch_bounds__(256) void sell_coded_matvec_kernel(
+    int nrows, int ncols, int nslices, const int *__restrict__ slice_ptr,
+    const uint2 *__restrict__ code, const double2 *__restrict__ vtab, const int *__restrict__ otab,
+    int vbits, const double2 *__restrict__ x, double2 *__restrict__ out, double2 scale,
+    int accumulate) {
+    __shared__ double2 vt[kCodeTabMax];
+    __shared__ int ot[kCodeTabMax];
+    code_tables_to_lds(vtab, otab, vbits, vt, ot);
+    __syncthreads();
+    const CodeSplit cs = {(1u << vbits) - 1u, vbits};
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = (int)(t / QA_WAVE), lane = (int)(t % QA_WAVE);
+    if (slice >= nslices) return;
+    int base, words;
+    sell_slice(slice_ptr, slice, base, words);
+    const int64_t r = (int64_t)slice * (QA_WAVE / L) + lane / L;
+    const double2 acc =
+        sell_coded_row_dot<L>(code, vt, ot, cs, x, base, words, lane, (int)r, ncols - 1);
+    if (lane % L == 0 && r < nrows) {
+        double2 o = cmul(scale, acc);
+        if (accumulate) o = cadd(o, out[r]);
+        out[r] = o;
+    }
+}
+
 // out = base + sum_j coef[j] * vecs[j]   (real coefficients, up to 16 terms)
 struct LinComb {
     const double2 *v[16];
@@ -363,6 +392,49 @@ int qa_csr_matmat_z(int nrows, int ncols, const int *rowptr, const int *col, con
         hipLaunchKernelGGL(csr_matmat_kernel<double2>, grid, dim3(256), 0, s, nrows, rowptr, col,
                            (const double2 *)vals, nvec, (const double2 *)x, ldx, (double2 *)out,
                            ldo, sc, accumulate);
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
+int qa_sell_coded_matvec_z(int nrows, int ncols, int lanes, int nslices, const int *slice_ptr,
+                           const void *codes, const void *vtab, const int *otab, int vbits,
+                           const void *x, void *out, double scale_re, double scale_im,
+                           int accumulate, void *stream) {
+    QA_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8,
+               "qa_sell_coded_matvec_z: lanes must be 1, 2, 4 or 8");
+    const int C = QA_WAVE / lanes;
+    QA_REQUIRE(nrows >= 0 && nslices == (nrows + C - 1) / C,
+               "qa_sell_coded_matvec_z: nslices must be ceil(nrows / (64 / lanes))");
+    QA_REQUIRE(ncols > 0 || nrows == 0, "qa_sell_coded_matvec_z: ncols must be positive");
+    QA_REQUIRE(vbits >= 6 && vbits <= 10, "qa_sell_coded_matvec_z: vbits must be 6..10");
+    if (nrows == 0) return QA_OK;
+    QA_REQUIRE(slice_ptr && codes && vtab && otab && x && out,
+               "qa_sell_coded_matvec_z: null pointer");
+    const unsigned blocks = (unsigned)(((int64_t)nslices * QA_WAVE + 255) / 256);
+    const double2 sc = make_double2(scale_re, scale_im);
+    hipStream_t s = (hipStream_t)stream;
+    const uint2 *cw = (const uint2 *)codes;
+    const double2 *vt = (const double2 *)vtab;
+    const double2 *xx = (const double2 *)x;
+    double2 *o = (double2 *)out;
+    switch (lanes) {
+        case 1:
+            hipLaunchKernelGGL(sell_coded_matvec_kernel<1>, dim3(blocks), dim3(256), 0, s, nrows,
+                               ncols, nslices, slice_ptr, cw, vt, otab, vbits, xx, o, sc, accumulate);
+            break;
+        case 2:
+            hipLaunchKernelGGL(sell_coded_matvec_kernel<2>, dim3(blocks), dim3(256), 0, s, nrows,
+                               ncols, nslices, slice_ptr, cw, vt, otab, vbits, xx, o, sc, accumulate);
+            break;
+        case 4:
+            hipLaunchKernelGGL(sell_coded_matvec_kernel<4>, dim3(blocks), dim3(256), 0, s, nrows,
+                               ncols, nslices, slice_ptr, cw, vt, otab, vbits, xx, o, sc, accumulate);
+            break;
+        default:
+            hipLaunchKernelGGL(sell_coded_matvec_kernel<8>, dim3(blocks), dim3(256), 0, s, nrows,
+                               ncols, nslices, slice_ptr, cw, vt, otab, vbits, xx, o, sc, accumulate);
+            break;
+    }
     QA_CHECK_LAUNCH();
     return QA_OK;
 }

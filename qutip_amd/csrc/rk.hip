@@ -210,6 +210,12 @@ struct StageArgs : EpiArgs {
     const int *col;
     const V *vals;
     int nslices;
+    // coded operator (rk_stage_kernel<.., .., true>): 16-bit codes packed
+    // four to a word, the value and column-offset tables
+    const uint2 *code;
+    const double2 *vtab;
+    const int *otab;
+    int vbits;
 };
 
 // The vectors a stage reads and writes, resolved from the control block.
@@ -297,18 +303,25 @@ __device__ __forceinline__ double epi_post(const EpiArgs &P, const RKCtl *ctl, c
     return 0.0;
 }
 
-template <typename V, int L>
+template <typename V, int L, bool CODED>
 __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
     RKCtl *ctl = P.ctl;
+    __shared__ double2 vt[CODED ? kCodeTabMax : 1];
+    __shared__ int ot[CODED ? kCodeTabMax : 1];
+    static_assert(kStepBlocks == 256, "code_tables_to_lds assumes 256 threads");
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int slice = (int)(t / QA_WAVE), lane = (int)(t % QA_WAVE);
     const int64_t r = (int64_t)slice * (QA_WAVE / L) + lane / L;  // the row of this lane
-    // The slice bounds and the control block are read before any branch,
-    // so their round trips overlap.
+    // The tables of a coded operator (straight into LDS), the slice bounds
+    // and the control block are all requested before anything waits, so
+    // their round trips overlap.  The barrier comes before the phase branch:
+    // no wave exits with a load into LDS outstanding.
+    if constexpr (CODED) code_tables_to_lds(P.vtab, P.otab, P.vbits, vt, ot);
     const int sl = slice < P.nslices ? slice : P.nslices - 1;
     const int sp0 = P.slice_ptr[sl], sp1 = P.slice_ptr[sl + 1];
     const int phase = ctl->phase;
     const StageVecs v = stage_vecs(P, ctl);
+    if constexpr (CODED) __syncthreads();
     if (phase != P.run_phase) return;  // uniform over the whole grid
     const int base = sp0, steps = (sp1 - sp0) / QA_WAVE;
     double part = 0.0;
@@ -318,7 +331,12 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
         const bool owner = (lane % L == 0);
         double2 o = make_double2(0.0, 0.0), er = make_double2(0.0, 0.0);
         if (owner) epi_pre(P, v, r, o, er);
-        const double2 acc = sell_row_dot<L>(P.col, P.vals, v.xin, base, steps, lane);
+        double2 acc;
+        if constexpr (CODED)
+            acc = sell_coded_row_dot<L>(P.code, vt, ot, CodeSplit{(1u << P.vbits) - 1u, P.vbits},
+                                        v.xin, base, steps, lane, (int)r, P.nrows - 1);
+        else
+            acc = sell_row_dot<L>(P.col, P.vals, v.xin, base, steps, lane);
         if (owner) part = epi_post(P, ctl, v, r, acc, o, er);
     }
     if (P.epi == EPI_FINAL) {
@@ -1028,8 +1046,9 @@ __global__ __launch_bounds__(256) void rk_store_kernel(OutArgs A) {
 // Functionals of the pending output states: linear f = sum w y[col] (a
 // vectorised density matrix: tr(O rho), tr(rho)) or quadratic
 // f = sum w conj(y[row]) y[col] (a ket: <psi|O|psi>, ||psi||^2).  Grid
-// (nchunks, nfun); each workgroup writes the partial sum of its chunk, which
-// rk_output_finish_kernel adds up.
+// (nchunks, nfun, kMaxPending): one workgroup per chunk, functional and
+// pending output time (layers past the pending count return at once); each
+// writes the partial sum of its chunk, which rk_output_finish_kernel adds up.
 __global__ __launch_bounds__(256) void rk_output_kernel(OutArgs A) {
     RKCtl *c = A.ctl;
     // the functional's range does not depend on the control block: it is
@@ -1044,24 +1063,23 @@ __global__ __launch_bounds__(256) void rk_output_kernel(OutArgs A) {
     __shared__ int nq_s;
     __shared__ double smem[256 / QA_WAVE];
     const int nq = pending_outputs(c, A, w, front, &nq_s);
+    const int q = blockIdx.z;  // one output time per workgroup layer
+    if (q >= nq) return;       // uniform over the workgroup
     const double2 *yprev = c->cur ? A.y1 : A.y0;
     const double2 *yfront = c->cur ? A.y0 : A.y1;
     const int beg = fbeg + blockIdx.x * kOutChunk;
     const int end = min(fend, beg + kOutChunk);
-    for (int q = 0; q < nq; ++q) {
-        double2 acc = make_double2(0.0, 0.0);
-        for (int e = beg + threadIdx.x; e < end; e += blockDim.x) {
-            double2 v = cmul(A.fw[e], state_at(A, yprev, yfront, w[q], front[q], A.fcol[e]));
-            if (quad) {
-                const double2 l = state_at(A, yprev, yfront, w[q], front[q], A.frow[e]);
-                v = cmul(make_double2(l.x, -l.y), v);
-            }
-            acc = cadd(acc, v);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int e = beg + threadIdx.x; e < end; e += blockDim.x) {
+        double2 v = cmul(A.fw[e], state_at(A, yprev, yfront, w[q], front[q], A.fcol[e]));
+        if (quad) {
+            const double2 l = state_at(A, yprev, yfront, w[q], front[q], A.frow[e]);
+            v = cmul(make_double2(l.x, -l.y), v);
         }
-        const double2 sq = block_sum2(acc, smem);
-        if (threadIdx.x == 0)
-            A.partials[((int64_t)q * A.nfun + f) * A.nchunks + blockIdx.x] = sq;
+        acc = cadd(acc, v);
     }
+    const double2 sq = block_sum2(acc, smem);
+    if (threadIdx.x == 0) A.partials[((int64_t)q * A.nfun + f) * A.nchunks + blockIdx.x] = sq;
 }
 
 // Completes an output launch: sums the partial sums of every (output time,
@@ -1146,6 +1164,11 @@ struct qa_rk_solver {
     const int *slice_ptr = nullptr;
     const int *col = nullptr;
     const void *vals = nullptr;
+    int coded = 0;  // coded SELL-C operator: codes / value table / offset table
+    int vbits = 8;
+    const void *code = nullptr;
+    const double2 *vtab = nullptr;
+    const int *otab = nullptr;
     // tableau
     int order = 0, rk_step = 0, rk_extra = 0, dorder = 0, interpolate = 1, adaptive = 1;
     int first_dense_from_front = 0;
@@ -1255,6 +1278,10 @@ StageArgs<V> stage_args(qa_rk_solver *S, int i) {
     P.col = S->col;
     P.vals = (const V *)S->vals;
     P.nslices = S->nslices;
+    P.code = (const uint2 *)S->code;
+    P.vtab = S->vtab;
+    P.otab = S->otab;
+    P.vbits = S->vbits;
     return P;
 }
 
@@ -1263,15 +1290,15 @@ int stage_blocks(const qa_rk_solver *S) {
     return (int)((threads + kStepBlocks - 1) / kStepBlocks);
 }
 
-template <typename V>
+template <typename V, bool CODED>
 int launch_sparse_stage(qa_rk_solver *S, int i) {
     StageArgs<V> P = stage_args<V>(S, i);
     const dim3 grid((unsigned)stage_blocks(S)), block(kStepBlocks);
     switch (S->lanes) {
-        case 1: hipLaunchKernelGGL((rk_stage_kernel<V, 1>), grid, block, 0, S->stream, P); break;
-        case 2: hipLaunchKernelGGL((rk_stage_kernel<V, 2>), grid, block, 0, S->stream, P); break;
-        case 4: hipLaunchKernelGGL((rk_stage_kernel<V, 4>), grid, block, 0, S->stream, P); break;
-        case 8: hipLaunchKernelGGL((rk_stage_kernel<V, 8>), grid, block, 0, S->stream, P); break;
+        case 1: hipLaunchKernelGGL((rk_stage_kernel<V, 1, CODED>), grid, block, 0, S->stream, P); break;
+        case 2: hipLaunchKernelGGL((rk_stage_kernel<V, 2, CODED>), grid, block, 0, S->stream, P); break;
+        case 4: hipLaunchKernelGGL((rk_stage_kernel<V, 4, CODED>), grid, block, 0, S->stream, P); break;
+        case 8: hipLaunchKernelGGL((rk_stage_kernel<V, 8, CODED>), grid, block, 0, S->stream, P); break;
         default:
             qa::set_error("lanes per row must be 1, 2, 4 or 8");
             return QA_ERR_ARG;
@@ -1307,7 +1334,9 @@ int launch_dense_stage(qa_rk_solver *S, int i) {
 
 int launch_stage(qa_rk_solver *S, int i) {
     if (S->kind == 1) return launch_dense_stage(S, i);
-    return S->vals_fp32 ? launch_sparse_stage<float2>(S, i) : launch_sparse_stage<double2>(S, i);
+    if (S->coded) return launch_sparse_stage<double2, true>(S, i);
+    return S->vals_fp32 ? launch_sparse_stage<float2, false>(S, i)
+                        : launch_sparse_stage<double2, false>(S, i);
 }
 
 // out = A x (ungated): the derivative for the first-step estimate.
@@ -1334,6 +1363,10 @@ int engine_rhs(qa_rk_solver *S, const double2 *x, double2 *out) {
         QA_CHECK_LAUNCH();
         return QA_OK;
     }
+    if (S->coded)
+        return qa_sell_coded_matvec_z(S->n, S->n, S->lanes, S->nslices, S->slice_ptr, S->code,
+                                      S->vtab, S->otab, S->vbits, x, out, 1.0, 0.0, 0,
+                                      S->stream);
     return qa_sell_matvec_z(S->n, S->lanes, S->nslices, S->slice_ptr, S->col, S->vals,
                             S->vals_fp32, x, out, 1.0, 0.0, 0, S->stream);
 }
@@ -1393,8 +1426,8 @@ int enqueue_iteration(qa_rk_solver *S) {
     unsigned sblocks = (unsigned)std::min<int64_t>((S->npad + 255) / 256, 2048);
     hipLaunchKernelGGL(rk_store_kernel, dim3(sblocks), dim3(256), 0, S->stream, A);
     QA_CHECK_LAUNCH();
-    hipLaunchKernelGGL(rk_output_kernel, dim3(S->nchunks, std::max(S->nfun, 1)), dim3(256), 0,
-                       S->stream, A);
+    hipLaunchKernelGGL(rk_output_kernel, dim3(S->nchunks, std::max(S->nfun, 1), kMaxPending),
+                       dim3(256), 0, S->stream, A);
     QA_CHECK_LAUNCH();
     hipLaunchKernelGGL(rk_output_finish_kernel, dim3(1), dim3(256), 0, S->stream, A);
     QA_CHECK_LAUNCH();
@@ -1590,6 +1623,43 @@ int qa_rk_create(qa_rk_solver **out, int n, int lanes, int nslices, const int *s
     S->col = col;
     S->vals = vals;
     S->vals_fp32 = vals_fp32;
+    rc = rk_finish_create(S, order, rk_step, rk_extra, dorder, a, b, c, e, bi, rtol, atol, nsteps,
+                          first_step, min_step, max_step, interpolate);
+    if (rc) return rc;
+    *out = S;
+    return QA_OK;
+}
+
+int qa_rk_create_coded(qa_rk_solver **out, int n, int lanes, int nslices, const int *slice_ptr,
+                       const void *codes, const void *vtab, const int *otab, int vbits, int order,
+                       int rk_step, int rk_extra, int dorder, const double *a, const double *b,
+                       const double *c, const double *e, const double *bi, double rtol,
+                       double atol, int nsteps, double first_step, double min_step,
+                       double max_step, int interpolate) {
+    QA_REQUIRE(out != nullptr, "qa_rk_create_coded: null output");
+    *out = nullptr;
+    QA_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8,
+               "qa_rk_create_coded: lanes must be 1, 2, 4 or 8");
+    QA_REQUIRE(n > 0 && nslices == (n + QA_WAVE / lanes - 1) / (QA_WAVE / lanes),
+               "qa_rk_create_coded: need n > 0 and nslices = ceil(n / (64 / lanes))");
+    QA_REQUIRE(slice_ptr && codes && vtab && otab, "qa_rk_create_coded: null operator pointer");
+    QA_REQUIRE(vbits >= 6 && vbits <= 10, "qa_rk_create_coded: vbits must be 6..10");
+    int rc = check_tableau(rk_step, rk_extra, dorder, a, b, c, bi, nsteps);
+    if (rc) return rc;
+    qa_rk_solver *S = new (std::nothrow) qa_rk_solver();
+    QA_REQUIRE(S != nullptr, "qa_rk_create_coded: out of host memory");
+    S->kind = 0;
+    S->n = n;
+    S->lanes = lanes;
+    S->nslices = nslices;
+    S->npad = (int64_t)nslices * (QA_WAVE / lanes);
+    S->nparts = stage_blocks(S);
+    S->slice_ptr = slice_ptr;
+    S->coded = 1;
+    S->code = codes;
+    S->vtab = (const double2 *)vtab;
+    S->otab = otab;
+    S->vbits = vbits;
     rc = rk_finish_create(S, order, rk_step, rk_extra, dorder, a, b, c, e, bi, rtol, atol, nsteps,
                           first_step, min_step, max_step, interpolate);
     if (rc) return rc;

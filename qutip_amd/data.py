@@ -337,27 +337,132 @@ def sell_arrays(A, lanes=1):
     return slice_ptr.astype(np.int32), cols, vals, npad
 
 
-class SellMatrix:
-    """A SELL-C copy of a CSR matrix in HBM (see sell_arrays)."""
+CODE_VBITS = range(6, 11)  # value-index bits of a 16-bit code (tables of <= 1024)
 
-    def __init__(self, A, fp32=False, lanes=1):
-        sptr, cols, vals, npad = sell_arrays(A, lanes)
+
+def code_split(nvals, noffs):
+    """Smallest value-index width b in 6..10 with nvals <= 2^b and
+    noffs <= 2^(16 - b), or None."""
+    for b in CODE_VBITS:
+        if nvals <= (1 << b) and noffs <= (1 << (16 - b)):
+            return b
+    return None
+
+
+def sell_coded_arrays(A, lanes=1, fp32=False):
+    """Coded SELL-C layout of a CSR matrix, or None when its distinct values
+    and distinct column offsets (column - row) fit no 16-bit split
+    (``code_split``: up to 1024 values with 64 offsets ... 64 values with
+    1024 offsets).
+
+    The slices and the lane -> entry assignment are those of ``sell_arrays``
+    with per-slice widths rounded up to a multiple of 4 * lanes.  Entry j of
+    row C s + r is the 16-bit code ``v | (o << vbits)``: v indexes ``vtab``
+    (its value, bit-exact; rounded to complex64 first when ``fp32``), o
+    indexes ``otab`` (its column minus its row).  A lane's steps 4w .. 4w + 3
+    are the four codes of word ``slice_ptr[s] + 64 w + lane`` (little-endian,
+    step 4w in the low half of the low 32 bits), so ``slice_ptr`` counts
+    8-byte words.  Padding entries are value 0 at offset 0.
+
+    Returns (slice_ptr int32, codes uint16, vtab complex128[2^vbits],
+    otab int32[2^(16 - vbits)], vbits, npad)."""
+    if lanes not in (1, 2, 4, 8):
+        raise ValueError("lanes must be 1, 2, 4 or 8")
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    n = A.shape[0]
+    vals = A.data.astype(np.complex128)
+    if fp32:
+        vals = vals.astype(np.complex64).astype(np.complex128)
+    # distinct values by their bits (-0.0 and 0.0 stay apart), value 0 first
+    bits = np.concatenate([np.zeros((1, 2), np.uint64), vals.view(np.uint64).reshape(-1, 2)])
+    ubits, vinv = np.unique(bits, axis=0, return_inverse=True)
+    rn = np.diff(A.indptr).astype(np.int64)
+    rows = np.repeat(np.arange(n, dtype=np.int64), rn)
+    offs = np.concatenate([[0], A.indices.astype(np.int64) - rows])
+    uoffs, oinv = np.unique(offs, return_inverse=True)
+    vbits = code_split(len(ubits), len(uoffs))
+    if vbits is None:
+        return None
+    vinv = vinv.reshape(-1).astype(np.int64)
+    oinv = oinv.reshape(-1).astype(np.int64)
+    vtab = np.zeros(1 << vbits, dtype=np.complex128)
+    vtab[:len(ubits)] = np.ascontiguousarray(ubits).view(np.complex128).reshape(-1)
+    otab = np.zeros(1 << (16 - vbits), dtype=np.int32)
+    otab[:len(uoffs)] = uoffs
+    pad_code = int(vinv[0]) | (int(oinv[0]) << vbits)
+    C = 64 // lanes
+    ns = (n + C - 1) // C
+    npad = ns * C
+    rnp = np.zeros(npad, dtype=np.int64)
+    rnp[:n] = rn
+    width = rnp.reshape(ns, C).max(axis=1)
+    quantum = 4 * lanes
+    width = (width + quantum - 1) // quantum * quantum
+    words = width * C // 4
+    slice_ptr = np.zeros(ns + 1, dtype=np.int64)
+    slice_ptr[1:] = np.cumsum(words)
+    total_words = int(slice_ptr[-1])
+    if total_words >= 2 ** 31:
+        raise ValueError("coded SELL storage exceeds int32 word offsets")
+    codes = np.full(4 * max(total_words, 1), pad_code, dtype=np.uint16)
+    j = np.arange(A.nnz, dtype=np.int64) - np.repeat(A.indptr[:-1].astype(np.int64), rn)
+    u, q = j // lanes, j % lanes
+    lane = (rows % C) * lanes + q
+    word = slice_ptr[rows // C] + (u // 4) * 64 + lane
+    codes[4 * word + u % 4] = (vinv[1:] | (oinv[1:] << vbits)).astype(np.uint16)
+    return slice_ptr.astype(np.int32), codes, vtab, otab, vbits, npad
+
+
+class SellMatrix:
+    """A SELL-C copy of a CSR matrix in HBM (see sell_arrays), or with
+    ``coded=True`` the coded form (sell_coded_arrays; ValueError when the
+    matrix does not fit the 256-entry tables)."""
+
+    def __init__(self, A, fp32=False, lanes=1, coded=False):
         self.shape = A.shape
         self.lanes = lanes
+        self.fp32 = fp32
+        self.coded = bool(coded)
+        if self.coded:
+            arrs = sell_coded_arrays(A, lanes, fp32)
+            if arrs is None:
+                raise ValueError("the operator's distinct values and column offsets fit no "
+                                 "16-bit code split: no coded SELL-C form")
+            sptr, codes, vtab, otab, self.vbits, npad = arrs
+            self.nslices = len(sptr) - 1
+            self.npad = npad
+            self.stored = 4 * int(sptr[-1])
+            self._sptr = DeviceBuffer.from_array(sptr)
+            self._cols = DeviceBuffer.from_array(codes)
+            self._vals = DeviceBuffer.from_array(vtab)
+            self._otab = DeviceBuffer.from_array(otab)
+            return
+        sptr, cols, vals, npad = sell_arrays(A, lanes)
         self.nslices = len(sptr) - 1
         self.npad = npad
         self.stored = int(sptr[-1])
-        self.fp32 = fp32
         self._sptr = DeviceBuffer.from_array(sptr)
         self._cols = DeviceBuffer.from_array(cols)
         self._vals = DeviceBuffer.from_array(vals.astype(np.complex64 if fp32 else np.complex128))
 
     @property
     def ptrs(self):
+        """(slice_ptr, columns, values); coded: (slice_ptr, codes, value table)."""
         return self._sptr.ptr, self._cols.ptr, self._vals.ptr
+
+    @property
+    def otab_ptr(self):
+        return self._otab.ptr
 
     def matvec(self, x, out, scale=1.0, accumulate=False):
         sre, sim = _scalar(scale)
+        if self.coded:
+            _lib.call("qa_sell_coded_matvec_z", self.shape[0], self.shape[1], self.lanes,
+                      self.nslices, _vp(self._sptr.ptr), _vp(self._cols.ptr),
+                      _vp(self._vals.ptr), _vp(self._otab.ptr), self.vbits, _vp(x.ptr),
+                      _vp(out.ptr), sre, sim, int(accumulate), None)
+            return out
         _lib.call("qa_sell_matvec_z", self.shape[0], self.lanes, self.nslices,
                   _vp(self._sptr.ptr), _vp(self._cols.ptr), _vp(self._vals.ptr), int(self.fp32),
                   _vp(x.ptr), _vp(out.ptr), sre, sim, int(accumulate), None)

@@ -15,8 +15,12 @@ explicit_rk.pyx, tableaus vern7 / tsit5):
 Options are those of IntegratorVern7 (qutip/solver/integrator/
 qutip_integrator.py:52-60) plus ``tableau`` ('vern7' | 'tsit5'), ``fused``
 (bool), ``operator_precision`` ('fp64' | 'fp32': complex64 operator
-values, FP64 state, accumulation and error norm) and ``dense_precision``
-('fp32' | 'bf16x3': the MFMA products of the matrix-form Lindblad RHS).
+values, FP64 state, accumulation and error norm), ``operator_format``
+('auto' | 'coded' | 'sell': the fused engine's operator as 2-byte codes into
+value and column-offset tables when it has at most 256 of each -- the same
+result bits as the plain SELL-C values and columns, a tenth of the bytes)
+and ``dense_precision`` ('fp32' | 'bf16x3': the MFMA products of the
+matrix-form Lindblad RHS).
 """
 import ctypes as C
 
@@ -59,10 +63,13 @@ class RKEngine:
 
     def __init__(self, op, tableau="vern7", rtol=1e-6, atol=1e-8, nsteps=1000, first_step=0.0,
                  min_step=0.0, max_step=0.0, interpolate=True, fp32=False, lanes=None,
-                 dense=None, dense_precision="fp32"):
+                 dense=None, dense_precision="fp32", coded="auto"):
         """``op``: the sparse generator A (dy/dt = A y), or None with
         ``dense=(H_nh, [c_ops])`` (numpy arrays) for the matrix-form
-        Lindblad right-hand side on MFMA."""
+        Lindblad right-hand side on MFMA.  ``coded``: store A as coded
+        SELL-C (2-byte codes into 256-entry value and column-offset tables,
+        the same result bits) -- "auto" when A fits the tables, True to
+        require it, False for the plain form."""
         tab = TABLEAUS[tableau] if isinstance(tableau, str) else tableau
         self.tableau = tab
         self._a = np.ascontiguousarray(tab["a"], dtype=np.float64)
@@ -85,6 +92,7 @@ class RKEngine:
             self.n = N * N
             self.lanes = 0
             self.sell = None
+            self.coded = False
             self.dense_precision = dense_precision
             Hf = np.asfortranarray(Hnh, dtype=np.complex128)
             cf = np.concatenate([np.asfortranarray(c, dtype=np.complex128).ravel(order="F")
@@ -103,15 +111,26 @@ class RKEngine:
             if A.shape[0] != A.shape[1]:
                 raise ValueError("the generator must be square")
             self.n = A.shape[0]
+            if coded == "auto":
+                coded = _data.sell_coded_arrays(A, 1, fp32) is not None
+            coded = bool(coded)
             if lanes is None:
-                lanes = autotune.lanes_for(A, fp32)
+                lanes = autotune.lanes_for(A, fp32, coded)
             self.lanes = lanes
-            self.sell = _data.SellMatrix(A, fp32=fp32, lanes=lanes)
+            self.coded = coded
+            self.sell = _data.SellMatrix(A, fp32=fp32, lanes=lanes, coded=coded)
             sptr, cols, vals = self.sell.ptrs
-            rc = L.qa_rk_create(
-                C.byref(h), self.n, lanes, self.sell.nslices, C.c_void_p(sptr),
-                C.c_void_p(cols), C.c_void_p(vals), int(fp32), *tab_args)
-            _lib.check(rc, "qa_rk_create")
+            if coded:
+                rc = L.qa_rk_create_coded(
+                    C.byref(h), self.n, lanes, self.sell.nslices, C.c_void_p(sptr),
+                    C.c_void_p(cols), C.c_void_p(vals), C.c_void_p(self.sell.otab_ptr),
+                    self.sell.vbits, *tab_args)
+                _lib.check(rc, "qa_rk_create_coded")
+            else:
+                rc = L.qa_rk_create(
+                    C.byref(h), self.n, lanes, self.sell.nslices, C.c_void_p(sptr),
+                    C.c_void_p(cols), C.c_void_p(vals), int(fp32), *tab_args)
+                _lib.check(rc, "qa_rk_create")
         self.handle = h
         self.nfun = 0
         self._y_dev = _data.HipDense((self.n, 1))
@@ -404,6 +423,7 @@ class IntegratorHipRK(Integrator):
         "tableau": "vern7",
         "fused": True,
         "operator_precision": "fp64",
+        "operator_format": "auto",
         "dense_precision": "fp32",
     }
     method = "hip_rk"
@@ -429,8 +449,10 @@ class IntegratorHipRK(Integrator):
         elif op is not None:
             self._engine = RKEngine(op, o["tableau"], o["rtol"], o["atol"], o["nsteps"],
                                     o["first_step"], o["min_step"], o["max_step"],
-                                    o["interpolate"], fp32=(o["operator_precision"] == "fp32"))
-            self.name = "hip_rk (fused)"
+                                    o["interpolate"], fp32=(o["operator_precision"] == "fp32"),
+                                    coded={"auto": "auto", "coded": True,
+                                           "sell": False}[o["operator_format"]])
+            self.name = "hip_rk (fused, coded operator)" if self._engine.coded else "hip_rk (fused)"
             if op.shape[0] <= _small.MAX_N:
                 # whole solves (Solver.run) of a small system in one workgroup
                 self._small = _small.SmallSolve(
@@ -557,5 +579,7 @@ class IntegratorHipRK(Integrator):
             raise ValueError(f"unknown tableau {self._options.get('tableau')!r}")
         if self._options.get("operator_precision") not in ("fp64", "fp32"):
             raise ValueError("operator_precision must be 'fp64' or 'fp32'")
+        if self._options.get("operator_format") not in ("auto", "coded", "sell"):
+            raise ValueError("operator_format must be 'auto', 'coded' or 'sell'")
         if self._options.get("dense_precision") not in ("fp32", "bf16x3"):
             raise ValueError("dense_precision must be 'fp32' or 'bf16x3'")

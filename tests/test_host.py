@@ -134,6 +134,82 @@ def test_ell_layout_reproduces_csr(d, density):
     assert val32.dtype == np.complex64
 
 
+def _sell_coded_matvec(sptr, codes, vtab, otab, vbits, shape, x, lanes):
+    """numpy emulation of the coded SELL-C kernel: word w of lane l of slice
+    s is the uint64 at 8 (sptr[s] + 64 w + l) bytes holding the lane's steps
+    4w .. 4w + 3 in its 16-bit quarters, low first; a code's low vbits bits
+    pick the value, the rest the column offset from the row (clamped)."""
+    n, ncols = shape
+    ns = len(sptr) - 1
+    C = 64 // lanes
+    out = np.zeros(ns * C, dtype=np.complex128)
+    for s in range(ns):
+        words = (sptr[s + 1] - sptr[s]) // 64
+        c = codes[4 * sptr[s]:4 * sptr[s + 1]].reshape(words, 64, 4).astype(np.int64)
+        rows = s * C + np.arange(64) // lanes
+        cols = np.clip(rows[None, :, None] + otab[c >> vbits], 0, ncols - 1)
+        lane_sum = (vtab[c & ((1 << vbits) - 1)] * x[cols]).sum(axis=(0, 2))
+        out[s * C:(s + 1) * C] = lane_sum.reshape(C, lanes).sum(axis=1)
+    return out[:n]
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8])
+def test_coded_layout_reproduces_csr(lanes):
+    """Coded SELL-C of the dissipative Ising Liouvillian (L=5) and of a
+    ragged banded matrix (n not a multiple of 64, explicit zeros): the
+    kernel's reading of the codes gives A x, and slices are whole words of
+    the 64 lanes."""
+    m = models.ising(5, 0.1)
+    Lm = models.liouvillian(m["H"], m["c_ops"]).tocsr()
+    rng = np.random.default_rng(lanes)
+    n = 1000
+    band = sp.diags([rng.choice([0.5, -1j, 2.0], n - k) for k in (0, 1, 7, 130)],
+                    [0, 1, -7, 130], shape=(n, n), format="csr")
+    band.data[::17] = 0.0  # explicit zeros are entries like any other
+    for A in (Lm, band):
+        arrs = qdata.sell_coded_arrays(A, lanes)
+        assert arrs is not None
+        sptr, codes, vtab, otab, vbits, npad = arrs
+        assert np.all(np.diff(sptr) % 64 == 0)
+        assert len(vtab) == 1 << vbits and len(otab) == 1 << (16 - vbits)
+        x = rng.standard_normal(A.shape[1]) + 1j * rng.standard_normal(A.shape[1])
+        assert_allclose(_sell_coded_matvec(sptr, codes, vtab, otab, vbits, A.shape, x, lanes),
+                        A @ x, atol=1e-12)
+    # the same value bits as the plain form: its values appear in the table
+    sptr, codes, vtab, otab, vbits, _ = qdata.sell_coded_arrays(Lm, lanes)
+    assert set(Lm.data.view(np.uint64).reshape(-1, 2).tobytes()[i:i + 16]
+               for i in range(0, 16 * Lm.nnz, 16)) <= set(
+        vtab.view(np.uint64).reshape(-1, 2).tobytes()[i:i + 16] for i in range(0, 16 * 256, 16))
+
+
+def test_coded_layout_limits():
+    """The 16-bit split: value bits b in 6..10 with <= 2^b values and
+    <= 2^(16-b) column offsets, the smallest b that fits; no coded form when
+    none fits.  -0.0 and 0.0 are distinct values (their products can differ
+    in sign)."""
+    assert qdata.code_split(64, 1024) == 6 and qdata.code_split(65, 1024) is None
+    assert qdata.code_split(352, 41) == 9 and qdata.code_split(1024, 64) == 10
+    assert qdata.code_split(1025, 1) is None
+    rng = np.random.default_rng(0)
+    A = sp.random(300, 300, density=0.05, random_state=rng, format="csr")
+    assert qdata.sell_coded_arrays(A) is None  # ~4500 distinct values
+    # row r at column r + r // 2 (mod n): 300 distinct column offsets
+    n = 600
+    cols = (np.arange(n) + np.arange(n) // 2) % n
+    assert len(np.unique(cols - np.arange(n))) == 300
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), cols)), shape=(n, n))
+    arrs = qdata.sell_coded_arrays(P)
+    assert arrs is not None and arrs[4] == 6  # 2 values (1 and the padding 0)
+    P2 = sp.csr_matrix((np.arange(1.0, n + 1), (np.arange(n), cols)), shape=(n, n))
+    assert qdata.sell_coded_arrays(P2) is None  # 601 values with 300 offsets
+    Z = sp.csr_matrix((np.array([-0.0, 1.0]), np.array([1, 0]), np.array([0, 1, 2])), shape=(2, 2))
+    sptr, codes, vtab, otab, vbits, _ = qdata.sell_coded_arrays(Z)
+    signs = {np.signbit(v.real) for v in vtab[codes[:4] & ((1 << vbits) - 1)]}
+    assert signs == {True, False}
+    with pytest.raises(ValueError, match="code split"):
+        qdata.SellMatrix(A, coded=True)
+
+
 def test_server_fails_a_request_above_the_budget():
     """A request that can never fit the HBM budget fails at once instead of
     waiting forever (no device work is started)."""
