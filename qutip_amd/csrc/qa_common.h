@@ -227,6 +227,57 @@ __device__ __forceinline__ double2 sell_coded_row_dot(const uint2 *__restrict__ 
     return acc;
 }
 
+// Coded rows of uniform width (every slice `uwords` words per lane, word w
+// of slice s at s uwords 64 + 64 w + lane): the words do not depend on a
+// slice pointer, so a kernel can request them with its first loads.  `cw`
+// holds the lane's words (kUWordsMax, the ones past uwords unused); the
+// terms are summed as by sell_coded_row_dot (the same bits).
+constexpr int kUWordsMax = kRowChunk / 4;
+
+template <int M>
+__device__ __forceinline__ void sell_coded_words(const uint2 (&cw)[kUWordsMax], const double2 *vt,
+                                                 const int *ot, CodeSplit cs,
+                                                 const double2 *__restrict__ x, int row, int cmax,
+                                                 double2 &a0, double2 &a1) {
+    unsigned h[4 * M];
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        h[4 * u + 0] = cw[u].x & 0xffffu;
+        h[4 * u + 1] = cw[u].x >> 16;
+        h[4 * u + 2] = cw[u].y & 0xffffu;
+        h[4 * u + 3] = cw[u].y >> 16;
+    }
+    double2 xv[4 * M];
+#pragma unroll
+    for (int e = 0; e < 4 * M; ++e) xv[e] = x[min(max(row + ot[h[e] >> cs.vbits], 0), cmax)];
+#pragma unroll
+    for (int e = 0; e < 4 * M; ++e) cfma((e & 1) ? a1 : a0, vt[h[e] & cs.vmask], xv[e]);
+}
+
+template <int L>
+__device__ __forceinline__ double2 sell_coded_row_dot_uniform(const uint2 (&cw)[kUWordsMax],
+                                                              int uwords, const double2 *vt,
+                                                              const int *ot, CodeSplit cs,
+                                                              const double2 *__restrict__ x,
+                                                              int row, int cmax) {
+    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+    switch (uwords) {
+        case 6: sell_coded_words<6>(cw, vt, ot, cs, x, row, cmax, a0, a1); break;
+        case 5: sell_coded_words<5>(cw, vt, ot, cs, x, row, cmax, a0, a1); break;
+        case 4: sell_coded_words<4>(cw, vt, ot, cs, x, row, cmax, a0, a1); break;
+        case 3: sell_coded_words<3>(cw, vt, ot, cs, x, row, cmax, a0, a1); break;
+        case 2: sell_coded_words<2>(cw, vt, ot, cs, x, row, cmax, a0, a1); break;
+        default: sell_coded_words<1>(cw, vt, ot, cs, x, row, cmax, a0, a1); break;
+    }
+    double2 acc = cadd(a0, a1);
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) {
+        acc.x += __shfl_xor(acc.x, off, QA_WAVE);
+        acc.y += __shfl_xor(acc.y, off, QA_WAVE);
+    }
+    return acc;
+}
+
 // The tables of a coded operator into LDS (`vt`: kCodeTabMax double2,
 // `ot`: kCodeTabMax int) by a workgroup of exactly 256 threads, with direct
 // global -> LDS loads: lane l of a wave writes entry i0 + l, i0 the wave's

@@ -216,7 +216,11 @@ struct StageArgs : EpiArgs {
     const double2 *vtab;
     const int *otab;
     int vbits;
+    int uwords;  // > 0: every slice uwords words per lane (FORM_CODED_UNIFORM)
 };
+
+// Operator forms of rk_stage_kernel.
+enum StageForm { FORM_PLAIN = 0, FORM_CODED = 1, FORM_CODED_UNIFORM = 2 };
 
 // The vectors a stage reads and writes, resolved from the control block.
 struct StageVecs {
@@ -303,8 +307,9 @@ __device__ __forceinline__ double epi_post(const EpiArgs &P, const RKCtl *ctl, c
     return 0.0;
 }
 
-template <typename V, int L, bool CODED>
+template <typename V, int L, int FORM>
 __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
+    constexpr bool CODED = FORM != FORM_PLAIN;
     RKCtl *ctl = P.ctl;
     __shared__ double2 vt[CODED ? kCodeTabMax : 1];
     __shared__ int ot[CODED ? kCodeTabMax : 1];
@@ -313,12 +318,22 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
     const int slice = (int)(t / QA_WAVE), lane = (int)(t % QA_WAVE);
     const int64_t r = (int64_t)slice * (QA_WAVE / L) + lane / L;  // the row of this lane
     // The tables of a coded operator (straight into LDS), the slice bounds
-    // and the control block are all requested before anything waits, so
-    // their round trips overlap.  The barrier comes before the phase branch:
-    // no wave exits with a load into LDS outstanding.
+    // (or, for uniform slices, the code words themselves) and the control
+    // block are all requested before anything waits, so their round trips
+    // overlap.  The barrier comes before the phase branch: no wave exits
+    // with a load into LDS outstanding.
     if constexpr (CODED) code_tables_to_lds(P.vtab, P.otab, P.vbits, vt, ot);
     const int sl = slice < P.nslices ? slice : P.nslices - 1;
-    const int sp0 = P.slice_ptr[sl], sp1 = P.slice_ptr[sl + 1];
+    int sp0 = 0, sp1 = 0;
+    uint2 cw[kUWordsMax];
+    if constexpr (FORM == FORM_CODED_UNIFORM) {
+        const int wb = sl * P.uwords * QA_WAVE + lane;
+#pragma unroll
+        for (int u = 0; u < kUWordsMax; ++u) cw[u] = P.code[wb + min(u, P.uwords - 1) * QA_WAVE];
+    } else {
+        sp0 = P.slice_ptr[sl];
+        sp1 = P.slice_ptr[sl + 1];
+    }
     const int phase = ctl->phase;
     const StageVecs v = stage_vecs(P, ctl);
     if constexpr (CODED) __syncthreads();
@@ -332,9 +347,13 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
         double2 o = make_double2(0.0, 0.0), er = make_double2(0.0, 0.0);
         if (owner) epi_pre(P, v, r, o, er);
         double2 acc;
-        if constexpr (CODED)
-            acc = sell_coded_row_dot<L>(P.code, vt, ot, CodeSplit{(1u << P.vbits) - 1u, P.vbits},
-                                        v.xin, base, steps, lane, (int)r, P.nrows - 1);
+        const CodeSplit cs = {(1u << P.vbits) - 1u, P.vbits};
+        if constexpr (FORM == FORM_CODED_UNIFORM)
+            acc = sell_coded_row_dot_uniform<L>(cw, P.uwords, vt, ot, cs, v.xin, (int)r,
+                                                P.nrows - 1);
+        else if constexpr (FORM == FORM_CODED)
+            acc = sell_coded_row_dot<L>(P.code, vt, ot, cs, v.xin, base, steps, lane, (int)r,
+                                        P.nrows - 1);
         else
             acc = sell_row_dot<L>(P.col, P.vals, v.xin, base, steps, lane);
         if (owner) part = epi_post(P, ctl, v, r, acc, o, er);
@@ -1166,6 +1185,7 @@ struct qa_rk_solver {
     const void *vals = nullptr;
     int coded = 0;  // coded SELL-C operator: codes / value table / offset table
     int vbits = 8;
+    int uwords = 0;  // coded, every slice the same width: words per lane
     const void *code = nullptr;
     const double2 *vtab = nullptr;
     const int *otab = nullptr;
@@ -1282,6 +1302,7 @@ StageArgs<V> stage_args(qa_rk_solver *S, int i) {
     P.vtab = S->vtab;
     P.otab = S->otab;
     P.vbits = S->vbits;
+    P.uwords = S->uwords;
     return P;
 }
 
@@ -1290,15 +1311,15 @@ int stage_blocks(const qa_rk_solver *S) {
     return (int)((threads + kStepBlocks - 1) / kStepBlocks);
 }
 
-template <typename V, bool CODED>
+template <typename V, int FORM>
 int launch_sparse_stage(qa_rk_solver *S, int i) {
     StageArgs<V> P = stage_args<V>(S, i);
     const dim3 grid((unsigned)stage_blocks(S)), block(kStepBlocks);
     switch (S->lanes) {
-        case 1: hipLaunchKernelGGL((rk_stage_kernel<V, 1, CODED>), grid, block, 0, S->stream, P); break;
-        case 2: hipLaunchKernelGGL((rk_stage_kernel<V, 2, CODED>), grid, block, 0, S->stream, P); break;
-        case 4: hipLaunchKernelGGL((rk_stage_kernel<V, 4, CODED>), grid, block, 0, S->stream, P); break;
-        case 8: hipLaunchKernelGGL((rk_stage_kernel<V, 8, CODED>), grid, block, 0, S->stream, P); break;
+        case 1: hipLaunchKernelGGL((rk_stage_kernel<V, 1, FORM>), grid, block, 0, S->stream, P); break;
+        case 2: hipLaunchKernelGGL((rk_stage_kernel<V, 2, FORM>), grid, block, 0, S->stream, P); break;
+        case 4: hipLaunchKernelGGL((rk_stage_kernel<V, 4, FORM>), grid, block, 0, S->stream, P); break;
+        case 8: hipLaunchKernelGGL((rk_stage_kernel<V, 8, FORM>), grid, block, 0, S->stream, P); break;
         default:
             qa::set_error("lanes per row must be 1, 2, 4 or 8");
             return QA_ERR_ARG;
@@ -1334,9 +1355,11 @@ int launch_dense_stage(qa_rk_solver *S, int i) {
 
 int launch_stage(qa_rk_solver *S, int i) {
     if (S->kind == 1) return launch_dense_stage(S, i);
-    if (S->coded) return launch_sparse_stage<double2, true>(S, i);
-    return S->vals_fp32 ? launch_sparse_stage<float2, false>(S, i)
-                        : launch_sparse_stage<double2, false>(S, i);
+    if (S->coded)
+        return S->uwords ? launch_sparse_stage<double2, FORM_CODED_UNIFORM>(S, i)
+                         : launch_sparse_stage<double2, FORM_CODED>(S, i);
+    return S->vals_fp32 ? launch_sparse_stage<float2, FORM_PLAIN>(S, i)
+                        : launch_sparse_stage<double2, FORM_PLAIN>(S, i);
 }
 
 // out = A x (ungated): the derivative for the first-step estimate.
@@ -1660,6 +1683,21 @@ int qa_rk_create_coded(qa_rk_solver **out, int n, int lanes, int nslices, const 
     S->vtab = (const double2 *)vtab;
     S->otab = otab;
     S->vbits = vbits;
+    // Slices of one width (words per lane <= kUWordsMax) take the kernel form
+    // that requests its code words without the slice pointers.
+    {
+        std::vector<int> sp(nslices + 1);
+        hipError_t he = hipMemcpy(sp.data(), slice_ptr, sizeof(int) * (nslices + 1),
+                                  hipMemcpyDeviceToHost);
+        if (he != hipSuccess) {
+            delete S;
+            return qa::hip_fail(he, "hipMemcpy(slice_ptr)", __FILE__, __LINE__);
+        }
+        const int w = sp[1] - sp[0];
+        bool uniform = w > 0 && w % QA_WAVE == 0 && w / QA_WAVE <= kUWordsMax;
+        for (int k = 0; uniform && k <= nslices; ++k) uniform = sp[k] == k * w;
+        S->uwords = uniform ? w / QA_WAVE : 0;
+    }
     rc = rk_finish_create(S, order, rk_step, rk_extra, dorder, a, b, c, e, bi, rtol, atol, nsteps,
                           first_step, min_step, max_step, interpolate);
     if (rc) return rc;

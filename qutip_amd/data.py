@@ -349,7 +349,11 @@ def code_split(nvals, noffs):
     return None
 
 
-def sell_coded_arrays(A, lanes=1, fp32=False):
+UNIFORM_WORDS_MAX = 6     # words per lane of the uniform-width coded form
+UNIFORM_PAD_MAX = 1.25    # uniform when it stores at most this many times the words
+
+
+def sell_coded_arrays(A, lanes=1, fp32=False, uniform="auto"):
     """Coded SELL-C layout of a CSR matrix, or None when its distinct values
     and distinct column offsets (column - row) fit no 16-bit split
     (``code_split``: up to 1024 values with 64 offsets ... 64 values with
@@ -363,6 +367,12 @@ def sell_coded_arrays(A, lanes=1, fp32=False):
     are the four codes of word ``slice_ptr[s] + 64 w + lane`` (little-endian,
     step 4w in the low half of the low 32 bits), so ``slice_ptr`` counts
     8-byte words.  Padding entries are value 0 at offset 0.
+
+    ``uniform``: give every slice the widest slice's width (then slice s
+    starts at word s * 64 * words-per-lane, which the stage kernel uses to
+    request its codes without reading slice_ptr); "auto" does so when a
+    lane holds at most 6 words and the padding stores at most 1.25 times
+    the words.
 
     Returns (slice_ptr int32, codes uint16, vtab complex128[2^vbits],
     otab int32[2^(16 - vbits)], vbits, npad)."""
@@ -399,6 +409,12 @@ def sell_coded_arrays(A, lanes=1, fp32=False):
     width = rnp.reshape(ns, C).max(axis=1)
     quantum = 4 * lanes
     width = (width + quantum - 1) // quantum * quantum
+    if uniform == "auto":
+        wmax = int(width.max()) if len(width) else 0
+        uniform = (wmax // lanes // 4 <= UNIFORM_WORDS_MAX
+                   and wmax * len(width) <= UNIFORM_PAD_MAX * int(width.sum()))
+    if uniform and len(width):
+        width = np.full_like(width, width.max())
     words = width * C // 4
     slice_ptr = np.zeros(ns + 1, dtype=np.int64)
     slice_ptr[1:] = np.cumsum(words)
@@ -419,13 +435,13 @@ class SellMatrix:
     ``coded=True`` the coded form (sell_coded_arrays; ValueError when the
     matrix does not fit the 256-entry tables)."""
 
-    def __init__(self, A, fp32=False, lanes=1, coded=False):
+    def __init__(self, A, fp32=False, lanes=1, coded=False, uniform="auto"):
         self.shape = A.shape
         self.lanes = lanes
         self.fp32 = fp32
         self.coded = bool(coded)
         if self.coded:
-            arrs = sell_coded_arrays(A, lanes, fp32)
+            arrs = sell_coded_arrays(A, lanes, fp32, uniform)
             if arrs is None:
                 raise ValueError("the operator's distinct values and column offsets fit no "
                                  "16-bit code split: no coded SELL-C form")
@@ -433,6 +449,7 @@ class SellMatrix:
             self.nslices = len(sptr) - 1
             self.npad = npad
             self.stored = 4 * int(sptr[-1])
+            self.uniform = len(set(np.diff(sptr).tolist())) == 1
             self._sptr = DeviceBuffer.from_array(sptr)
             self._cols = DeviceBuffer.from_array(codes)
             self._vals = DeviceBuffer.from_array(vtab)

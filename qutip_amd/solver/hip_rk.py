@@ -63,13 +63,14 @@ class RKEngine:
 
     def __init__(self, op, tableau="vern7", rtol=1e-6, atol=1e-8, nsteps=1000, first_step=0.0,
                  min_step=0.0, max_step=0.0, interpolate=True, fp32=False, lanes=None,
-                 dense=None, dense_precision="fp32", coded="auto"):
+                 dense=None, dense_precision="fp32", coded="auto", uniform="auto"):
         """``op``: the sparse generator A (dy/dt = A y), or None with
         ``dense=(H_nh, [c_ops])`` (numpy arrays) for the matrix-form
         Lindblad right-hand side on MFMA.  ``coded``: store A as coded
         SELL-C (2-byte codes into 256-entry value and column-offset tables,
         the same result bits) -- "auto" when A fits the tables, True to
-        require it, False for the plain form."""
+        require it, False for the plain form; ``uniform``: the coded slices'
+        layout (see data.sell_coded_arrays)."""
         tab = TABLEAUS[tableau] if isinstance(tableau, str) else tableau
         self.tableau = tab
         self._a = np.ascontiguousarray(tab["a"], dtype=np.float64)
@@ -118,7 +119,7 @@ class RKEngine:
                 lanes = autotune.lanes_for(A, fp32, coded)
             self.lanes = lanes
             self.coded = coded
-            self.sell = _data.SellMatrix(A, fp32=fp32, lanes=lanes, coded=coded)
+            self.sell = _data.SellMatrix(A, fp32=fp32, lanes=lanes, coded=coded, uniform=uniform)
             sptr, cols, vals = self.sell.ptrs
             if coded:
                 rc = L.qa_rk_create_coded(

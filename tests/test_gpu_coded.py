@@ -49,8 +49,9 @@ def test_coded_matvec_equals_plain(name, lanes):
 @pytest.mark.parametrize("lanes", [1, 4])
 def test_coded_solve_equals_plain_and_oracle(lanes):
     """Dissipative Ising L=5 Liouvillian (1024 x 1024) through the fused
-    engine: coded and plain give the same functional and state bits, the
-    oracle's attempt count, and <O>(t) within 1e-6 of expm."""
+    engine: plain, coded with per-slice widths and coded with one width for
+    all slices give the same functional and state bits, the oracle's attempt
+    count, and <O>(t) within 1e-6 of expm."""
     print(f"coded solve Ising L=5 lanes={lanes}", flush=True)
     m = models.ising(5, 0.1)
     Lm = models.liouvillian(m["H"], m["c_ops"]).tocsr()
@@ -59,9 +60,13 @@ def test_coded_solve_equals_plain_and_oracle(lanes):
     tl = np.linspace(0.0, 2.0, 21)
     fptr, fidx, fw = cbind.functionals_from_ops(m["e_ops"], d)
     res = {}
-    for coded in (False, True):
-        eng = RKEngine(Lm, "vern7", 1e-8, 1e-10, lanes=lanes, coded=coded)
-        assert eng.coded == coded
+    for form in ("plain", "sliced", "uniform"):
+        eng = RKEngine(Lm, "vern7", 1e-8, 1e-10, lanes=lanes, coded=form != "plain",
+                       uniform=form == "uniform")
+        if form == "uniform":
+            assert eng.sell.uniform
+        if form == "sliced" and lanes == 1:  # (at 4 lanes every slice has one width anyway)
+            assert not eng.sell.uniform
         eng.set_functionals(np.zeros(len(fptr) - 1, np.int32), fptr.astype(np.int32),
                             np.zeros(len(fidx), np.int32), fidx.astype(np.int32),
                             fw.view(np.complex128))
@@ -69,15 +74,16 @@ def test_coded_solve_equals_plain_and_oracle(lanes):
         vals, st, _ = eng.run(tl[1:])
         assert st >= 0
         _, y = eng.get_state()
-        res[coded] = (vals, y.to_array().ravel(), eng.stats())
-    assert np.array_equal(res[False][0], res[True][0])
-    assert np.array_equal(res[False][1], res[True][1])
-    assert res[False][2]["attempts"] == res[True][2]["attempts"]
+        res[form] = (vals, y.to_array().ravel(), eng.stats())
+    for form in ("sliced", "uniform"):
+        assert np.array_equal(res["plain"][0], res[form][0]), form
+        assert np.array_equal(res["plain"][1], res[form][1]), form
+        assert res["plain"][2]["attempts"] == res[form][2]["attempts"], form
     _, _, ost = cbind.rk_solve(Lm, rho0, tl, tableaus.vern7, (fptr, fidx, fw), rtol=1e-8,
                                atol=1e-10)
-    assert res[True][2]["attempts"] == ost["n_attempts"]
+    assert res["uniform"][2]["attempts"] == ost["n_attempts"]
     ref = exact.mesolve_expect(m["H"], m["c_ops"], m["psi0"], tl, m["e_ops"])
-    vals = res[True][0]
+    vals = res["uniform"][0]
     got = (vals[:, :-1] / vals[:, -1:]).real
     assert np.abs(got.T - ref[:, 1:]).max() < 1e-6
 

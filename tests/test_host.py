@@ -166,12 +166,14 @@ def test_coded_layout_reproduces_csr(lanes):
     band = sp.diags([rng.choice([0.5, -1j, 2.0], n - k) for k in (0, 1, 7, 130)],
                     [0, 1, -7, 130], shape=(n, n), format="csr")
     band.data[::17] = 0.0  # explicit zeros are entries like any other
-    for A in (Lm, band):
-        arrs = qdata.sell_coded_arrays(A, lanes)
+    for A, uniform in ((Lm, "auto"), (Lm, True), (Lm, False), (band, "auto"), (band, True)):
+        arrs = qdata.sell_coded_arrays(A, lanes, uniform=uniform)
         assert arrs is not None
         sptr, codes, vtab, otab, vbits, npad = arrs
         assert np.all(np.diff(sptr) % 64 == 0)
         assert len(vtab) == 1 << vbits and len(otab) == 1 << (16 - vbits)
+        if uniform is True:
+            assert len(set(np.diff(sptr).tolist())) == 1
         x = rng.standard_normal(A.shape[1]) + 1j * rng.standard_normal(A.shape[1])
         assert_allclose(_sell_coded_matvec(sptr, codes, vtab, otab, vbits, A.shape, x, lanes),
                         A @ x, atol=1e-12)
