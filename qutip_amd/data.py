@@ -353,6 +353,36 @@ UNIFORM_WORDS_MAX = 6     # words per lane of the uniform-width coded form
 UNIFORM_PAD_MAX = 1.25    # uniform when it stores at most this many times the words
 
 
+def code_dictionary(A, fp32=False):
+    """The value and column-offset dictionaries of a sorted CSR matrix:
+    (value index per entry, offset index per entry, vtab complex128[2^vbits],
+    otab int32[2^(16 - vbits)], vbits, pad code) or None when no 16-bit
+    split fits.  Values are told apart by their bits (-0.0 and 0.0 stay
+    distinct; ``fp32``: rounded to complex64 first); value 0 and offset 0
+    are always present (the padding entry, code ``pad``)."""
+    n = A.shape[0]
+    vals = A.data.astype(np.complex128)
+    if fp32:
+        vals = vals.astype(np.complex64).astype(np.complex128)
+    bits = np.concatenate([np.zeros((1, 2), np.uint64), vals.view(np.uint64).reshape(-1, 2)])
+    ubits, vinv = np.unique(bits, axis=0, return_inverse=True)
+    rn = np.diff(A.indptr).astype(np.int64)
+    rows = np.repeat(np.arange(n, dtype=np.int64), rn)
+    offs = np.concatenate([[0], A.indices.astype(np.int64) - rows])
+    uoffs, oinv = np.unique(offs, return_inverse=True)
+    vbits = code_split(len(ubits), len(uoffs))
+    if vbits is None:
+        return None
+    vinv = vinv.reshape(-1).astype(np.int64)
+    oinv = oinv.reshape(-1).astype(np.int64)
+    vtab = np.zeros(1 << vbits, dtype=np.complex128)
+    vtab[:len(ubits)] = np.ascontiguousarray(ubits).view(np.complex128).reshape(-1)
+    otab = np.zeros(1 << (16 - vbits), dtype=np.int32)
+    otab[:len(uoffs)] = uoffs
+    pad = int(vinv[0]) | (int(oinv[0]) << vbits)
+    return vinv[1:], oinv[1:], vtab, otab, vbits, pad
+
+
 def sell_coded_arrays(A, lanes=1, fp32=False, uniform="auto"):
     """Coded SELL-C layout of a CSR matrix, or None when its distinct values
     and distinct column offsets (column - row) fit no 16-bit split
@@ -381,26 +411,12 @@ def sell_coded_arrays(A, lanes=1, fp32=False, uniform="auto"):
     A = sp.csr_matrix(A)
     A.sort_indices()
     n = A.shape[0]
-    vals = A.data.astype(np.complex128)
-    if fp32:
-        vals = vals.astype(np.complex64).astype(np.complex128)
-    # distinct values by their bits (-0.0 and 0.0 stay apart), value 0 first
-    bits = np.concatenate([np.zeros((1, 2), np.uint64), vals.view(np.uint64).reshape(-1, 2)])
-    ubits, vinv = np.unique(bits, axis=0, return_inverse=True)
+    dic = code_dictionary(A, fp32)
+    if dic is None:
+        return None
+    vinv, oinv, vtab, otab, vbits, pad_code = dic
     rn = np.diff(A.indptr).astype(np.int64)
     rows = np.repeat(np.arange(n, dtype=np.int64), rn)
-    offs = np.concatenate([[0], A.indices.astype(np.int64) - rows])
-    uoffs, oinv = np.unique(offs, return_inverse=True)
-    vbits = code_split(len(ubits), len(uoffs))
-    if vbits is None:
-        return None
-    vinv = vinv.reshape(-1).astype(np.int64)
-    oinv = oinv.reshape(-1).astype(np.int64)
-    vtab = np.zeros(1 << vbits, dtype=np.complex128)
-    vtab[:len(ubits)] = np.ascontiguousarray(ubits).view(np.complex128).reshape(-1)
-    otab = np.zeros(1 << (16 - vbits), dtype=np.int32)
-    otab[:len(uoffs)] = uoffs
-    pad_code = int(vinv[0]) | (int(oinv[0]) << vbits)
     C = 64 // lanes
     ns = (n + C - 1) // C
     npad = ns * C
@@ -426,7 +442,7 @@ def sell_coded_arrays(A, lanes=1, fp32=False, uniform="auto"):
     u, q = j // lanes, j % lanes
     lane = (rows % C) * lanes + q
     word = slice_ptr[rows // C] + (u // 4) * 64 + lane
-    codes[4 * word + u % 4] = (vinv[1:] | (oinv[1:] << vbits)).astype(np.uint16)
+    codes[4 * word + u % 4] = (vinv | (oinv << vbits)).astype(np.uint16)
     return slice_ptr.astype(np.int32), codes, vtab, otab, vbits, npad
 
 
