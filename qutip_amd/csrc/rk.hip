@@ -964,7 +964,10 @@ struct OutArgs {
     double2 *partials;  // [kMaxPending][nfun][nchunks]
     int nchunks;
     double2 *y_last;
-    double bi[kMaxStages * kMaxDenseOrder];  // dense-output table, [i][j] at i * dorder + j
+    // dense-output table in HBM, [i][j] at i * kMaxDenseOrder + j (a device
+    // array: a per-thread index into a kernel-argument array becomes one
+    // dependent kernarg load per element)
+    const double *bi;
 };
 
 // Number of pending outputs (output times inside [t_prev, t_front]) and
@@ -982,15 +985,22 @@ __device__ int pending_outputs(const RKCtl *c, const OutArgs &A, double (*w)[kMa
         const int idx = c->out_idx + q;
         const double t_front = c->t_front, t_prev = c->t_prev, dt_int = c->dt_int;
         const bool in_range = idx < c->ntimes;
+        // this thread's row of the dense-output table, every load issued at
+        // once (they do not depend on the control block)
+        double bv[kMaxDenseOrder];
+#pragma unroll
+        for (int j = 0; j < kMaxDenseOrder; ++j) bv[j] = A.bi[i * kMaxDenseOrder + j];
         const double tq = in_range ? c->times[idx] : 0.0;
         const bool ok = in_range && tq <= t_front;
         double v = 0.0;
         if (ok && tq != t_front && A.has_bi && i < A.nk) {
             const double tau = (tq - t_prev) / dt_int;
-            for (int j = A.dorder - 1; j >= 0; --j) {
-                v += A.bi[i * A.dorder + j];
-                v *= tau;
-            }
+#pragma unroll
+            for (int j = kMaxDenseOrder - 1; j >= 0; --j)
+                if (j < A.dorder) {  // Horner from j = dorder - 1 down
+                    v += bv[j];
+                    v *= tau;
+                }
             v *= dt_int;
         }
         w[q][i] = v;
@@ -1404,8 +1414,7 @@ OutArgs out_args(qa_rk_solver *S) {
     A.nk = S->interpolate ? S->rk_extra : S->rk_step;
     A.dorder = S->dorder;
     A.has_bi = S->interpolate ? 1 : 0;
-    for (int k = 0; k < kMaxStages * kMaxDenseOrder; ++k)
-        A.bi[k] = (S->interpolate && k < (int)S->bi.size()) ? S->bi[k] : 0.0;
+    A.bi = S->bi_dev;
     A.nfun = S->nfun;
     A.fptr = S->fptr_dev;
     A.fkind = S->fkind_dev;
@@ -1579,6 +1588,19 @@ int rk_finish_create(qa_rk_solver *S, int order, int rk_step, int rk_extra, int 
     if (rc) return fail(rc);
     he = hipMalloc((void **)&S->ctl, sizeof(RKCtl));
     if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMalloc(ctl)", __FILE__, __LINE__));
+    {
+        // dense-output table, rows padded to kMaxDenseOrder (zeros without
+        // interpolation: the output kernels then never read it)
+        std::vector<double> bt((size_t)kMaxStages * kMaxDenseOrder, 0.0);
+        if (S->interpolate)
+            for (int i = 0; i < S->rk_extra && i < kMaxStages; ++i)
+                for (int j = 0; j < S->dorder && j < kMaxDenseOrder; ++j)
+                    bt[(size_t)i * kMaxDenseOrder + j] = S->bi[(size_t)i * S->dorder + j];
+        he = hipMalloc((void **)&S->bi_dev, sizeof(double) * bt.size());
+        if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMalloc(bi)", __FILE__, __LINE__));
+        he = hipMemcpy(S->bi_dev, bt.data(), sizeof(double) * bt.size(), hipMemcpyHostToDevice);
+        if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMemcpy(bi)", __FILE__, __LINE__));
+    }
     he = hipHostMalloc((void **)&S->ctl_host, sizeof(RKCtl), hipHostMallocDefault);
     if (he != hipSuccess) return fail(qa::hip_fail(he, "hipHostMalloc", __FILE__, __LINE__));
     RKCtl init;
