@@ -185,6 +185,22 @@ int qa_rk_create_coded(qa_rk_solver **out, int n, int lanes, int nslices, const 
                        const double *b, const double *c, const double *e, const double *bi,
                        double rtol, double atol, int nsteps, double first_step,
                        double min_step, double max_step, int interpolate);
+/* The matrix-form engine (as qa_rk_create_dense) for banded operators, in
+ * FP64: drho/dt = -i H_nh rho + i rho H_nh^dag + sum_c c rho c^dag computed
+ * from the diagonals of H_nh and of each c (Dia products:
+ * lindblad_matrix_form.pyx:105-203 over matmul_dia_dense_dense,
+ * matmul.pyx:429-507).  ndiag diagonals (host arrays): diagonal d belongs to
+ * operator diag_op[d] (0 = H_nh, 1 + k = c_k, k < nc), has offset
+ * diag_off[d] (|offset| < N) and holds A[r, r + offset] at diag_vals[d][r]
+ * (complex128, N entries; entries outside the matrix are ignored).  The
+ * products of H_nh's diagonals (2 per diagonal) and of each c's diagonals
+ * (pairs) may number at most 96.  rho: N x N complex128 column-major, any
+ * N >= 1. */
+int qa_rk_create_dia(qa_rk_solver **out, int N, int nc, int ndiag, const int *diag_op,
+                     const int *diag_off, const double *diag_vals, int order, int rk_step,
+                     int rk_extra, int dorder, const double *a, const double *b, const double *c,
+                     const double *e, const double *bi, double rtol, double atol, int nsteps,
+                     double first_step, double min_step, double max_step, int interpolate);
 /* The same engine for the Lindblad equation in matrix form (mesolve with
  * options {"matrix_form": True}: qutip/core/cy/lindblad_matrix_form.pyx:
  * 105-203): the state is rho, N x N column-major (N a multiple of 16), and

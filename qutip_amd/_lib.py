@@ -66,6 +66,8 @@ SIGNATURES = {
                      P_f64, P_f64, P_f64, P_f64, P_f64, f64, f64, i32, f64, f64, f64, i32],
     "qa_rk_create_coded": [C.POINTER(vp), i32, i32, i32, vp, vp, vp, vp, i32, i32, i32, i32, i32,
                            P_f64, P_f64, P_f64, P_f64, P_f64, f64, f64, i32, f64, f64, f64, i32],
+    "qa_rk_create_dia": [C.POINTER(vp), i32, i32, i32, P_i32, P_i32, P_f64, i32, i32, i32, i32,
+                         P_f64, P_f64, P_f64, P_f64, P_f64, f64, f64, i32, f64, f64, f64, i32],
     "qa_rk_create_dense": [C.POINTER(vp), i32, i32, P_f64, P_f64, i32, i32, i32, i32, i32,
                            P_f64, P_f64, P_f64, P_f64, P_f64, f64, f64, i32, f64, f64, f64, i32],
     "qa_rk_apply": [vp, vp, vp],

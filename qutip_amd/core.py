@@ -409,13 +409,14 @@ class LindbladMatrixForm:
     lindblad_matrix_form.pyx:59-203): drho/dt = -i H_nh rho + i rho H_nh^dag
     + sum_c c rho c^dag with H_nh = H - i/2 sum_c c^dag c (:92-97).  The
     state is rho itself (N x N, not vectorised).  ``matmul_data`` runs the
-    MFMA product of the dense engine; the hip_rk integrator picks up
+    dense engine's product (MFMA, or FP64 from the diagonals of banded
+    operators: ``precision``); the hip_rk integrator picks up
     ``dense_operators`` and runs the whole step loop on the device."""
 
     issuper = False
     isconstant = True
 
-    def __init__(self, H, c_ops, precision="fp32"):
+    def __init__(self, H, c_ops, precision="auto"):
         if isinstance(H, QobjEvo):
             H = H.qobj
         self.H = H

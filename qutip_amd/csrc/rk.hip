@@ -365,6 +365,101 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
     }
 }
 
+// ---- banded (DIA) Lindblad right-hand side ----------------------------------
+// The matrix-form RHS k = Hm X + X Hp + sum_c C_c X Cd_c (Hm = -i H_nh,
+// Hp = i H_nh^dag, Cd_c = C_c^dag) for banded operators, as LindbladMatrix-
+// Form computes it with Dia products (lindblad_matrix_form.pyx:105-203 over
+// matmul_dia_dense_dense, matmul.pyx:429-507), in FP64.  Every product is a
+// list of terms acc += A[i, i+o] x[i+o, j-p] B[j-p, j]: a diagonal of Hm
+// (B = 1), of Hp (A = 1), or a pair of diagonals of C_c and Cd_c.  Values
+// are rows of `val` (row d: the diagonal's entries by row index; the last
+// row all ones).  One thread per element (i, j) of the column-major N x N
+// state, i fastest: the x and A loads of a wavefront are coalesced and its
+// B load is one value.  Terms go in chunks of at most eight, each a body of
+// exactly its length, so a chunk's loads are issued together; out-of-range
+// neighbours are clamped and their terms dropped by a select.
+constexpr int kDiaMaxTerms = 96;
+
+struct DiaTerm {
+    int o, p, a, b;
+};
+
+struct DiaOp {
+    int N, nterms;
+    const double2 *val;  // [rows][N]
+    DiaTerm t[kDiaMaxTerms];
+};
+
+template <int M>
+__device__ __forceinline__ void dia_terms(const DiaOp &D, const double2 *__restrict__ x, int i,
+                                          int j, int t0, double2 &acc) {
+    const int N = D.N;
+    double2 av[M], bv[M], xv[M];
+    bool ok[M];
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        const DiaTerm T = D.t[t0 + u];
+        const int ii = i + T.o, jj = j - T.p;
+        ok[u] = (unsigned)ii < (unsigned)N && (unsigned)jj < (unsigned)N;
+        const int iic = ok[u] ? ii : i, jjc = ok[u] ? jj : j;
+        av[u] = D.val[(int64_t)T.a * N + i];
+        bv[u] = D.val[(int64_t)T.b * N + jjc];
+        xv[u] = x[(int64_t)jjc * N + iic];
+    }
+#pragma unroll
+    for (int u = 0; u < M; ++u) {
+        const double2 t = cmul(cmul(av[u], xv[u]), bv[u]);
+        if (ok[u]) acc = cadd(acc, t);
+    }
+}
+
+__device__ __forceinline__ double2 dia_rhs_at(const DiaOp &D, const double2 *__restrict__ x,
+                                              int i, int j) {
+    double2 acc = make_double2(0.0, 0.0);
+    int t = 0;
+    for (; t + 8 <= D.nterms; t += 8) dia_terms<8>(D, x, i, j, t, acc);
+    switch (D.nterms - t) {
+        case 7: dia_terms<7>(D, x, i, j, t, acc); break;
+        case 6: dia_terms<6>(D, x, i, j, t, acc); break;
+        case 5: dia_terms<5>(D, x, i, j, t, acc); break;
+        case 4: dia_terms<4>(D, x, i, j, t, acc); break;
+        case 3: dia_terms<3>(D, x, i, j, t, acc); break;
+        case 2: dia_terms<2>(D, x, i, j, t, acc); break;
+        case 1: dia_terms<1>(D, x, i, j, t, acc); break;
+        default: break;
+    }
+    return acc;
+}
+
+// One RK stage of the banded matrix form: the product for element r plus
+// the stage epilogue (the sparse stage kernel's, element for element).
+__global__ __launch_bounds__(256) void rk_dia_stage_kernel(EpiArgs P, DiaOp D) {
+    RKCtl *ctl = P.ctl;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int phase = ctl->phase;
+    const StageVecs v = stage_vecs(P, ctl);
+    if (phase != P.run_phase) return;  // uniform over the whole grid
+    double part = 0.0;
+    if (r < P.nrows) {
+        double2 o, er;
+        epi_pre(P, v, r, o, er);
+        const double2 acc = dia_rhs_at(D, v.xin, (int)(r % D.N), (int)(r / D.N));
+        part = epi_post(P, ctl, v, r, acc, o, er);
+    }
+    if (P.epi == EPI_FINAL) {
+        __shared__ double smem[256 / QA_WAVE];
+        const double sum = block_sum(part, smem);
+        if (threadIdx.x == 0) P.partials[blockIdx.x] = sum;
+    }
+}
+
+// out = L(x) (ungated): the derivative for the first-step estimate and apply.
+__global__ __launch_bounds__(256) void dia_rhs_kernel(DiaOp D, const double2 *__restrict__ x,
+                                                      double2 *__restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < (int64_t)D.N * D.N) out[r] = dia_rhs_at(D, x, (int)(r % D.N), (int)(r / D.N));
+}
+
 // ---- dense Lindblad right-hand side on MFMA ----------------------------------
 // drho/dt = -i H_nh rho + i rho H_nh^dag + sum_c c rho c^dag with
 // H_nh = H - i/2 sum c^dag c (qutip/core/cy/lindblad_matrix_form.pyx:92-97,
@@ -1181,7 +1276,8 @@ __global__ void rk_set_phase_kernel(RKCtl *c, int phase) { c->phase = phase; }
 // host-side engine object
 // ---------------------------------------------------------------------------
 struct qa_rk_solver {
-    int kind = 0;  // 0: sparse SELL-C generator, 1: dense Lindblad (matrix form)
+    int kind = 0;  // 0: sparse SELL-C generator, 1: dense Lindblad (matrix form), 2: banded
+    DiaOp dia = {};  // kind 2: term list and the diagonals in HBM (dense_block)
     // buffers replaced while the engine lives; freed with it (hipFree waits
     // for the whole device, which would stall other engines' streams)
     std::vector<void *> retired;
@@ -1365,6 +1461,14 @@ int launch_dense_stage(qa_rk_solver *S, int i) {
 
 int launch_stage(qa_rk_solver *S, int i) {
     if (S->kind == 1) return launch_dense_stage(S, i);
+    if (S->kind == 2) {
+        EpiArgs P;
+        fill_epi(P, S, i);
+        hipLaunchKernelGGL(rk_dia_stage_kernel, dim3((unsigned)S->nparts), dim3(256), 0, S->stream,
+                           P, S->dia);
+        QA_CHECK_LAUNCH();
+        return QA_OK;
+    }
     if (S->coded)
         return S->uwords ? launch_sparse_stage<double2, FORM_CODED_UNIFORM>(S, i)
                          : launch_sparse_stage<double2, FORM_CODED>(S, i);
@@ -1374,6 +1478,12 @@ int launch_stage(qa_rk_solver *S, int i) {
 
 // out = A x (ungated): the derivative for the first-step estimate.
 int engine_rhs(qa_rk_solver *S, const double2 *x, double2 *out) {
+    if (S->kind == 2) {
+        hipLaunchKernelGGL(dia_rhs_kernel, dim3((unsigned)S->nparts), dim3(256), 0, S->stream,
+                           S->dia, x, out);
+        QA_CHECK_LAUNCH();
+        return QA_OK;
+    }
     if (S->kind == 1 && S->dop.tiled) {
         const int nt = S->dop.N / kGT;
         hipLaunchKernelGGL(dense_t_tiled_kernel, dim3(nt, nt, std::max(S->dop.nc, 1)),
@@ -1720,6 +1830,100 @@ int qa_rk_create_coded(qa_rk_solver **out, int n, int lanes, int nslices, const 
         for (int k = 0; uniform && k <= nslices; ++k) uniform = sp[k] == k * w;
         S->uwords = uniform ? w / QA_WAVE : 0;
     }
+    rc = rk_finish_create(S, order, rk_step, rk_extra, dorder, a, b, c, e, bi, rtol, atol, nsteps,
+                          first_step, min_step, max_step, interpolate);
+    if (rc) return rc;
+    *out = S;
+    return QA_OK;
+}
+
+int qa_rk_create_dia(qa_rk_solver **out, int N, int nc, int ndiag, const int *diag_op,
+                     const int *diag_off, const double *diag_vals, int order, int rk_step,
+                     int rk_extra, int dorder, const double *a, const double *b, const double *c,
+                     const double *e, const double *bi, double rtol, double atol, int nsteps,
+                     double first_step, double min_step, double max_step, int interpolate) {
+    QA_REQUIRE(out != nullptr, "qa_rk_create_dia: null output");
+    *out = nullptr;
+    QA_REQUIRE(N >= 1 && (int64_t)N * N < (1LL << 31), "qa_rk_create_dia: 1 <= N, N^2 < 2^31");
+    QA_REQUIRE(nc >= 0 && ndiag >= 1 && diag_op && diag_off && diag_vals,
+               "qa_rk_create_dia: operators");
+    for (int d = 0; d < ndiag; ++d)
+        QA_REQUIRE(diag_op[d] >= 0 && diag_op[d] <= nc && diag_off[d] > -N && diag_off[d] < N,
+                   "qa_rk_create_dia: diagonal %d: operator index in [0, nc], |offset| < N", d);
+    int rc = check_tableau(rk_step, rk_extra, dorder, a, b, c, bi, nsteps);
+    if (rc) return rc;
+    // value rows: Hm = -i H_nh and Hp = i H_nh^dag (one row per H_nh
+    // diagonal each), C_c and Cd_c = C_c^dag (one row per c diagonal each),
+    // then a row of ones.  Operator 0 is H_nh, 1 + c is c_c; diagonal d holds
+    // A[r, r + off] at diag_vals[d][r].
+    const size_t Nz = (size_t)N;
+    auto src = [&](int d, int r) {  // the entry of diagonal d in row r (0 outside)
+        const int col = r + diag_off[d];
+        if (r < 0 || r >= N || col < 0 || col >= N) return make_double2(0.0, 0.0);
+        return make_double2(diag_vals[2 * ((size_t)d * Nz + r)], diag_vals[2 * ((size_t)d * Nz + r) + 1]);
+    };
+    std::vector<double2> rows;
+    std::vector<DiaTerm> terms;
+    std::vector<int> hm_row(ndiag, -1), hp_row(ndiag, -1), c_row(ndiag, -1), cd_row(ndiag, -1);
+    for (int d = 0; d < ndiag; ++d) {
+        const int o = diag_off[d];
+        if (diag_op[d] == 0) {
+            hm_row[d] = (int)(rows.size() / Nz);
+            for (int r = 0; r < N; ++r) {  // -i h
+                const double2 h = src(d, r);
+                rows.push_back(make_double2(h.y, -h.x));
+            }
+            hp_row[d] = (int)(rows.size() / Nz);
+            for (int r = 0; r < N; ++r) {  // Hp[r, r - o] = i conj(H[r - o, r])
+                const double2 h = src(d, r - o);
+                rows.push_back(make_double2(h.y, h.x));
+            }
+        } else {
+            c_row[d] = (int)(rows.size() / Nz);
+            for (int r = 0; r < N; ++r) rows.push_back(src(d, r));
+            cd_row[d] = (int)(rows.size() / Nz);
+            for (int r = 0; r < N; ++r) {  // Cd[r, r - o] = conj(C[r - o, r])
+                const double2 cv = src(d, r - o);
+                rows.push_back(make_double2(cv.x, -cv.y));
+            }
+        }
+    }
+    const int ones = (int)(rows.size() / Nz);
+    for (int r = 0; r < N; ++r) rows.push_back(make_double2(1.0, 0.0));
+    for (int d = 0; d < ndiag; ++d)
+        if (diag_op[d] == 0) {
+            terms.push_back(DiaTerm{diag_off[d], 0, hm_row[d], ones});   // Hm X
+            terms.push_back(DiaTerm{0, -diag_off[d], ones, hp_row[d]});  // X Hp
+        }
+    for (int k = 1; k <= nc; ++k)
+        for (int d1 = 0; d1 < ndiag; ++d1)
+            if (diag_op[d1] == k)
+                for (int d2 = 0; d2 < ndiag; ++d2)
+                    if (diag_op[d2] == k)  // C_k[i, i + o1] X Cd_k[j - p2, j], p2 = -o2
+                        terms.push_back(DiaTerm{diag_off[d1], -diag_off[d2], c_row[d1], cd_row[d2]});
+    QA_REQUIRE((int)terms.size() <= kDiaMaxTerms,
+               "qa_rk_create_dia: %d products of diagonals, at most %d", (int)terms.size(),
+               kDiaMaxTerms);
+    qa_rk_solver *S = new (std::nothrow) qa_rk_solver();
+    QA_REQUIRE(S != nullptr, "qa_rk_create_dia: out of host memory");
+    S->kind = 2;
+    S->n = N * N;
+    S->npad = (int64_t)N * N;
+    S->nparts = (int)(((int64_t)N * N + 255) / 256);
+    hipError_t he = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipMalloc(&S->dense_block, sizeof(double2) * rows.size());
+    if (he == hipSuccess)
+        he = hipMemcpy(S->dense_block, rows.data(), sizeof(double2) * rows.size(),
+                       hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+        free_all(S);
+        delete S;
+        return qa::hip_fail(he, "banded operator upload", __FILE__, __LINE__);
+    }
+    S->dia.N = N;
+    S->dia.nterms = (int)terms.size();
+    S->dia.val = (const double2 *)S->dense_block;
+    for (size_t k = 0; k < terms.size(); ++k) S->dia.t[k] = terms[k];
     rc = rk_finish_create(S, order, rk_step, rk_extra, dorder, a, b, c, e, bi, rtol, atol, nsteps,
                           first_step, min_step, max_step, interpolate);
     if (rc) return rc;

@@ -19,8 +19,10 @@ values, FP64 state, accumulation and error norm), ``operator_format``
 ('auto' | 'coded' | 'sell': the fused engine's operator as 2-byte codes into
 value and column-offset tables when it has at most 256 of each -- the same
 result bits as the plain SELL-C values and columns, a tenth of the bytes)
-and ``dense_precision`` ('fp32' | 'bf16x3': the MFMA products of the
-matrix-form Lindblad RHS).
+and ``dense_precision`` ('auto' | 'fp32' | 'bf16x3' | 'fp64_banded': the
+matrix-form Lindblad RHS as MFMA products, or in FP64 from the operators'
+diagonals; 'auto' takes the banded form when H_nh and the collapse
+operators have few diagonals).
 """
 import ctypes as C
 
@@ -58,6 +60,36 @@ def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
+DIA_MAX_TERMS = 96  # products of diagonals the banded kernel takes (kDiaMaxTerms)
+
+
+def banded_operators(Hnh, cops):
+    """The diagonals of H_nh and of each collapse operator for the banded
+    matrix-form kernel (qa_rk_create_dia): (diag_op int32, diag_off int32,
+    diag_vals complex128 [ndiag, N], row-indexed: A[r, r + off] at [d, r]),
+    or None when the products of diagonals (2 per H_nh diagonal, the square
+    of each c's count) exceed DIA_MAX_TERMS or there is nothing to multiply."""
+    Hnh = np.asarray(Hnh, dtype=np.complex128)
+    N = Hnh.shape[0]
+    ops, offs, vals = [], [], []
+    nterms = 0
+    for k, A in enumerate([Hnh] + [np.asarray(c, dtype=np.complex128) for c in cops]):
+        r, c = np.nonzero(A)
+        od = np.unique(c - r)
+        nterms += 2 * len(od) if k == 0 else len(od) ** 2
+        for o in od:
+            v = np.zeros(N, dtype=np.complex128)
+            rr = np.arange(max(0, -o), min(N, N - o))
+            v[rr] = A[rr, rr + o]
+            ops.append(k)
+            offs.append(int(o))
+            vals.append(v)
+    if not ops or nterms > DIA_MAX_TERMS:
+        return None
+    return (np.array(ops, dtype=np.int32), np.array(offs, dtype=np.int32),
+            np.ascontiguousarray(np.array(vals)))
+
+
 class RKEngine:
     """Owner of one ``qa_rk_solver`` (fused device RK for dy/dt = A y)."""
 
@@ -66,7 +98,9 @@ class RKEngine:
                  dense=None, dense_precision="fp32", coded="auto", uniform="auto"):
         """``op``: the sparse generator A (dy/dt = A y), or None with
         ``dense=(H_nh, [c_ops])`` (numpy arrays) for the matrix-form
-        Lindblad right-hand side on MFMA.  ``coded``: store A as coded
+        Lindblad right-hand side: on MFMA (``dense_precision`` 'fp32' or
+        'bf16x3'), or from the operators' diagonals in FP64 ('fp64_banded';
+        'auto' takes it when the operators are banded, else 'fp32').  ``coded``: store A as coded
         SELL-C (2-byte codes into 256-entry value and column-offset tables,
         the same result bits) -- "auto" when A fits the tables, True to
         require it, False for the plain form; ``uniform``: the coded slices'
@@ -94,7 +128,21 @@ class RKEngine:
             self.lanes = 0
             self.sell = None
             self.coded = False
+            band = None
+            if dense_precision in ("auto", "fp64_banded"):
+                band = banded_operators(Hnh, cops)
+                if band is None and dense_precision == "fp64_banded":
+                    raise ValueError("the operators are not banded enough for the banded "
+                                     f"kernel (at most {DIA_MAX_TERMS} products of diagonals)")
+                dense_precision = "fp64_banded" if band is not None else "fp32"
             self.dense_precision = dense_precision
+        if dense is not None and band is not None:
+            dop, doff, dval = band
+            self._dense_host = band
+            rc = L.qa_rk_create_dia(C.byref(h), N, len(cops), len(dop), _ip(dop), _ip(doff),
+                                    _dp(dval.view(np.float64)), *tab_args)
+            _lib.check(rc, "qa_rk_create_dia")
+        elif dense is not None:
             Hf = np.asfortranarray(Hnh, dtype=np.complex128)
             cf = np.concatenate([np.asfortranarray(c, dtype=np.complex128).ravel(order="F")
                                  for c in cops]) if cops else np.zeros(1, np.complex128)
@@ -425,7 +473,7 @@ class IntegratorHipRK(Integrator):
         "fused": True,
         "operator_precision": "fp64",
         "operator_format": "auto",
-        "dense_precision": "fp32",
+        "dense_precision": "auto",
     }
     method = "hip_rk"
     name = "hip_rk"
@@ -446,7 +494,9 @@ class IntegratorHipRK(Integrator):
                                     o["first_step"], o["min_step"], o["max_step"],
                                     o["interpolate"], dense=dense(),
                                     dense_precision=o["dense_precision"])
-            self.name = f"hip_rk (fused, matrix form, {o['dense_precision']} MFMA)"
+            dp = self._engine.dense_precision
+            self.name = ("hip_rk (fused, matrix form, FP64 banded)" if dp == "fp64_banded"
+                         else f"hip_rk (fused, matrix form, {dp} MFMA)")
         elif op is not None:
             self._engine = RKEngine(op, o["tableau"], o["rtol"], o["atol"], o["nsteps"],
                                     o["first_step"], o["min_step"], o["max_step"],
@@ -582,5 +632,5 @@ class IntegratorHipRK(Integrator):
             raise ValueError("operator_precision must be 'fp64' or 'fp32'")
         if self._options.get("operator_format") not in ("auto", "coded", "sell"):
             raise ValueError("operator_format must be 'auto', 'coded' or 'sell'")
-        if self._options.get("dense_precision") not in ("fp32", "bf16x3"):
-            raise ValueError("dense_precision must be 'fp32' or 'bf16x3'")
+        if self._options.get("dense_precision") not in ("auto", "fp32", "bf16x3", "fp64_banded"):
+            raise ValueError("dense_precision must be 'auto', 'fp32', 'bf16x3' or 'fp64_banded'")

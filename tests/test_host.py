@@ -212,6 +212,27 @@ def test_coded_layout_limits():
         qdata.SellMatrix(A, coded=True)
 
 
+def test_banded_operators_reproduce_the_matrices():
+    """The diagonals handed to the banded matrix-form kernel: row-indexed
+    (A[r, r + o] at [d, r]), one operator index per diagonal, and None past
+    96 products of diagonals."""
+    from qutip_amd.solver.hip_rk import banded_operators
+    rng = np.random.default_rng(5)
+    N = 21
+    H = np.diag(rng.standard_normal(N)) + np.diag(rng.standard_normal(N - 2) + 1j, 2)
+    c = np.diag(np.sqrt(np.arange(1, N)), 1) + np.diag(np.ones(N - 5), -5)
+    dop, doff, dval = banded_operators(H, [c])
+    for k, A in enumerate((H, c)):
+        R = np.zeros((N, N), complex)
+        for d in np.nonzero(dop == k)[0]:
+            o = doff[d]
+            rr = np.arange(max(0, -o), min(N, N - o))
+            R[rr, rr + o] = dval[d, rr]
+        assert np.array_equal(R, A)
+    dense = rng.standard_normal((N, N))
+    assert banded_operators(H, [dense]) is None  # 41 diagonals: 1681 products
+
+
 def test_server_fails_a_request_above_the_budget():
     """A request that can never fit the HBM budget fails at once instead of
     waiting forever (no device work is started)."""
