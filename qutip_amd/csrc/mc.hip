@@ -878,18 +878,40 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj
 // integrate(t) loop of the reference with interpolation), evaluating the
 // functionals at each time, with no host round trip.  Systems are
 // independent initial states (and optionally operators, h_stride apart),
-// picked up grid-stride.
-template <typename R, int NR, int RS>
+// picked up grid-stride.  HL: the one operator of every system (h_stride
+// 0) is copied into LDS after the other arrays once per workgroup, so the
+// RHS products read it there instead of from L2 (the launcher picks this
+// when it fits the configuration's LDS budget).
+template <typename R, int NR, int RS, bool HL>
 __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void rk_small_kernel(McArgs A) {
     extern __shared__ __align__(16) unsigned char mc_dyn_lds[];
     __shared__ McTableau T;
     McLds<R> S;
     kernel_prologue<R, NR, RS>(A, mc_dyn_lds, T, S);
     Ctx<R, NR, RS> c{A, S, T};
+    const int *lcol = nullptr;
+    const C2<R> *lval = nullptr;
+    if constexpr (HL) {
+        unsigned char *p = (unsigned char *)(S.probs + kMcMaxChan);
+        p = (unsigned char *)(((uintptr_t)p + 15) & ~(uintptr_t)15);
+        const int nent = A.h_width * A.d;
+        C2<R> *v = (C2<R> *)p;
+        int *col = (int *)(p + sizeof(C2<R>) * (size_t)nent);
+        const C2<R> *gv = (const C2<R> *)A.h_vals;
+        for (int i = threadIdx.x; i < nent; i += blockDim.x) {
+            v[i] = gv[i];
+            col[i] = A.h_col[i];
+        }
+        __syncthreads();
+        lcol = col;
+        lval = v;
+    }
     for (int sys = blockIdx.x; sys < A.ntraj; sys += gridDim.x) {
         const size_t hoff = (size_t)sys * (size_t)A.h_stride;
-        ctx_rows(c, A.h_col + hoff,
-                 (const void *)((const C2<R> *)A.h_vals + hoff));
+        if constexpr (HL)
+            ctx_rows(c, lcol, (const void *)lval);
+        else
+            ctx_rows(c, A.h_col + hoff, (const void *)((const C2<R> *)A.h_vals + hoff));
         Traj<R, NR, RS> s;
         s.status = MC_OK;
         s.attempts = 0;
@@ -925,21 +947,38 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void rk_smal
     }
 }
 
+constexpr size_t kLdsBudget = 160 * 1024;  // LDS of one gfx950 CU
+
 template <typename R, int NR, int RS, bool SMALL>
 int launch_mc_cfg(const McArgs &A, int threads, int blocks, hipStream_t s) {
-    const size_t lds = mc_lds_bytes<R>(threads * NR, threads, RS);
-    const void *fn = SMALL ? (const void *)rk_small_kernel<R, NR, RS>
+    size_t lds = mc_lds_bytes<R>(threads * NR, threads, RS);
+    const void *fn = SMALL ? (const void *)rk_small_kernel<R, NR, RS, false>
                            : (const void *)mc_traj_kernel<R, NR, RS>;
+    if (SMALL && A.h_stride == 0 && A.h_width > 0) {
+        // the shared operator in LDS when it fits the budget of this shape
+        // (half the LDS for the all-in-LDS shape, which runs two workgroups
+        // per CU)
+        const size_t ell = (size_t)A.h_width * A.d * (sizeof(C2<R>) + sizeof(int));
+        const size_t cap = RS == 0 ? kLdsBudget / 2 : kLdsBudget;
+        if (lds + ell <= cap) {
+            lds += ell;
+            fn = (const void *)rk_small_kernel<R, NR, RS, true>;
+            QA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL((rk_small_kernel<R, NR, RS, true>), dim3(blocks), dim3(threads), lds,
+                               s, A);
+            QA_CHECK_LAUNCH();
+            return QA_OK;
+        }
+    }
     QA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (SMALL)
-        hipLaunchKernelGGL((rk_small_kernel<R, NR, RS>), dim3(blocks), dim3(threads), lds, s, A);
+        hipLaunchKernelGGL((rk_small_kernel<R, NR, RS, false>), dim3(blocks), dim3(threads), lds,
+                           s, A);
     else
         hipLaunchKernelGGL((mc_traj_kernel<R, NR, RS>), dim3(blocks), dim3(threads), lds, s, A);
     QA_CHECK_LAUNCH();
     return QA_OK;
 }
-
-constexpr size_t kLdsBudget = 160 * 1024;  // LDS of one gfx950 CU
 
 // Launch shape (measured on the TFIM trajectories at d = 512 and 1024).
 // Every stage in LDS, one row per thread, when the 18 vectors take at most

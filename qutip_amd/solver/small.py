@@ -80,6 +80,13 @@ class SmallSolve:
     def set_functionals(self, kinds, fptr, rows, cols, w):
         """Linear (kind 0: sum w y[col]) or quadratic (kind 1:
         sum w conj(y[row]) y[col]) functionals, CSR-like by ``fptr``."""
+        arrs = [np.ascontiguousarray(x, dtype=dt) for x, dt in
+                ((kinds, np.int32), (fptr, np.int32), (rows, np.int32), (cols, np.int32),
+                 (w, np.complex128))]
+        key = tuple((a.shape, a.tobytes()) for a in arrs)
+        if key == getattr(self, "_fun_key", None):
+            return  # the same functionals are on the device already
+        self._fun_key = key
         self.nfun = len(kinds)
         up = lambda x, dt: _data.DeviceBuffer.from_array(
             np.ascontiguousarray(x if len(x) else np.zeros(1), dtype=dt))
@@ -89,26 +96,43 @@ class SmallSolve:
         self._fc = up(cols, np.int32)
         self._fw = up(w, np.complex128)
 
-    def launch(self, t0, y0, times, grid=0, stream=None):
+    def _scratch(self, name, nbytes, reuse):
+        """A device buffer of at least nbytes: a fresh one, or (reuse, for the
+        synchronous run()) the one kept from the previous call when it is big
+        enough -- allocations and frees synchronise the device."""
+        if not reuse:
+            return _data.DeviceBuffer(nbytes)
+        bufs = self.__dict__.setdefault("_bufs", {})
+        b = bufs.get(name)
+        if b is None or b.nbytes < nbytes:
+            b = bufs[name] = _data.DeviceBuffer(nbytes)
+        return b
+
+    def launch(self, t0, y0, times, grid=0, stream=None, reuse=False):
         """Enqueue the solves on ``stream`` and return a handle that keeps the
         device buffers alive; ``collect`` reads the results once the stream
-        has reached this point."""
+        has reached this point.  ``reuse``: inputs and outputs go to buffers
+        kept on this object (only for a launch that is collected before the
+        next one, as run() does)."""
+        times = np.ascontiguousarray(np.atleast_1d(times), dtype=np.float64)
+        nt = len(times)
         if isinstance(y0, _data.HipDense):
             nsys = y0.size // self.n
             y0buf = y0
         else:
             y = np.ascontiguousarray(np.asarray(y0, dtype=np.complex128).reshape(-1, self.n))
             nsys = y.shape[0]
-            y0buf = _data.DeviceBuffer.from_array(y)
+            y0buf = self._scratch("y0", y.nbytes, reuse)
+            y0buf.upload(y)
         if self.nops > 1 and nsys > self.nops:
             raise ValueError(f"{self.nops} operators for {nsys} systems")
-        times = np.ascontiguousarray(np.atleast_1d(times), dtype=np.float64)
-        nt = len(times)
-        H = {"nsys": nsys, "nt": nt, "stream": stream, "y0": y0buf,
-             "times": _data.DeviceBuffer.from_array(times if nt else np.zeros(1)),
-             "out": _data.DeviceBuffer(16 * max(nsys * nt * self.nfun, 1)),
+        tb = self._scratch("times", 8 * max(nt, 1), reuse)
+        if nt:
+            tb.upload(times)
+        H = {"nsys": nsys, "nt": nt, "stream": stream, "y0": y0buf, "times": tb,
+             "out": self._scratch("out", 16 * max(nsys * nt * self.nfun, 1), reuse),
              "yfin": _data.DeviceBuffer(16 * nsys * self.n),
-             "info": _data.DeviceBuffer(16 * max(nsys, 1))}
+             "info": self._scratch("info", 16 * max(nsys, 1), reuse)}
         o = self.opts
         rc = _lib.lib().qa_rk_small_run(
             self.n, nsys, float(t0), nt, _vp(H["times"].ptr), _vp(y0buf.ptr), self.width,
@@ -138,7 +162,7 @@ class SmallSolve:
         (functionals [nsys, len(times), nfun] complex, final states as a
         DeviceBuffer [nsys, n] complex128, info [nsys, 4]: status, outputs
         written, dense-output preparations, attempts)."""
-        return self.collect(self.launch(t0, y0, times, grid, stream))
+        return self.collect(self.launch(t0, y0, times, grid, stream, reuse=True))
 
     def rhs_evals(self, info_row):
         """RHS evaluations of one system from its info row (the reference's
