@@ -381,6 +381,7 @@ def stage_roofline(RKEngine, L, stats, lanes, coded, reps=20):
     rho0 = np.zeros(n, dtype=np.complex128)
     rho0[0] = 1.0
     eng.set_state(0.0, rho0)
+    form = (2 if eng.sell.uniform else 1) if coded else 0  # the kernel's FORM argument
     t_step, t_dense = C.c_double(), C.c_double()
     _lib_call("qa_rk_profile_sequence", eng.handle, reps, C.byref(t_step), C.byref(t_dense))
     byts = stage_bytes(eng, nnz, n, False)
@@ -395,7 +396,9 @@ def stage_roofline(RKEngine, L, stats, lanes, coded, reps=20):
         f"{avg_bytes / 1e6:.2f} MB -> {achieved:.0f} GB/s")
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(lanes, coded),
-            "kernel": f"rk_stage_kernel<double2, {lanes}, {'true' if coded else 'false'}>",
+            "kernel": f"rk_stage_kernel<double2, {lanes}, {form}>",
+            "kernel_form": {0: "plain SELL-C", 1: "coded SELL-C",
+                            2: "coded SELL-C, slices of one width"}[form],
             "avg_launch_us": round(avg_ms * 1e3, 3),
             "algorithmic_bytes_per_launch": int(avg_bytes)}
 
