@@ -51,3 +51,58 @@ def test_missing_library_fails_loudly(tmp_path):
             mod.load(tmp_path / "nope.so")
     finally:
         importlib.reload(_lib)
+
+
+def _tableau_args():
+    """vern7 as qa_rk_create* take it (host arrays kept alive by the caller)."""
+    import numpy as np
+    from qutip_amd.solver.tableaus import TABLEAUS
+    t = TABLEAUS["vern7"]
+    arrs = [np.ascontiguousarray(t[k], dtype=np.float64) for k in ("a", "b", "c", "e", "bi")]
+    dp = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    args = (int(t["order"]), len(arrs[1]), len(arrs[2]), arrs[4].shape[1], *map(dp, arrs),
+            1e-8, 1e-10, 1000, 0.0, 0.0, 0.0, 1)
+    return arrs, args
+
+
+def test_coded_entry_points_reject_bad_splits():
+    """The coded operator's code split must be 6..10 value bits; the checks
+    run before any device call."""
+    L = _lib.load()
+    vp = ctypes.c_void_p
+    rc = L.qa_sell_coded_matvec_z(64, 64, 1, 1, vp(16), vp(16), vp(16), vp(16), 5, vp(16),
+                                  vp(16), 1.0, 0.0, 0, None)
+    assert rc == _lib.QA_ERR_ARG and b"vbits" in L.qa_last_error()
+    keep, targs = _tableau_args()
+    h = vp()
+    rc = L.qa_rk_create_coded(ctypes.byref(h), 64, 1, 1, vp(16), vp(16), vp(16), vp(16), 11,
+                              *targs)
+    assert rc == _lib.QA_ERR_ARG and b"vbits" in L.qa_last_error() and not h.value
+
+
+def test_banded_entry_point_rejects_bad_diagonals():
+    """qa_rk_create_dia: a diagonal of a missing operator, an offset of N, or
+    more than 96 products of diagonals are argument errors, raised before
+    any device call."""
+    import numpy as np
+    L = _lib.load()
+    keep, targs = _tableau_args()
+    N = 128
+    ip = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    dp = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def create(ops, offs, nc):
+        ops = np.asarray(ops, np.int32)
+        offs = np.asarray(offs, np.int32)
+        vals = np.ones((len(ops), N), np.complex128)
+        h = ctypes.c_void_p()
+        rc = L.qa_rk_create_dia(ctypes.byref(h), N, nc, len(ops), ip(ops), ip(offs),
+                                dp(vals.view(np.float64)), *targs)
+        return rc, h
+
+    rc, h = create([0, 2], [0, 1], 1)  # operator 2 with one collapse operator
+    assert rc == _lib.QA_ERR_ARG and not h.value
+    rc, h = create([0], [N], 0)  # offset N lies outside the matrix
+    assert rc == _lib.QA_ERR_ARG and not h.value
+    rc, h = create([0] * 50, list(range(-25, 25)), 0)  # 100 products of H_nh's diagonals
+    assert rc == _lib.QA_ERR_ARG and b"products of diagonals" in L.qa_last_error()
