@@ -192,11 +192,19 @@ class RKEngine:
             self.handle = None
 
     def set_functionals(self, kinds, fptr, rows, cols, w):
+        """Linear / quadratic functionals of the state evaluated at every
+        output time (qa_rk_set_functionals).  The same functionals as the
+        last call are left as they are: new ones are uploaded and make the
+        engine capture its loop graph again."""
         kinds = np.ascontiguousarray(kinds, dtype=np.int32)
         fptr = np.ascontiguousarray(fptr, dtype=np.int32)
         rows = np.ascontiguousarray(rows, dtype=np.int32)
         cols = np.ascontiguousarray(cols, dtype=np.int32)
         w = np.ascontiguousarray(w, dtype=np.complex128).view(np.float64)
+        key = tuple((a.shape, a.tobytes()) for a in (kinds, fptr, rows, cols, w))
+        if key == getattr(self, "_fun_key", None):
+            return
+        self._fun_key = key
         _lib.call("qa_rk_set_functionals", self.handle, len(kinds), _ip(kinds), _ip(fptr),
                   _ip(rows), _ip(cols), _dp(w))
         self.nfun = len(kinds)
