@@ -451,8 +451,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the "
-              "solves are independent and would shard by request/trajectory, see DESIGN.md",
-              file=sys.stderr)
+              "solves are independent and would shard one process per GPU, by request "
+              "or by trajectory", file=sys.stderr)
         sys.exit(2)
 
     import qutip_amd as qa
