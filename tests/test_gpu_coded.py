@@ -126,3 +126,29 @@ def test_mesolve_default_uses_the_coded_operator(monkeypatch):
     assert sref._integrator._engine.lanes == solver._integrator._engine.lanes
     for k in range(len(s["e_ops"])):
         assert np.array_equal(np.asarray(res.expect[k]), np.asarray(ref.expect[k]))
+
+
+def test_sesolve_fused_coded_quadratic_functionals(monkeypatch):
+    """sesolve of TFIM L=11 (ket of 2048 entries, above the whole-solve size):
+    the fused engine with the coded operator and quadratic e_ops
+    functionals, within 1e-6 of the FP64 matrix exponential and bit for bit
+    the plain form's values at the same lane count."""
+    print("sesolve L=11 fused coded", flush=True)
+    import qutip_amd as qa
+    from tests import systems
+    monkeypatch.setenv("QUTIP_AMD_NO_TUNE", "1")
+    s = systems.ising(11, 0.1, tf=2.0, npts=11)
+    m = models.ising(11, 0.1)
+    opts = {"rtol": 1e-8, "atol": 1e-10}
+    solver = qa.SESolver(s["H"], options=opts)
+    res = solver.run(s["psi0"], s["tlist"], e_ops=s["e_ops"])
+    assert solver._integrator._engine.coded
+    ref = exact.sesolve_expect(m["H"], m["psi0"], s["tlist"], m["e_ops"])
+    err = max(np.abs(np.asarray(res.expect[k]) - ref[k]).max() for k in range(2))
+    print(f"  max err vs expm {err:.2e}", flush=True)
+    assert err < 1e-6
+    plain = qa.SESolver(s["H"], options={**opts, "operator_format": "sell"})
+    pres = plain.run(s["psi0"], s["tlist"], e_ops=s["e_ops"])
+    assert not plain._integrator._engine.coded
+    for k in range(2):
+        assert np.array_equal(np.asarray(res.expect[k]), np.asarray(pres.expect[k]))
