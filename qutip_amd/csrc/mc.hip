@@ -30,6 +30,7 @@
 // values per instruction; c_ops and e_ops (used at jumps and outputs only)
 // are CSR.  All operators are shared by all trajectories and served from L2.  Norms, error norms and expectation values accumulate in FP64;
 // with the complex64 variant the state, stages and RHS products are FP32.
+#include <algorithm>
 #include <math.h>
 #include <string.h>
 
@@ -97,6 +98,7 @@ struct McArgs {
     const double2 *f_w;
     double2 *fun_out;  // [nsys][ntimes][nfun]
     double2 *y_final;  // [nsys][d] or null
+    int fun_lds;       // LDS bytes reserved for the functionals and times (0: none)
 };
 
 template <typename R>
@@ -152,7 +154,7 @@ struct McLds {
     C2<R> *x;      // [npad]
     C2<R> *kd;     // [kMcStages - RS][npad]
     C2<R> *kfsal;  // [npad]
-    double *red;   // [threads / 64]
+    double *red;   // [2 * threads / 64]
     double *probs; // [kMcMaxChan]
     int npad;
 };
@@ -160,7 +162,7 @@ struct McLds {
 template <typename R>
 inline size_t mc_lds_bytes(int npad, int threads, int RS) {
     return sizeof(C2<R>) * (size_t)npad * (2 + kMcStages - RS) +
-           sizeof(double) * ((threads / 64) + kMcMaxChan) + 64;
+           sizeof(double) * (2 * (threads / 64) + kMcMaxChan) + 64;
 }
 
 // A per-row value: each thread owns NR rows (r = threadIdx.x + q * blockDim.x).
@@ -234,6 +236,28 @@ __device__ __forceinline__ double bsum(Ctx<R, NR, RS> &c, double v) {
     double t = 0.0;
     for (int w = 0; w < nw; ++w) t += c.S.red[w];
     return t;
+}
+
+// Two block sums with one pair of barriers; each component is summed in
+// the order bsum uses, so the results are bsum's bits.
+template <typename R, int NR, int RS>
+__device__ __forceinline__ double2 bsum2(Ctx<R, NR, RS> &c, double a, double b) {
+    const int lane = threadIdx.x & (QA_WAVE - 1), wid = threadIdx.x / QA_WAVE;
+    const int nw = blockDim.x / QA_WAVE;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    __syncthreads();
+    if (lane == 0) {
+        c.S.red[wid] = a;
+        c.S.red[nw + wid] = b;
+    }
+    __syncthreads();
+    double ta = 0.0, tb = 0.0;
+    for (int w = 0; w < nw; ++w) {
+        ta += c.S.red[w];
+        tb += c.S.red[nw + w];
+    }
+    return make_double2(ta, tb);
 }
 
 template <typename R, int NR, int RS>
@@ -573,11 +597,10 @@ __device__ __forceinline__ void write_expect(Ctx<R, NR, RS> &c, const V<R, NR> &
             re += (double)y.v[q].x * oy.v[q].x + (double)y.v[q].y * oy.v[q].y;
             im += (double)y.v[q].x * oy.v[q].y - (double)y.v[q].y * oy.v[q].x;
         }
-        re = bsum(c, re);
-        im = bsum(c, im);
+        const double2 r = bsum2(c, re, im);
         if (threadIdx.x == 0)
             c.A.expect_out[((size_t)traj * c.A.ntimes + k) * c.A.neops + e] =
-                make_double2(re / n2, im / n2);
+                make_double2(r.x / n2, r.y / n2);
     }
 }
 
@@ -597,7 +620,7 @@ __device__ __forceinline__ void kernel_prologue(const McArgs &A, unsigned char *
         S.kfsal = (C2<R> *)p;
         p += sizeof(C2<R>) * npad;
         S.red = (double *)p;
-        p += sizeof(double) * (nt / QA_WAVE);
+        p += sizeof(double) * 2 * (nt / QA_WAVE);
         S.probs = (double *)p;
     }
     {
@@ -662,23 +685,37 @@ __device__ __forceinline__ void integrate_to(Ctx<R, NR, RS> &c, Traj<R, NR, RS> 
     }
 }
 
+template <typename T>
+__device__ __forceinline__ unsigned char *align16(T *p) {
+    return (unsigned char *)(((uintptr_t)p + 15) & ~(uintptr_t)15);
+}
+
+// The functionals as the whole-solve kernel reads them: kind, CSR-style
+// entry pointer, rows, columns and weights, in LDS when they were staged
+// there, else in global memory.
+struct FunView {
+    const int *kind, *ptr, *row, *col;
+    const double2 *w;
+};
+
 // Functionals of the state y (linear: sum w y[col]; quadratic:
 // sum w conj(y[row]) y[col]), accumulated in FP64 in a fixed order, into
 // out[0 .. nfun-1] (thread 0 writes).
 template <typename R, int NR, int RS>
 __device__ __forceinline__ void write_functionals(Ctx<R, NR, RS> &c, const V<R, NR> &y,
-                                                  double2 *out) {
+                                                  const FunView &fv, double2 *out) {
     stage_in(c, y);
     for (int f = 0; f < c.A.nfun; ++f) {
-        const int quad = c.A.f_kind[f];
+        const int quad = fv.kind[f];
+        const int e1 = fv.ptr[f + 1];
         double re = 0.0, im = 0.0;
-        for (int e = c.A.f_ptr[f] + threadIdx.x; e < c.A.f_ptr[f + 1]; e += blockDim.x) {
-            const C2<R> xc = c.S.x[c.A.f_col[e]];
-            const double2 w = c.A.f_w[e];
+        for (int e = fv.ptr[f] + threadIdx.x; e < e1; e += blockDim.x) {
+            const C2<R> xc = c.S.x[fv.col[e]];
+            const double2 w = fv.w[e];
             double vr = w.x * (double)xc.x - w.y * (double)xc.y;
             double vi = w.x * (double)xc.y + w.y * (double)xc.x;
             if (quad) {
-                const C2<R> xr = c.S.x[c.A.f_row[e]];
+                const C2<R> xr = c.S.x[fv.row[e]];
                 const double lr = (double)xr.x, li = -(double)xr.y;
                 const double tr = lr * vr - li * vi;
                 vi = lr * vi + li * vr;
@@ -687,9 +724,8 @@ __device__ __forceinline__ void write_functionals(Ctx<R, NR, RS> &c, const V<R, 
             re += vr;
             im += vi;
         }
-        re = bsum(c, re);
-        im = bsum(c, im);
-        if (threadIdx.x == 0) out[f] = make_double2(re, im);
+        const double2 r = bsum2(c, re, im);
+        if (threadIdx.x == 0) out[f] = r;
     }
 }
 
@@ -710,7 +746,7 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj
         S.kfsal = (C2<R> *)p;
         p += sizeof(C2<R>) * npad;
         S.red = (double *)p;
-        p += sizeof(double) * (nt / QA_WAVE);
+        p += sizeof(double) * 2 * (nt / QA_WAVE);
         S.probs = (double *)p;
     }
     {
@@ -891,21 +927,57 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void rk_smal
     Ctx<R, NR, RS> c{A, S, T};
     const int *lcol = nullptr;
     const C2<R> *lval = nullptr;
+    unsigned char *tail = align16(S.probs + kMcMaxChan);
     if constexpr (HL) {
-        unsigned char *p = (unsigned char *)(S.probs + kMcMaxChan);
-        p = (unsigned char *)(((uintptr_t)p + 15) & ~(uintptr_t)15);
         const int nent = A.h_width * A.d;
-        C2<R> *v = (C2<R> *)p;
-        int *col = (int *)(p + sizeof(C2<R>) * (size_t)nent);
+        C2<R> *v = (C2<R> *)tail;
+        int *col = (int *)(tail + sizeof(C2<R>) * (size_t)nent);
         const C2<R> *gv = (const C2<R> *)A.h_vals;
         for (int i = threadIdx.x; i < nent; i += blockDim.x) {
             v[i] = gv[i];
             col[i] = A.h_col[i];
         }
-        __syncthreads();
         lcol = col;
         lval = v;
+        tail = align16(col + nent);
     }
+    // The functionals, then the output times, into the fun_lds bytes the
+    // launcher reserved after the arrays above, each when it fits there
+    // (the sizes are checked here, where the entry count is known).
+    FunView fv{A.f_kind, A.f_ptr, A.f_row, A.f_col, A.f_w};
+    const double *times = A.times;
+    if (A.fun_lds > 0) {
+        size_t used = 0;
+        if (A.nfun > 0) {
+            const int nfun = A.nfun, nnz = A.f_ptr[nfun];
+            const size_t need = sizeof(double2) * (size_t)nnz +
+                                sizeof(int) * (2 * (size_t)nnz + 2 * (size_t)nfun + 1);
+            if (nnz >= 0 && need <= (size_t)A.fun_lds) {
+                double2 *w = (double2 *)tail;
+                int *ptr = (int *)(w + nnz);
+                int *kind = ptr + nfun + 1;
+                int *row = kind + nfun;
+                int *col = row + nnz;
+                for (int i = threadIdx.x; i < nnz; i += blockDim.x) {
+                    w[i] = A.f_w[i];
+                    row[i] = A.f_row[i];
+                    col[i] = A.f_col[i];
+                }
+                for (int i = threadIdx.x; i <= nfun; i += blockDim.x) {
+                    ptr[i] = A.f_ptr[i];
+                    if (i < nfun) kind[i] = A.f_kind[i];
+                }
+                fv = FunView{kind, ptr, row, col, w};
+                used = (size_t)(align16(col + nnz) - tail);
+            }
+        }
+        if (used + sizeof(double) * (size_t)A.ntimes <= (size_t)A.fun_lds) {
+            double *tl = (double *)(tail + used);
+            for (int i = threadIdx.x; i < A.ntimes; i += blockDim.x) tl[i] = A.times[i];
+            times = tl;
+        }
+    }
+    __syncthreads();
     for (int sys = blockIdx.x; sys < A.ntraj; sys += gridDim.x) {
         const size_t hoff = (size_t)sys * (size_t)A.h_stride;
         if constexpr (HL)
@@ -926,9 +998,9 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void rk_smal
         set_initial_value(c, s, y0, A.t0);
         int k = 0;
         for (; k < A.ntimes && s.status >= 0; ++k) {
-            integrate_to(c, s, A.times[k]);
+            integrate_to(c, s, times[k]);
             if (s.status < 0) break;
-            write_functionals(c, s.y, A.fun_out + ((size_t)sys * A.ntimes + k) * A.nfun);
+            write_functionals(c, s.y, fv, A.fun_out + ((size_t)sys * A.ntimes + k) * A.nfun);
         }
         if (A.y_final) {
 #pragma unroll
@@ -949,33 +1021,48 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void rk_smal
 
 constexpr size_t kLdsBudget = 160 * 1024;  // LDS of one gfx950 CU
 
+constexpr size_t kFunLdsMax = 16 * 1024;  // LDS for the functionals and times
+
 template <typename R, int NR, int RS, bool SMALL>
-int launch_mc_cfg(const McArgs &A, int threads, int blocks, hipStream_t s) {
+int launch_mc_cfg(const McArgs &A0, int threads, int blocks, hipStream_t s) {
+    McArgs A = A0;
     size_t lds = mc_lds_bytes<R>(threads * NR, threads, RS);
-    const void *fn = SMALL ? (const void *)rk_small_kernel<R, NR, RS, false>
-                           : (const void *)mc_traj_kernel<R, NR, RS>;
-    if (SMALL && A.h_stride == 0 && A.h_width > 0) {
-        // the shared operator in LDS when it fits the budget of this shape
-        // (half the LDS for the all-in-LDS shape, which runs two workgroups
-        // per CU)
-        const size_t ell = (size_t)A.h_width * A.d * (sizeof(C2<R>) + sizeof(int));
-        const size_t cap = RS == 0 ? kLdsBudget / 2 : kLdsBudget;
-        if (lds + ell <= cap) {
-            lds += ell;
-            fn = (const void *)rk_small_kernel<R, NR, RS, true>;
-            QA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL((rk_small_kernel<R, NR, RS, true>), dim3(blocks), dim3(threads), lds,
-                               s, A);
-            QA_CHECK_LAUNCH();
-            return QA_OK;
+    // the all-in-LDS shape runs two workgroups per CU: half the LDS each
+    const size_t cap = RS == 0 ? kLdsBudget / 2 : kLdsBudget;
+    bool hl = false;
+    A.fun_lds = 0;
+    if (SMALL) {
+        if (A.h_stride == 0 && A.h_width > 0) {
+            // the shared operator in LDS when it fits the budget of this shape
+            const size_t ell = (size_t)A.h_width * A.d * (sizeof(C2<R>) + sizeof(int));
+            if (lds + ell <= cap) {
+                lds += ell;
+                hl = true;
+            }
+        }
+        if (A.nfun > 0 || A.ntimes > 0) {
+            // room for the functionals and times, up to kFunLdsMax, that
+            // leaves the workgroups an LDS-limited CU holds unchanged
+            const size_t per_wg = kLdsBudget / (kLdsBudget / lds);
+            const size_t room = std::min(std::min(per_wg, cap) - lds, kFunLdsMax + 16);
+            if (room >= 16 + 256) {
+                A.fun_lds = (int)((room - 16) & ~(size_t)15);
+                lds += 16 + (size_t)A.fun_lds;
+            }
         }
     }
+    const void *fn = !SMALL ? (const void *)mc_traj_kernel<R, NR, RS>
+                     : hl   ? (const void *)rk_small_kernel<R, NR, RS, true>
+                            : (const void *)rk_small_kernel<R, NR, RS, false>;
     QA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (SMALL)
-        hipLaunchKernelGGL((rk_small_kernel<R, NR, RS, false>), dim3(blocks), dim3(threads), lds,
+    if (!SMALL)
+        hipLaunchKernelGGL((mc_traj_kernel<R, NR, RS>), dim3(blocks), dim3(threads), lds, s, A);
+    else if (hl)
+        hipLaunchKernelGGL((rk_small_kernel<R, NR, RS, true>), dim3(blocks), dim3(threads), lds,
                            s, A);
     else
-        hipLaunchKernelGGL((mc_traj_kernel<R, NR, RS>), dim3(blocks), dim3(threads), lds, s, A);
+        hipLaunchKernelGGL((rk_small_kernel<R, NR, RS, false>), dim3(blocks), dim3(threads), lds,
+                           s, A);
     QA_CHECK_LAUNCH();
     return QA_OK;
 }
