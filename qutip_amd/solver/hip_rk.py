@@ -594,14 +594,20 @@ class IntegratorHipRK(Integrator):
             self._check()
         return self.get_state(copy)
 
-    def run_functionals(self, tlist, kinds, fptr, rows, cols, w):
+    def run_functionals(self, tlist, kinds, fptr, rows, cols, w, at_start=False):
         """Fused Solver.run: functionals of the state at every time of
-        ``tlist[1:]`` evaluated on the device.  Returns [len(tlist)-1, nfun]."""
+        ``tlist[1:]`` evaluated on the device.  Returns (values, started):
+        values is [len(tlist)-1, nfun], or, when ``at_start`` is asked and
+        the whole-solve kernel runs (started True), [len(tlist), nfun] with
+        the functionals of the initial state at tlist[0] as row 0."""
         if self._engine is None:
             raise IntegratorException("run_functionals needs the fused engine")
         times = np.asarray(tlist[1:], dtype=np.float64)
         if self._small is not None and self._small_start is not None:
             t0, y0 = self._small_start
+            started = bool(at_start) and len(tlist) > 0 and float(tlist[0]) == t0
+            if started:
+                times = np.asarray(tlist, dtype=np.float64)
             self._small.set_functionals(kinds, fptr, rows, cols, w)
             vals, yfin, info = self._small.run(t0, y0, times)
             st = int(info[0, 0])
@@ -615,12 +621,12 @@ class IntegratorHipRK(Integrator):
                                  "rhs_evals": self._small.rhs_evals(info[0]),
                                  "outputs": int(info[0, 1]), "status": st,
                                  "path": "whole solve in one workgroup"}
-            return vals[0]
+            return vals[0], started
         self._small_stats = None
         self._leave_small()
         self._engine.set_functionals(kinds, fptr, rows, cols, w)
         vals, _, _ = self._engine.run(times)
-        return vals
+        return vals, False
 
     def stats(self):
         if self._small_stats is not None:

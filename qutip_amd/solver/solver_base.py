@@ -158,7 +158,19 @@ class Solver:
         return list(e_ops)
 
     def _functionals(self, ops):
-        """Device functionals for operator e_ops plus the norm functional last."""
+        """Device functionals for operator e_ops plus the norm functional
+        last; the arrays of the last e_ops list are kept (Qobj data is
+        treated as immutable, as its cached isherm already is)."""
+        key = (self._state_is_ket, self._state_shape,
+               tuple((id(O), id(O.data)) for O in ops))
+        hit = getattr(self, "_fun_cache", None)
+        if hit is not None and hit[0] == key:
+            return hit[2]
+        f = self._build_functionals(ops)
+        self._fun_cache = (key, [(O, O.data) for O in ops], f)  # refs keep the ids valid
+        return f
+
+    def _build_functionals(self, ops):
         kinds, fptr, rows, cols, ws = [], [0], [], [], []
         if self._state_is_ket:
             n = self._state_shape[0]
@@ -227,14 +239,23 @@ class Solver:
         fused = (getattr(integ, "fused", False) and all(isinstance(O, Qobj) for O in ops)
                  and not store_states)
         if fused:
-            # initial point on the host-prepared state, then the device loop
-            v0 = self._expect_on_device(ops, y0)
-            res.add_expect_values(tlist[0], v0)
+            # the device loop; the initial point comes from the same
+            # functionals when the whole-solve kernel evaluates them at
+            # tlist[0], else from the host-prepared state
+            vals, started = None, False
             if len(tlist) > 1:
-                vals = integ.run_functionals(tlist, *self._functionals(ops))
+                vals, started = integ.run_functionals(tlist, *self._functionals(ops),
+                                                      at_start=True)
+            if started:
                 per_op = self._values_from_functionals(ops, vals)
-                for k, t in enumerate(tlist[1:]):
+                for k, t in enumerate(tlist):
                     res.add_expect_values(t, [per_op[j][k] for j in range(len(ops))])
+            else:
+                res.add_expect_values(tlist[0], self._expect_on_device(ops, y0))
+                if vals is not None:
+                    per_op = self._values_from_functionals(ops, vals)
+                    for k, t in enumerate(tlist[1:]):
+                        res.add_expect_values(t, [per_op[j][k] for j in range(len(ops))])
             if store_final:
                 res.final_state = self._restore_state(integ.get_state(copy=False)[1])
             stats.update(integ.stats())

@@ -143,3 +143,37 @@ def test_statuses():
                                atol=1e-12, min_step=0.5, max_step=0.5)
     assert ost["status"] == -2
     assert info[0, 0] == -2 and info[0, 3] == ost["n_attempts"]  # DT_UNDERFLOW
+
+
+def test_solver_takes_the_initial_point_from_the_kernel():
+    """MESolver.run on the whole-solve path evaluates the e_ops at tlist[0]
+    in the kernel (the functionals of the initial state): the values equal
+    tr(O rho0) computed on the host, and a second run with the same e_ops
+    list (functionals kept from the first run) gives the same bits; a new
+    list with other operators gives its own values."""
+    print("small: solver initial point from the kernel", flush=True)
+    import qutip_amd as qa
+    from tests import systems
+    import pathlib
+    s = systems.jaynes_cummings()
+    g = np.load(pathlib.Path(__file__).parent / "golden" / "jc_mesolve.npz")
+    tl = g["tlist"]
+    solver = qa.MESolver(s["H"], s["c_ops"], options=OPTS)
+    r1 = solver.run(s["psi0"], tl, e_ops=s["e_ops"])
+    assert solver._integrator.stats()["path"] == "whole solve in one workgroup"
+    assert solver._integrator.stats()["outputs"] == len(tl)
+    psi = s["psi0"].full().ravel()
+    for k, O in enumerate(s["e_ops"]):
+        want = np.vdot(psi, O.full() @ psi).real
+        assert len(r1.expect[k]) == len(tl)
+        assert abs(r1.expect[k][0] - want) < 1e-14, (k, r1.expect[k][0], want)
+    r2 = solver.run(s["psi0"], tl, e_ops=s["e_ops"])
+    for k in range(2):
+        assert np.array_equal(np.asarray(r1.expect[k]), np.asarray(r2.expect[k]))
+    # other operators: the kept functionals must not be reused
+    ops3 = [s["e_ops"][1], s["e_ops"][0]]
+    r3 = solver.run(s["psi0"], tl, e_ops=ops3)
+    assert np.array_equal(np.asarray(r3.expect[0]), np.asarray(r1.expect[1]))
+    assert np.array_equal(np.asarray(r3.expect[1]), np.asarray(r1.expect[0]))
+    for k in range(2):  # the golden QuTiP run on the same times
+        assert np.abs(np.asarray(r1.expect[k]) - g["expect"][k]).max() < 1e-6
