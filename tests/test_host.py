@@ -253,3 +253,35 @@ def test_shortest_work_first_estimates():
         assert w[me512[0].id] > w[se16[0].id]
     with pytest.raises(ValueError):
         SolveServer(policy="lifo")
+
+
+def test_solver_functionals_kept_per_e_ops_list():
+    """Solver._functionals builds the device functionals of an e_ops list
+    once: the same operators (same objects, same data) give back the kept
+    arrays; a reordered list, an operator whose data was replaced or another
+    state shape rebuild them, and every result equals a fresh build."""
+    from qutip_amd.solver.solver_base import Solver
+
+    def fresh(s, ops):
+        t = Solver.__new__(Solver)
+        t._state_is_ket, t._state_shape = s._state_is_ket, s._state_shape
+        return t._build_functionals(ops)
+
+    def same(a, b):
+        return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+    s = Solver.__new__(Solver)
+    s._state_is_ket, s._state_shape = False, (6, 6)
+    A, B = qa.num(6), qa.destroy(6) + qa.create(6)
+    f1 = s._functionals([A, B])
+    assert same(f1, fresh(s, [A, B]))
+    assert s._functionals([A, B]) is f1  # kept
+    f2 = s._functionals([B, A])
+    assert f2 is not f1 and same(f2, fresh(s, [B, A]))
+    B.data = (2.0 * B).data  # new data object on the same Qobj
+    f3 = s._functionals([B, A])
+    assert f3 is not f2 and same(f3, fresh(s, [B, A]))
+    assert not same(f3, f2)
+    s._state_is_ket, s._state_shape = True, (6, 1)  # a ket: quadratic forms
+    f4 = s._functionals([B, A])
+    assert same(f4, fresh(s, [B, A])) and (f4[0] == 1).all()
