@@ -177,3 +177,27 @@ def test_solver_takes_the_initial_point_from_the_kernel():
     assert np.array_equal(np.asarray(r3.expect[1]), np.asarray(r1.expect[0]))
     for k in range(2):  # the golden QuTiP run on the same times
         assert np.abs(np.asarray(r1.expect[k]) - g["expect"][k]).max() < 1e-6
+
+
+def test_sesolve_initial_point_from_the_kernel():
+    """SESolver.run on the whole-solve path (a ket: quadratic functionals
+    conj(psi) O psi) takes the e_ops values at tlist[0] from the kernel:
+    they equal <psi0|O|psi0> on the host, and every value is within 1e-6 of
+    the FP64 matrix exponential."""
+    print("small: sesolve initial point from the kernel", flush=True)
+    import qutip_amd as qa
+    from tests import systems
+    s = systems.jaynes_cummings()
+    psi0 = (qa.tensor(qa.basis(10, 2), qa.basis(2, 0)) +
+            0.5j * qa.tensor(qa.basis(10, 1), qa.basis(2, 1))).unit()
+    tl = np.linspace(0.0, 5.0, 41)
+    solver = qa.SESolver(s["H"], options=OPTS)
+    r = solver.run(psi0, tl, e_ops=s["e_ops"])
+    assert solver._integrator.stats()["path"] == "whole solve in one workgroup"
+    psi = psi0.full().ravel()
+    ref = exact.sesolve_expect(s["H"].full(), psi, tl, [O.full() for O in s["e_ops"]])
+    for k, O in enumerate(s["e_ops"]):
+        want = np.vdot(psi, O.full() @ psi).real
+        got = np.asarray(r.expect[k])
+        assert abs(got[0] - want) < 1e-14, (k, got[0], want)
+        assert np.abs(got - ref[k]).max() < 1e-6, k
