@@ -1043,8 +1043,8 @@ int launch_mc_cfg(const McArgs &A0, int threads, int blocks, hipStream_t s) {
         if (A.nfun > 0 || A.ntimes > 0) {
             // room for the functionals and times, up to kFunLdsMax, that
             // leaves the workgroups an LDS-limited CU holds unchanged
-            const size_t per_wg = kLdsBudget / (kLdsBudget / lds);
-            const size_t room = std::min(std::min(per_wg, cap) - lds, kFunLdsMax + 16);
+            const size_t lim = std::min(kLdsBudget / (kLdsBudget / lds), cap);
+            const size_t room = lim > lds ? std::min(lim - lds, kFunLdsMax + 16) : 0;
             if (room >= 16 + 256) {
                 A.fun_lds = (int)((room - 16) & ~(size_t)15);
                 lds += 16 + (size_t)A.fun_lds;
