@@ -494,6 +494,7 @@ class IntegratorHipRK(Integrator):
         self._small_start = None   # (t0, device state) of the last set_state
         self._small_final = None   # (t, device state) after a whole-solve run
         self._small_stats = None
+        self._engine_pending = False  # the engine still waits for _small_start
         owner = getattr(self.derivative, "__self__", None)
         dense = getattr(owner, "dense_operators", None) if o["fused"] else None
         op = constant_operator(self.derivative) if (o["fused"] and dense is None) else None
@@ -546,7 +547,13 @@ class IntegratorHipRK(Integrator):
                                   o["nsteps"], o["first_step"], o["min_step"], o["max_step"],
                                   o["interpolate"])
             self.name = "hip_rk (plain)"
-        if self._engine is not None:
+        self._engine_pending = False
+        if self._engine is not None and self._small is not None:
+            # the whole-solve kernel starts from _small_start; the engine
+            # (its first-step estimate included) is set only when a
+            # step-wise call needs it
+            self._engine_pending = True
+        elif self._engine is not None:
             self._engine.set_state(t, y.reshape((y.size, 1)))
         else:
             self._plain.set_initial_value(y.copy(), t)
@@ -555,15 +562,23 @@ class IntegratorHipRK(Integrator):
         self._is_set = True
 
     def _leave_small(self):
-        """Continue step-wise from where a whole-solve run ended."""
+        """Continue step-wise from where a whole-solve run ended, or from
+        the initial state when the engine has not been set yet."""
         if self._small_final is not None:
             t, y = self._small_final
             self._small_final = None
             self._engine.set_state(t, y.reshape((y.size, 1)))
+        elif self._engine_pending:
+            t, y = self._small_start
+            self._engine.set_state(t, y.reshape((y.size, 1)))
+        self._engine_pending = False
 
     def get_state(self, copy=True):
         if self._small_final is not None:
             t, y = self._small_final
+            return t, (y.copy() if copy else y).reshape(self._shape)
+        if self._engine_pending:
+            t, y = self._small_start
             return t, (y.copy() if copy else y).reshape(self._shape)
         if self._engine is not None:
             t, y = self._engine.get_state(copy)
