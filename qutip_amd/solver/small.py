@@ -108,12 +108,13 @@ class SmallSolve:
             b = bufs[name] = _data.DeviceBuffer(nbytes)
         return b
 
-    def launch(self, t0, y0, times, grid=0, stream=None, reuse=False):
+    def launch(self, t0, y0, times, grid=0, stream=None, reuse=False, yfin=None):
         """Enqueue the solves on ``stream`` and return a handle that keeps the
         device buffers alive; ``collect`` reads the results once the stream
         has reached this point.  ``reuse``: inputs and outputs go to buffers
         kept on this object (only for a launch that is collected before the
-        next one, as run() does)."""
+        next one, as run() does).  ``yfin``: a device buffer (``.ptr``) of at
+        least nsys * n complex128 for the final states, else a new one."""
         times = np.ascontiguousarray(np.atleast_1d(times), dtype=np.float64)
         nt = len(times)
         if isinstance(y0, _data.HipDense):
@@ -131,7 +132,7 @@ class SmallSolve:
             tb.upload(times)
         H = {"nsys": nsys, "nt": nt, "stream": stream, "y0": y0buf, "times": tb,
              "out": self._scratch("out", 16 * max(nsys * nt * self.nfun, 1), reuse),
-             "yfin": _data.DeviceBuffer(16 * nsys * self.n),
+             "yfin": yfin if yfin is not None else _data.DeviceBuffer(16 * nsys * self.n),
              "info": self._scratch("info", 16 * max(nsys, 1), reuse)}
         o = self.opts
         rc = _lib.lib().qa_rk_small_run(
@@ -156,13 +157,13 @@ class SmallSolve:
         H["info"].download(inf)
         return vals, H["yfin"], inf
 
-    def run(self, t0, y0, times, grid=0, stream=None):
+    def run(self, t0, y0, times, grid=0, stream=None, yfin=None):
         """Integrate every system from (t0, y0[s]) through ``times`` (system
         s uses operator s when there is one per system).  Returns
         (functionals [nsys, len(times), nfun] complex, final states as a
         DeviceBuffer [nsys, n] complex128, info [nsys, 4]: status, outputs
         written, dense-output preparations, attempts)."""
-        return self.collect(self.launch(t0, y0, times, grid, stream, reuse=True))
+        return self.collect(self.launch(t0, y0, times, grid, stream, reuse=True, yfin=yfin))
 
     def rhs_evals(self, info_row):
         """RHS evaluations of one system from its info row (the reference's
