@@ -48,6 +48,13 @@ class Result:
         for k, v in enumerate(values):
             self._raw[k].append(v)
 
+    def add_expect_block(self, times, per_op):
+        """add_expect_values for many times at once: per_op[k] holds the
+        values of e_op k at every time of ``times``."""
+        self.times.extend(times)
+        for k, v in enumerate(per_op):
+            self._raw[k].extend(v)
+
     def add_state(self, state, store_states, store_final):
         if store_states:
             self.states.append(state)

@@ -211,13 +211,15 @@ class Solver:
         return val
 
     def _values_from_functionals(self, ops, vals):
+        """One array per e_op over the times of ``vals`` (_cast applied to
+        the whole array)."""
         norm = vals[:, len(ops)]
         out = []
         for j, O in enumerate(ops):
             v = vals[:, j]
             if self._normalize_output:
                 v = v / norm.real
-            out.append([self._cast(O, x) for x in v])
+            out.append(self._cast(O, v))
         return out
 
     # -- run ------------------------------------------------------------------
@@ -247,15 +249,11 @@ class Solver:
                 vals, started = integ.run_functionals(tlist, *self._functionals(ops),
                                                       at_start=True)
             if started:
-                per_op = self._values_from_functionals(ops, vals)
-                for k, t in enumerate(tlist):
-                    res.add_expect_values(t, [per_op[j][k] for j in range(len(ops))])
+                res.add_expect_block(tlist, self._values_from_functionals(ops, vals))
             else:
                 res.add_expect_values(tlist[0], self._expect_on_device(ops, y0))
                 if vals is not None:
-                    per_op = self._values_from_functionals(ops, vals)
-                    for k, t in enumerate(tlist[1:]):
-                        res.add_expect_values(t, [per_op[j][k] for j in range(len(ops))])
+                    res.add_expect_block(tlist[1:], self._values_from_functionals(ops, vals))
             if store_final:
                 res.final_state = self._restore_state(integ.get_state(copy=False)[1])
             stats.update(integ.stats())
