@@ -293,6 +293,15 @@ int qa_rk_iterate(qa_rk_solver *s, int count);
 /* Size in bytes of the packed tableau the trajectory kernel reads. */
 int qa_mc_tableau_size(void);
 
+/* The LDS plan of a trajectory (small == 0) or whole-solve (small != 0)
+ * launch for d rows, without launching: out[0..7] = threads, rows per
+ * thread, stages kept in registers, dynamic LDS bytes, static LDS bytes,
+ * operator staged in LDS (0/1), bytes for functionals and times, and the
+ * dynamic-LDS budget of the shape.  Host arithmetic only (no device call).
+ * No reference counterpart (launch configuration). */
+int qa_mc_lds_plan(int d, int h_width, int nfun, int ntimes, int fp32, int small,
+                   long long *out);
+
 /* Pack a tableau (host arrays as for qa_rk_create) into `out` (host buffer
  * of qa_mc_tableau_size() bytes); the caller copies it to the device. */
 int qa_mc_pack_tableau(void *out, int order, int rk_step, int rk_extra, int dorder,
@@ -341,6 +350,14 @@ int qa_mc_run(int d, int ntraj, int ntimes, const double *times_dev, const void 
  * outputs written, dense-output preparations, RK attempts.  The tableau is packed by
  * qa_mc_pack_tableau.  fp32 != 0: complex64 operator, state and stages with
  * FP64 error norm and functionals. */
+/* Validate functionals (host arrays, layout of qa_rk_set_functionals) for
+ * a state of n entries: fptr[0] == 0 and non-decreasing, every column in
+ * [0, n), every row of a quadratic functional in [0, n).  qa_rk_small_run
+ * takes its functionals on the device and cannot read them: its callers run
+ * this check on the host copies before uploading.  Host only. */
+int qa_functionals_check(int n, int nfun, const int *kinds, const int *fptr, const int *rows,
+                         const int *cols);
+
 int qa_rk_small_run(int n, int nsys, double t0, int ntimes, const double *times_dev,
                     const void *y0_dev, int h_width, const int *h_col, const void *h_vals,
                     long long h_stride, int nfun, const int *fkind, const int *fptr,

@@ -85,6 +85,8 @@ SIGNATURES = {
     "qa_rk_profile_stages": [vp, i32, P_f64],
     "qa_rk_profile_sequence": [vp, i32, P_f64, P_f64],
     "qa_mc_tableau_size": [],
+    "qa_mc_lds_plan": [i32, i32, i32, i32, i32, i32, P_i64],
+    "qa_functionals_check": [i32, i32, P_i32, P_i32, P_i32, P_i32],
     "qa_mc_pack_tableau": [vp, i32, i32, i32, i32, P_f64, P_f64, P_f64, P_f64, P_f64, i32],
     "qa_rk_small_run": [i32, i32, f64, i32, vp, vp, i32, vp, vp, C.c_longlong, i32, vp, vp, vp,
                         vp, vp, vp, f64, f64, i32, f64, f64, f64, i32, vp, vp, vp, i32, vp],
@@ -159,5 +161,12 @@ def check(rc, what=""):
 
 def call(name, *args):
     rc = getattr(lib(), name)(*args)
+    check(rc, name)
+    return rc
+
+
+def host_call(name, *args):
+    """Call an entry point that does host arithmetic only (no GPU needed)."""
+    rc = getattr(load(), name)(*args)
     check(rc, name)
     return rc

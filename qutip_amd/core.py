@@ -408,42 +408,47 @@ class LindbladMatrixForm:
     """Lindblad right-hand side in matrix form (qutip/core/cy/
     lindblad_matrix_form.pyx:59-203): drho/dt = -i H_nh rho + i rho H_nh^dag
     + sum_c c rho c^dag with H_nh = H - i/2 sum_c c^dag c (:92-97).  The
-    state is rho itself (N x N, not vectorised).  ``matmul_data`` runs the
-    dense engine's product (MFMA, or FP64 from the diagonals of banded
-    operators: ``precision``); the hip_rk integrator picks up
-    ``dense_operators`` and runs the whole step loop on the device."""
+    state is rho itself (N x N, not vectorised).  As in the reference
+    (lindblad_matrix_form.pxd:7-10) ``H_nh`` is a QobjEvo, ``c_ops`` a list
+    of QobjEvo, ``issuper`` is True and calling the object raises
+    NotImplementedError (:217-222).  ``matmul_data`` runs the dense engine's
+    product (MFMA, or FP64 from the diagonals of banded operators:
+    ``precision``); the hip_rk integrator reads ``H_nh`` / ``c_ops`` and runs
+    the whole step loop on the device."""
 
-    issuper = False
-    isconstant = True
+    issuper = True
+    isoper = True
 
-    def __init__(self, H, c_ops, precision="auto"):
+    def __init__(self, H, c_ops, precision="auto", assume_hermitian_state=True):
         if isinstance(H, QobjEvo):
             H = H.qobj
-        self.H = H
-        self.c_ops = [c.qobj if isinstance(c, QobjEvo) else c for c in (c_ops or [])]
+        self.c_ops = [c if isinstance(c, QobjEvo) else QobjEvo(c) for c in (c_ops or [])]
         self.dims = H.dims
         self.shape = H.shape
         self.num_collapse = len(self.c_ops)
+        self.assume_hermitian_state = assume_hermitian_state
         Hnh = H.full().astype(np.complex128)
         for cop in self.c_ops:
-            cm = cop.full()
+            cm = cop.qobj.full()
             Hnh = Hnh - 0.5j * (cm.conj().T @ cm)
-        self.H_nh = Hnh
+        self.H_nh = QobjEvo(Qobj(Hnh, dims=H.dims))
         self.precision = precision
         self._engine = None
 
-    def dense_operators(self):
-        return self.H_nh, [c.full() for c in self.c_ops]
+    @property
+    def isconstant(self):
+        return self.H_nh.isconstant and all(c.isconstant for c in self.c_ops)
 
     def __call__(self, t=0.0, args=None):
-        raise TypeError("LindbladMatrixForm cannot be called as a Qobj "
-                        "(lindblad_matrix_form.pyx:217-222)")
+        raise NotImplementedError(
+            "LindbladMatrixForm cannot be called to produce an operator. "
+            "Use matmul_data() to compute the RHS.")
 
     def matmul_data(self, t, rho, out=None):
         """out += L(rho)  (matmul_data, lindblad_matrix_form.pyx:105-203)."""
         if self._engine is None:
-            from .solver.hip_rk import RKEngine
-            self._engine = RKEngine(None, dense=self.dense_operators(),
+            from .solver.hip_rk import RKEngine, matrix_form_operators
+            self._engine = RKEngine(None, dense=matrix_form_operators(self),
                                     dense_precision=self.precision)
         res = self._engine.apply(rho)
         if out is None:

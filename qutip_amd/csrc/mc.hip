@@ -1023,34 +1023,69 @@ constexpr size_t kLdsBudget = 160 * 1024;  // LDS of one gfx950 CU
 
 constexpr size_t kFunLdsMax = 16 * 1024;  // LDS for the functionals and times
 
-template <typename R, int NR, int RS, bool SMALL>
-int launch_mc_cfg(const McArgs &A0, int threads, int blocks, hipStream_t s) {
-    McArgs A = A0;
-    size_t lds = mc_lds_bytes<R>(threads * NR, threads, RS);
-    // the all-in-LDS shape runs two workgroups per CU: half the LDS each
-    const size_t cap = RS == 0 ? kLdsBudget / 2 : kLdsBudget;
-    bool hl = false;
-    A.fun_lds = 0;
-    if (SMALL) {
+// Static LDS of both kernels (the `__shared__ McTableau T`), rounded up to
+// the 16-byte granule the dynamic part starts on.  Every budget below counts
+// it: a workgroup's LDS is the static part plus the dynamic bytes.
+constexpr size_t kLdsStatic = (sizeof(McTableau) + 15) & ~(size_t)15;
+
+// The LDS plan of one launch: dynamic bytes, the operator in LDS (HL), and
+// the bytes reserved for functionals and times.
+struct McPlan {
+    size_t lds;   // dynamic LDS bytes per workgroup
+    bool hl;      // the shared ELL operator is staged in LDS
+    int fun_lds;  // bytes for the functionals and times (0: none)
+};
+
+// RS == 0 runs two workgroups per CU, so each gets half of the CU's LDS,
+// its static part included.
+inline size_t mc_lds_cap(int RS) {
+    return (RS == 0 ? kLdsBudget / 2 : kLdsBudget) - kLdsStatic;
+}
+
+template <typename R>
+McPlan plan_mc(const McArgs &A, int threads, int NR, int RS, bool small) {
+    McPlan P;
+    P.lds = mc_lds_bytes<R>(threads * NR, threads, RS);
+    P.hl = false;
+    P.fun_lds = 0;
+    const size_t cap = mc_lds_cap(RS);
+    if (small) {
         if (A.h_stride == 0 && A.h_width > 0) {
             // the shared operator in LDS when it fits the budget of this shape
             const size_t ell = (size_t)A.h_width * A.d * (sizeof(C2<R>) + sizeof(int));
-            if (lds + ell <= cap) {
-                lds += ell;
-                hl = true;
+            if (P.lds + ell <= cap) {
+                P.lds += ell;
+                P.hl = true;
             }
         }
         if (A.nfun > 0 || A.ntimes > 0) {
             // room for the functionals and times, up to kFunLdsMax, that
-            // leaves the workgroups an LDS-limited CU holds unchanged
-            const size_t lim = std::min(kLdsBudget / (kLdsBudget / lds), cap);
-            const size_t room = lim > lds ? std::min(lim - lds, kFunLdsMax + 16) : 0;
+            // keeps the number of workgroups an LDS-limited CU holds
+            const size_t per_wg = P.lds + kLdsStatic;
+            const size_t nwg = kLdsBudget / per_wg;  // >= 1: P.lds <= cap
+            const size_t lim = std::min(kLdsBudget / nwg - kLdsStatic, cap);
+            const size_t room = lim > P.lds ? std::min(lim - P.lds, kFunLdsMax + 16) : 0;
             if (room >= 16 + 256) {
-                A.fun_lds = (int)((room - 16) & ~(size_t)15);
-                lds += 16 + (size_t)A.fun_lds;
+                P.fun_lds = (int)((room - 16) & ~(size_t)15);
+                P.lds += 16 + (size_t)P.fun_lds;
             }
         }
     }
+    return P;
+}
+
+template <typename R, int NR, int RS, bool SMALL>
+int launch_mc_cfg(const McArgs &A0, int threads, int blocks, hipStream_t s) {
+    McArgs A = A0;
+    const McPlan P = plan_mc<R>(A, threads, NR, RS, SMALL);
+    if (P.lds > mc_lds_cap(RS)) {
+        qa::set_error("trajectory kernel: %zu B of dynamic LDS exceed the %zu B budget", P.lds,
+                      mc_lds_cap(RS));
+        return QA_ERR_ARG;
+    }
+    A.fun_lds = P.fun_lds;
+    const size_t lds = P.lds;
+    const bool hl = P.hl;
     const void *fn = !SMALL ? (const void *)mc_traj_kernel<R, NR, RS>
                      : hl   ? (const void *)rk_small_kernel<R, NR, RS, true>
                             : (const void *)rk_small_kernel<R, NR, RS, false>;
@@ -1067,25 +1102,43 @@ int launch_mc_cfg(const McArgs &A0, int threads, int blocks, hipStream_t s) {
     return QA_OK;
 }
 
-// Launch shape (measured on the TFIM trajectories at d = 512 and 1024).
-// Every stage in LDS, one row per thread, when the 18 vectors take at most
-// half of the LDS: two workgroups then share a CU, which beats keeping the
-// step stages in registers.  Otherwise the step stages stay in registers,
-// one row per thread up to 512 rows and two rows per thread above.
+// Launch shape (measured on the TFIM trajectories at d = 512 and 1024):
+// (threads, rows per thread NR, register stages RS) for d rows, false when
+// d does not fit.  Every stage in LDS, one row per thread, when the 18
+// vectors take at most half of the LDS: two workgroups then share a CU,
+// which beats keeping the step stages in registers.  Otherwise the step
+// stages stay in registers, one row per thread up to 512 rows and two rows
+// per thread above.
+template <typename R>
+bool shape_mc(int d, int *threads, int *NR, int *RS) {
+    const int t1 = ((d + QA_WAVE - 1) / QA_WAVE) * QA_WAVE;
+    if (t1 <= 2 * kMcThreads && mc_lds_bytes<R>(t1, t1, 0) <= mc_lds_cap(0)) {
+        *threads = t1, *NR = 1, *RS = 0;
+        return true;
+    }
+    if (t1 <= kMcThreads && mc_lds_bytes<R>(t1, t1, kRegStagesMax) <= mc_lds_cap(kRegStagesMax)) {
+        *threads = t1, *NR = 1, *RS = kRegStagesMax;
+        return true;
+    }
+    const int rows = (d + 1) / 2;
+    const int t2 = ((rows + QA_WAVE - 1) / QA_WAVE) * QA_WAVE;
+    if (t2 > kMcThreads ||
+        mc_lds_bytes<R>(2 * t2, t2, kRegStagesMax) > mc_lds_cap(kRegStagesMax))
+        return false;
+    *threads = t2, *NR = 2, *RS = kRegStagesMax;
+    return true;
+}
+
 template <typename R, bool SMALL = false>
 int launch_mc(const McArgs &A, int blocks, hipStream_t s) {
-    const int t1 = ((A.d + QA_WAVE - 1) / QA_WAVE) * QA_WAVE;
-    if (t1 <= 2 * kMcThreads && mc_lds_bytes<R>(t1, t1, 0) <= kLdsBudget / 2)
-        return launch_mc_cfg<R, 1, 0, SMALL>(A, t1, blocks, s);
-    if (t1 <= kMcThreads && mc_lds_bytes<R>(t1, t1, kRegStagesMax) <= kLdsBudget)
-        return launch_mc_cfg<R, 1, kRegStagesMax, SMALL>(A, t1, blocks, s);
-    const int rows = (A.d + 1) / 2;
-    const int t2 = ((rows + QA_WAVE - 1) / QA_WAVE) * QA_WAVE;
-    if (t2 > kMcThreads || mc_lds_bytes<R>(2 * t2, t2, kRegStagesMax) > kLdsBudget) {
+    int threads, NR, RS;
+    if (!shape_mc<R>(A.d, &threads, &NR, &RS)) {
         qa::set_error("qa_mc_run: d = %d does not fit the trajectory kernel", A.d);
         return QA_ERR_ARG;
     }
-    return launch_mc_cfg<R, 2, kRegStagesMax, SMALL>(A, t2, blocks, s);
+    if (NR == 1 && RS == 0) return launch_mc_cfg<R, 1, 0, SMALL>(A, threads, blocks, s);
+    if (NR == 1) return launch_mc_cfg<R, 1, kRegStagesMax, SMALL>(A, threads, blocks, s);
+    return launch_mc_cfg<R, 2, kRegStagesMax, SMALL>(A, threads, blocks, s);
 }
 
 }  // namespace
@@ -1093,6 +1146,36 @@ int launch_mc(const McArgs &A, int blocks, hipStream_t s) {
 extern "C" {
 
 int qa_mc_tableau_size(void) { return (int)sizeof(McTableau); }
+
+int qa_mc_lds_plan(int d, int h_width, int nfun, int ntimes, int fp32, int small,
+                   long long *out) {
+    QA_REQUIRE(out != nullptr, "qa_mc_lds_plan: null output");
+    QA_REQUIRE(d >= 1 && h_width >= 0 && nfun >= 0 && ntimes >= 0, "qa_mc_lds_plan: bad size");
+    McArgs A;
+    memset(&A, 0, sizeof(A));
+    A.d = d;
+    A.h_width = h_width;
+    A.nfun = nfun;
+    A.ntimes = ntimes;
+    int threads, NR, RS;
+    const bool ok = fp32 ? shape_mc<float>(d, &threads, &NR, &RS)
+                         : shape_mc<double>(d, &threads, &NR, &RS);
+    if (!ok) {
+        qa::set_error("qa_mc_lds_plan: d = %d does not fit the trajectory kernel", d);
+        return QA_ERR_ARG;
+    }
+    const McPlan P = fp32 ? plan_mc<float>(A, threads, NR, RS, small != 0)
+                          : plan_mc<double>(A, threads, NR, RS, small != 0);
+    out[0] = threads;
+    out[1] = NR;
+    out[2] = RS;
+    out[3] = (long long)P.lds;
+    out[4] = (long long)kLdsStatic;
+    out[5] = P.hl ? 1 : 0;
+    out[6] = P.fun_lds;
+    out[7] = (long long)mc_lds_cap(RS);
+    return QA_OK;
+}
 
 // Pack a tableau (host arrays as for qa_rk_create) into the device layout
 // the trajectory kernel reads; `out` is a host buffer of

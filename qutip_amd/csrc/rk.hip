@@ -2024,26 +2024,34 @@ int qa_rk_destroy(qa_rk_solver *S) {
 
 void *qa_rk_stream(qa_rk_solver *S) { return S ? (void *)S->stream : nullptr; }
 
-int qa_rk_set_functionals(qa_rk_solver *S, int nfun, const int *kinds, const int *fptr,
-                          const int *rows, const int *cols, const double *w) {
-    QA_REQUIRE(S != nullptr, "qa_rk_set_functionals: null solver");
-    QA_REQUIRE(nfun >= 0 && fptr && fptr[0] == 0, "qa_rk_set_functionals: bad fptr");
+int qa_functionals_check(int n, int nfun, const int *kinds, const int *fptr, const int *rows,
+                         const int *cols) {
+    QA_REQUIRE(nfun >= 0 && fptr && fptr[0] == 0, "functionals: bad fptr");
+    QA_REQUIRE(nfun == 0 || kinds, "functionals: null kinds");
+    for (int f = 0; f < nfun; ++f)
+        QA_REQUIRE(fptr[f + 1] >= fptr[f], "functionals: fptr must be non-decreasing");
     const int nnz = fptr[nfun];
-    int maxlen = 0;
-    for (int f = 0; f < nfun; ++f) {
-        QA_REQUIRE(fptr[f + 1] >= fptr[f], "qa_rk_set_functionals: fptr must be non-decreasing");
-        maxlen = std::max(maxlen, fptr[f + 1] - fptr[f]);
-    }
+    QA_REQUIRE(nnz == 0 || (rows && cols), "functionals: null entries");
     for (int p = 0; p < nnz; ++p)
-        QA_REQUIRE(cols[p] >= 0 && cols[p] < S->n, "functional entry %d: column %d out of range",
+        QA_REQUIRE(cols[p] >= 0 && cols[p] < n, "functional entry %d: column %d out of range",
                    p, cols[p]);
     for (int f = 0; f < nfun; ++f) {
         if (kinds[f]) {
             for (int p = fptr[f]; p < fptr[f + 1]; ++p)
-                QA_REQUIRE(rows[p] >= 0 && rows[p] < S->n,
+                QA_REQUIRE(rows[p] >= 0 && rows[p] < n,
                            "functional entry %d: row %d out of range", p, rows[p]);
         }
     }
+    return QA_OK;
+}
+
+int qa_rk_set_functionals(qa_rk_solver *S, int nfun, const int *kinds, const int *fptr,
+                          const int *rows, const int *cols, const double *w) {
+    QA_REQUIRE(S != nullptr, "qa_rk_set_functionals: null solver");
+    QA_CHECK(qa_functionals_check(S->n, nfun, kinds, fptr, rows, cols));
+    const int nnz = fptr[nfun];
+    int maxlen = 0;
+    for (int f = 0; f < nfun; ++f) maxlen = std::max(maxlen, fptr[f + 1] - fptr[f]);
     QA_HIP(hipStreamSynchronize(S->stream));
     void *olds[] = {S->fptr_dev, S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev};
     QA_HIP(hipStreamSynchronize(S->stream));  // no launch of this engine still reads them

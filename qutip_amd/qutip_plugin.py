@@ -12,7 +12,9 @@ qutip/solver/solver_base.py:462-492):
   matmul, expect, inner, add, iadd, mul, imul, zeros, norm.l2, norm.frobenius,
   ode.wrmn_error, trace, column_stack, column_unstack;
 * the integrator ``hip_rk`` for SESolver / MESolver / MCSolver
-  (``options={'method': 'hip_rk'}``).
+  (``options={'method': 'hip_rk'}``); its ``run(tlist)`` is one device loop;
+* ``HipMCSolver``: qutip's MCSolver with every trajectory in one persistent
+  kernel (the subclass route of nm_mcsolve.py:324-422).
 
 qutip is not installed where this package is developed and tested, so this
 module is written against the documented API and imported lazily; nothing
@@ -20,7 +22,7 @@ else in qutip_amd depends on it.
 """
 import numpy as np
 
-__all__ = ["register", "available"]
+__all__ = ["register", "available", "HipMCSolver"]
 
 _registered = False
 
@@ -130,14 +132,36 @@ def register():
     qd.to.register_aliases(["hipcsr", "HipCSR"], HipCSRData)
     qd.to.register_group(["hip"], dense=HipDenseData, sparse=HipCSRData)
 
-    # -- specialisations (exact dispatcher signatures) -------------------------
-    def matmul_hcsr_hdense(left, right, scale=1, out=None):
-        o = None if out is None else out.hip
-        res = hd.matmul(left.hip, right.hip, scale, out=o)
-        return out if out is not None else HipDenseData(res, copy=False)
+    # -- specialisations: each has exactly its dispatcher's signature
+    # (dispatch.pyx:287-294), e.g. matmul(left, right, scale=1)
+    # (matmul.pyx:1018-1029) and zeros(rows, cols) (constant.py:18-27) ----
+    def matmul_hcsr_hdense(left, right, scale=1):
+        return HipDenseData(hd.matmul(left.hip, right.hip, scale), copy=False)
+
+    # host CSR / Dia operators met on the plain path (a QobjEvo's elements
+    # stay host types) are uploaded once and kept, keyed by object identity
+    # (the reference's Data objects are immutable once built)
+    dev_cache = {}
+
+    def device_operator(op):
+        hit = dev_cache.get(id(op))
+        if hit is not None and hit[0] is op:
+            return hit[1]
+        csr = op if isinstance(op, qd.CSR) else qd.to(qd.CSR, op)
+        dev = hd.HipCSR(csr.as_scipy())
+        if len(dev_cache) >= 256:
+            dev_cache.pop(next(iter(dev_cache)))
+        dev_cache[id(op)] = (op, dev)
+        return dev
+
+    def matmul_host_hdense(left, right, scale=1):
+        return HipDenseData(hd.matmul(device_operator(left), right.hip, scale), copy=False)
 
     def expect_hcsr_hdense(op, state):
         return hd.expect(op.hip, state.hip)
+
+    def expect_host_hdense(op, state):
+        return hd.expect(device_operator(op), state.hip)
 
     def inner_hdense(left, right, scalar_is_ket=False):
         return hd.inner(left.hip, right.hip, scalar_is_ket)
@@ -156,11 +180,11 @@ def register():
         hd.imul(matrix.hip, value)
         return matrix
 
-    def zeros_hdense(rows, cols, fortran=True):
+    def zeros_hdense(rows, cols):
         return HipDenseData(hd.zeros(rows, cols), copy=False)
 
-    def l2_hdense(matrix):
-        return hd.norm_l2(matrix.hip)
+    def l2_hdense(vector):
+        return hd.norm_l2(vector.hip)
 
     def frobenius_hdense(matrix):
         return hd.frobenius(matrix.hip)
@@ -177,8 +201,16 @@ def register():
     def column_unstack_hdense(matrix, rows):
         return HipDenseData(hd.column_unstack(matrix.hip, rows), copy=False)
 
-    qd.matmul.add_specialisations([(HipCSRData, HipDenseData, HipDenseData, matmul_hcsr_hdense)])
-    qd.expect.add_specialisations([(HipCSRData, HipDenseData, expect_hcsr_hdense)])
+    qd.matmul.add_specialisations([
+        (HipCSRData, HipDenseData, HipDenseData, matmul_hcsr_hdense),
+        (qd.CSR, HipDenseData, HipDenseData, matmul_host_hdense),
+        (qd.Dia, HipDenseData, HipDenseData, matmul_host_hdense),
+    ])
+    qd.expect.add_specialisations([
+        (HipCSRData, HipDenseData, expect_hcsr_hdense),
+        (qd.CSR, HipDenseData, expect_host_hdense),
+        (qd.Dia, HipDenseData, expect_host_hdense),
+    ])
     qd.inner.add_specialisations([(HipDenseData, HipDenseData, inner_hdense)])
     qd.add.add_specialisations([(HipDenseData, HipDenseData, HipDenseData, add_hdense)])
     qd.iadd.add_specialisations([(HipDenseData, HipDenseData, HipDenseData, iadd_hdense)])
@@ -205,9 +237,77 @@ def register():
             t, y = IntegratorHipRK.get_state(self, copy)
             return t, HipDenseData(y, copy=False)
 
+        def _wrap(self, y):
+            return HipDenseData(y, copy=False)
+
     for cls in (SESolver, MESolver, MCSolver):
         cls.add_integrator(QutipIntegratorHipRK, "hip_rk")
+
+    global HipMCSolver
+    HipMCSolver = _make_mc_solver(MCSolver)
     _registered = True
+
+
+HipMCSolver = None  # set by register(): qutip's MCSolver with batched trajectories
+
+
+def _mirror(q):
+    """A qutip Qobj (or constant QobjEvo) as this package's host Qobj."""
+    import scipy.sparse as sp
+
+    from . import core
+    if hasattr(q, "isconstant") and not hasattr(q, "full"):
+        if not q.isconstant:
+            raise NotImplementedError("the batched trajectory kernel needs constant operators")
+        q = q(0)
+    try:
+        m = sp.csr_matrix(q.to("CSR").data.as_scipy())
+    except Exception:  # a data type without a CSR conversion
+        m = np.asarray(q.full())
+    return core.Qobj(m, dims=q.dims)
+
+
+def _make_mc_solver(MCSolver):
+    """A subclass of qutip's MCSolver (qutip/solver/mcsolve.py:424-821) whose
+    ``run`` advances all trajectories in the persistent trajectory kernel
+    (qa_mc_run) instead of one MCIntegrator per trajectory: the route the
+    reference leaves for it is a subclass overriding ``run``, as
+    NonMarkovianMCSolver does (nm_mcsolve.py:324-422).  Options are qutip's
+    plus the device keys of qutip_amd's MCSolver ('precision', 'max_draws',
+    'max_jumps', 'grid'); the result is qutip_amd's McResult (average_expect,
+    std_expect, runs_expect, col_times, col_which, seeds)."""
+    from .solver.mcsolve import MCSolver as BatchMC
+    from .solver.tableaus import TABLEAUS
+
+    class HipMCSolver(MCSolver):
+        name = "mcsolve (hip batched)"
+
+        def __init__(self, H, c_ops, *, options=None):
+            opts = dict(options or {})
+            dev = {k: opts.pop(k) for k in ("precision", "max_draws", "max_jumps", "grid")
+                   if k in opts}
+            super().__init__(H, c_ops, options=opts)
+            mine = {k: v for k, v in self.options.items() if k in BatchMC.solver_options}
+            if mine.get("method") not in TABLEAUS:
+                mine["method"] = "vern7"
+            mine.update(dev)
+            self._batch = BatchMC(_mirror(H), [_mirror(c) for c in c_ops], options=mine)
+
+        def run(self, state, tlist, ntraj=1, *, args=None, e_ops=(), target_tol=None,
+                timeout=None, seeds=None):
+            if isinstance(e_ops, dict):
+                e_ops = list(e_ops.values())
+            elif e_ops is None:
+                e_ops = []
+            elif not isinstance(e_ops, (list, tuple)):
+                e_ops = [e_ops]
+            if any(callable(O) and not hasattr(O, "dims") for O in e_ops):
+                raise NotImplementedError("the batched trajectory kernel takes operator e_ops")
+            return self._batch.run(_mirror(state), tlist, ntraj,
+                                   e_ops=[_mirror(O) for O in e_ops], seeds=seeds,
+                                   target_tol=target_tol)
+
+    return HipMCSolver
 
 
 def _dense_from_array(arr):

@@ -182,6 +182,8 @@ class RKEngine:
                 _lib.check(rc, "qa_rk_create")
         self.handle = h
         self.nfun = 0
+        # entries between consecutive states in a states_out buffer
+        self.npad = self.sell.nslices * (64 // self.lanes) if self.sell is not None else self.n
         self._y_dev = _data.HipDense((self.n, 1))
 
     def __del__(self):
@@ -442,14 +444,49 @@ def _accepts_out(fn):
     return getattr(fn, "__name__", "") in ("matmul_data",)
 
 
+def _as_array(op):
+    """A constant operator (QobjEvo at t=0, Qobj, qutip Data or array) as a
+    dense complex128 numpy array."""
+    if callable(op) and not hasattr(op, "full"):
+        op = op(0.0)
+    if hasattr(op, "full"):
+        return np.asarray(op.full(), dtype=np.complex128)
+    if sp.issparse(op):
+        return op.toarray().astype(np.complex128)
+    if hasattr(op, "to_array"):
+        return np.asarray(op.to_array(), dtype=np.complex128)
+    return np.asarray(op, dtype=np.complex128)
+
+
+def matrix_form_operators(owner):
+    """(H_nh, [c_ops]) as dense arrays when ``owner`` is a constant
+    matrix-form Lindblad right-hand side: QuTiP's LindbladMatrixForm exposes
+    ``H_nh`` (QobjEvo) and ``c_ops`` (list of QobjEvo) as public attributes
+    (qutip/core/cy/lindblad_matrix_form.pxd:7-8) and cannot be called
+    (lindblad_matrix_form.pyx:217-222).  None for any other owner or for a
+    time-dependent one."""
+    if owner is None or not (hasattr(owner, "H_nh") and hasattr(owner, "c_ops")):
+        return None
+    Hnh, cops = owner.H_nh, list(owner.c_ops or [])
+    if not all(getattr(x, "isconstant", True) for x in [Hnh] + cops):
+        return None
+    return _as_array(Hnh), [_as_array(c) for c in cops]
+
+
 def constant_operator(derivative):
     """The constant generator behind ``derivative`` when it is the bound
     ``matmul_data`` of a constant QobjEvo (qutip/solver/integrator/_rhs.pyx:
-    52-57 finds the QobjEvo the same way); None otherwise."""
+    52-57 finds the QobjEvo the same way); None otherwise, including for
+    owners that cannot be called (a matrix-form right-hand side)."""
     owner = getattr(derivative, "__self__", None)
     if owner is None or not getattr(owner, "isconstant", False):
         return None
-    q = owner(0.0)
+    if hasattr(owner, "H_nh"):
+        return None
+    try:
+        q = owner(0.0)
+    except (NotImplementedError, TypeError):
+        return None
     d = getattr(q, "data", None)
     if d is None:
         return None
@@ -496,12 +533,12 @@ class IntegratorHipRK(Integrator):
         self._small_stats = None
         self._engine_pending = False  # the engine still waits for _small_start
         owner = getattr(self.derivative, "__self__", None)
-        dense = getattr(owner, "dense_operators", None) if o["fused"] else None
+        dense = matrix_form_operators(owner) if o["fused"] else None
         op = constant_operator(self.derivative) if (o["fused"] and dense is None) else None
         if dense is not None:
             self._engine = RKEngine(None, o["tableau"], o["rtol"], o["atol"], o["nsteps"],
                                     o["first_step"], o["min_step"], o["max_step"],
-                                    o["interpolate"], dense=dense(),
+                                    o["interpolate"], dense=dense,
                                     dense_precision=o["dense_precision"])
             dp = self._engine.dense_precision
             self.name = ("hip_rk (fused, matrix form, FP64 banded)" if dp == "fp64_banded"
@@ -608,6 +645,34 @@ class IntegratorHipRK(Integrator):
             self._plain.integrate(t, step=True)
             self._check()
         return self.get_state(copy)
+
+    def _wrap(self, y):
+        """The Data object handed to callers (the qutip plugin wraps it)."""
+        return y
+
+    def run(self, tlist):
+        """Integrator.run (qutip/solver/integrator/integrator.py:197-212):
+        yields (t, state) for every t of ``tlist[1:]``.  On the fused
+        engine the whole list is one device loop (qa_rk_run with states
+        out): the states land in one device buffer, and each yielded state
+        is a view of it that is valid until the next call of run (callers
+        copy what they keep, as Result does for this method)."""
+        if self._engine is None:
+            for t in tlist[1:]:
+                yield self.integrate(t, False)
+            return
+        times = np.asarray(tlist[1:], dtype=np.float64)
+        if len(times) == 0:
+            return
+        self._leave_small()
+        eng = self._engine
+        step = 16 * eng.npad
+        buf = _data.DeviceBuffer(step * len(times))
+        self._run_states = buf  # the views below share it
+        eng.run(times, states_out=buf.ptr)
+        for k, t in enumerate(times):
+            y = _data.HipDense((eng.n, 1), buf=buf, offset=step * k)
+            yield float(t), self._wrap(y.reshape(self._shape))
 
     def run_functionals(self, tlist, kinds, fptr, rows, cols, w, at_start=False):
         """Fused Solver.run: functionals of the state at every time of

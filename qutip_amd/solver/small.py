@@ -86,6 +86,11 @@ class SmallSolve:
         key = tuple((a.shape, a.tobytes()) for a in arrs)
         if key == getattr(self, "_fun_key", None):
             return  # the same functionals are on the device already
+        # the kernel reads the functionals from the device without bounds
+        # checks, so their host copies are checked against n first
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        _lib.host_call("qa_functionals_check", self.n, len(arrs[0]), ip(arrs[0]), ip(arrs[1]),
+                       ip(arrs[2]), ip(arrs[3]))
         self._fun_key = key
         self.nfun = len(kinds)
         up = lambda x, dt: _data.DeviceBuffer.from_array(

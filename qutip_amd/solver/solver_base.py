@@ -222,16 +222,30 @@ class Solver:
             out.append(self._cast(O, v))
         return out
 
+    @staticmethod
+    def _check_eops(ops, d):
+        """Operator e_ops must act on the state's space (state dims[0] = d):
+        the reference's expect raises ValueError on a dims mismatch
+        (qutip/core/expect.py:85-90), and the device functionals index the
+        state by the operator's rows and columns."""
+        for O in ops:
+            if isinstance(O, (Qobj, QobjEvo)):
+                if list(O.dims[1]) != list(d) or O.shape[0] != O.shape[1]:
+                    raise ValueError(f"incompatible dimensions {O.dims[1]} and {d}")
+            elif not callable(O):
+                raise TypeError(f"{O!r} has unsupported type {type(O)!r}.")
+
     # -- run ------------------------------------------------------------------
     def run(self, state0, tlist, *, e_ops=None, args=None):
         t_start = time.time()
         tlist = np.asarray(tlist, dtype=np.float64)
+        ops = self._eop_list(e_ops)
+        self._check_eops(ops, state0.dims[0])
         y0 = self._prepare_state(state0)
         integ = self._integrator
         integ.set_state(tlist[0], y0)
         stats = self._initialize_stats()
         res = Result(e_ops, self._options, solver=self.name, stats=stats)
-        ops = self._eop_list(e_ops)
         store_states = bool(self._options.get("store_states"))
         if self._options.get("store_states") is None:
             store_states = len(ops) == 0
@@ -251,7 +265,7 @@ class Solver:
             if started:
                 res.add_expect_block(tlist, self._values_from_functionals(ops, vals))
             else:
-                res.add_expect_values(tlist[0], self._expect_on_device(ops, y0))
+                res.add_expect_values(tlist[0], self._expect_on_device(ops, y0, tlist[0]))
                 if vals is not None:
                     res.add_expect_block(tlist[1:], self._values_from_functionals(ops, vals))
             if store_final:
@@ -271,7 +285,10 @@ class Solver:
             return st.reshape((d, d))
         return st
 
-    def _expect_on_device(self, ops, st):
+    def _expect_on_device(self, ops, st, t):
+        """e_ops values of the state at time t: operators by the device
+        expect kernels, QobjEvo at O(t), callables as f(t, state)
+        (result.py:247-262)."""
         view = self._state_view(st)
         vals = []
         if self._normalize_output:
@@ -281,9 +298,9 @@ class Solver:
                 nrm = view.trace().real
         for O in ops:
             if callable(O) and not isinstance(O, (Qobj, QobjEvo)):
-                vals.append(O(0.0, self._restore_state(st)))
+                vals.append(O(float(t), self._restore_state(st)))
                 continue
-            Oq = O(0.0) if isinstance(O, QobjEvo) else O
+            Oq = O(float(t)) if isinstance(O, QobjEvo) else O
             v = _data.expect(_data.HipCSR(Oq.to_csr()), view)
             if self._normalize_output:
                 v = v / nrm
@@ -291,7 +308,7 @@ class Solver:
         return vals
 
     def _add_generic(self, res, t, st, ops, store_states, store_final):
-        res.add_expect_values(t, self._expect_on_device(ops, st))
+        res.add_expect_values(t, self._expect_on_device(ops, st, t))
         if store_states or store_final:
             res.add_state(self._restore_state(st), store_states, store_final)
 

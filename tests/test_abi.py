@@ -106,3 +106,51 @@ def test_banded_entry_point_rejects_bad_diagonals():
     assert rc == _lib.QA_ERR_ARG and not h.value
     rc, h = create([0] * 50, list(range(-25, 25)), 0)  # 100 products of H_nh's diagonals
     assert rc == _lib.QA_ERR_ARG and b"products of diagonals" in L.qa_last_error()
+
+
+def _plan(d, h_width=0, nfun=0, ntimes=0, fp32=0, small=1):
+    out = (ctypes.c_int64 * 8)()
+    _lib.host_call("qa_mc_lds_plan", d, h_width, nfun, ntimes, fp32, small, out)
+    return dict(zip(["threads", "rows", "reg_stages", "lds", "static", "hl", "fun_lds", "cap"],
+                    list(out)))
+
+
+@pytest.mark.parametrize("d,w,fp32", [(512, 8, 0), (512, 8, 1), (256, 12, 0), (400, 8, 0),
+                                      (1024, 12, 0), (1024, 12, 1), (64, 4, 0), (20, 4, 0)])
+def test_lds_plan_counts_the_static_tableau(d, w, fp32):
+    """Every launch shape keeps dynamic + static LDS inside the CU's
+    160 KiB (half of it for the two-workgroup shape), with the whole-solve
+    room for functionals and times and the operator in LDS; the case that
+    overflowed once is d = 512 at ELL width 8 with functionals."""
+    budget = 160 * 1024
+    for nfun, ntimes in ((0, 0), (3, 100), (40, 2000)):
+        p = _plan(d, w, nfun, ntimes, fp32, 1)
+        share = budget // 2 if p["reg_stages"] == 0 else budget
+        assert p["static"] > 0
+        assert p["lds"] + p["static"] <= share, p
+        assert p["lds"] <= p["cap"]
+        if p["fun_lds"]:
+            assert p["fun_lds"] >= 256 and p["fun_lds"] % 16 == 0
+        # the room for functionals never lowers the workgroups a CU holds
+        base = _plan(d, w, 0, 0, fp32, 1)
+        assert budget // (p["lds"] + p["static"]) == budget // (base["lds"] + base["static"])
+    t = _plan(d, w, 3, 100, fp32, 0)  # the trajectory kernel reserves nothing extra
+    assert t["fun_lds"] == 0 and t["hl"] == 0
+
+
+def test_lds_plan_rejects_oversize():
+    with pytest.raises(ValueError):
+        _plan(4096)
+
+
+def test_functionals_check():
+    i32 = lambda *v: (ctypes.c_int * max(len(v), 1))(*v)
+    ok = _lib.host_call("qa_functionals_check", 4, 2, i32(0, 1), i32(0, 2, 3), i32(0, 0, 3),
+                        i32(1, 3, 2))
+    assert ok == 0
+    with pytest.raises(ValueError, match="column"):
+        _lib.host_call("qa_functionals_check", 4, 1, i32(0), i32(0, 1), i32(0), i32(4))
+    with pytest.raises(ValueError, match="row"):
+        _lib.host_call("qa_functionals_check", 4, 1, i32(1), i32(0, 1), i32(-1), i32(0))
+    with pytest.raises(ValueError, match="fptr"):
+        _lib.host_call("qa_functionals_check", 4, 1, i32(0), i32(1, 1), i32(0), i32(0))
