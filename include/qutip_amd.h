@@ -129,6 +129,19 @@ int qa_axpby_z(int64_t n, double are, double aim, const void *x, double bre, dou
 int qa_wrmn_error_z(int64_t n, const void *diff, const void *state, double atol, double rtol,
                     double *result, void *stream);
 
+/* qa_wrmn_error_z without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The per-workgroup
+ * partial sums and the result live in a fixed device array of the library
+ * (8208 bytes per device: 512 partial sums and 2 result words, kept until
+ * the process ends); the launch never uses more than 512 workgroups, so any
+ * n > 0 fits.  Same kernels and grid as qa_wrmn_error_z, hence the same
+ * bits.  Every call on one device shares that array: calls that overlap on
+ * one device (other threads, other streams) race and must be serialised by
+ * the caller. */
+int qa_wrmn_error_z_noalloc(int64_t n, const void *diff, const void *state, double atol,
+                            double rtol, double *result, void *stream);
+
 /* sum |x|^2 into *result (host).  norm.l2 / frobenius are its square root
  * (qutip/core/data/norm.pyx:127-135, 195-216). */
 int qa_norm2_z(int64_t n, const void *x, double *result, void *stream);

@@ -57,6 +57,7 @@ SIGNATURES = {
     "qa_lincomb_z": [i64, vp, i32, C.POINTER(vp), P_f64, vp, vp],
     "qa_axpby_z": [i64, f64, f64, vp, f64, f64, vp, vp, vp],
     "qa_wrmn_error_z": [i64, vp, vp, f64, f64, P_f64, vp],
+    "qa_wrmn_error_z_noalloc": [i64, vp, vp, f64, f64, P_f64, vp],
     "qa_norm2_z": [i64, vp, P_f64, vp],
     "qa_inner_z": [i64, vp, vp, P_f64, vp],
     "qa_trace_z": [i32, vp, P_f64, vp],

@@ -148,8 +148,8 @@ __global__ __launch_bounds__(256) void axpby_kernel(int64_t n, double2 alpha,
 }
 
 // ---------------------------------------------------------------------------
-// Reductions: per-workgroup partial sums, the last workgroup adds them in a
-// fixed order (deterministic) and writes res[0..1].
+// Reductions: per-workgroup partial sums, then a one-workgroup kernel adds
+// them in a fixed order (deterministic) and writes res[0..1].
 // kind 0: wrmn sum (|d|/(atol + rtol |s|))^2 -> sqrt(sum / n)
 // kind 1: sum |x|^2
 // kind 2: sum conj(x) y (complex)
@@ -250,6 +250,16 @@ __global__ __launch_bounds__(256) void expect_csr_kernel(int kind, int nrows,
 // ---------------------------------------------------------------------------
 namespace {
 constexpr int kMaxRedBlocks = 512;
+}  // namespace
+
+// Scratch of the allocation-free error norm (qa_wrmn_error_z_noalloc): the
+// partial sums of at most kMaxRedBlocks workgroups, then the two result
+// words; 8208 bytes per device, held until the process ends.  Host code
+// never passes this name to a kernel: it stands for a host-side shadow, not
+// the device address, which scratch_address() looks up per device.
+static __device__ __align__(256) double scratch[2 * kMaxRedBlocks + 2];
+
+namespace {
 std::mutex g_red_mutex;
 struct RedState {
     int device = -1;
@@ -269,6 +279,26 @@ int red_scratch(RedScratch *S) {
     char *b = (char *)st.buf;
     S->partials = (double *)b;
     S->res = (double *)(b + 2 * kMaxRedBlocks * sizeof(double));
+    return QA_OK;
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array at its own address).
+std::mutex g_scratch_mutex;
+double *g_scratch_addr[16] = {};
+
+int scratch_address(double **out) {
+    int dev = 0;
+    QA_HIP(hipGetDevice(&dev));
+    QA_REQUIRE(dev >= 0 && dev < 16, "device index %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        QA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        QA_REQUIRE(p != nullptr, "scratch: no device address on device %d", dev);
+        g_scratch_addr[dev] = (double *)p;
+    }
+    *out = g_scratch_addr[dev];
     return QA_OK;
 }
 
@@ -505,6 +535,27 @@ int qa_wrmn_error_z(int64_t n, const void *diff, const void *state, double atol,
                     double *result, void *stream) {
     QA_REQUIRE(n > 0 && diff && state && result, "qa_wrmn_error_z: bad arguments");
     return run_reduce(0, n, diff, state, atol, rtol, 1, result, 1, stream);
+}
+
+int qa_wrmn_error_z_noalloc(int64_t n, const void *diff, const void *state, double atol,
+                            double rtol, double *result, void *stream) {
+    QA_REQUIRE(n > 0 && diff && state && result, "qa_wrmn_error_z_noalloc: bad arguments");
+    // the grid never exceeds kMaxRedBlocks workgroups, so the partial sums of
+    // any n fit the fixed scratch; the kernels and the grid are those of
+    // qa_wrmn_error_z, hence the same result bits
+    const unsigned nb = red_blocks(n);
+    QA_REQUIRE(nb <= (unsigned)kMaxRedBlocks,
+               "qa_wrmn_error_z_noalloc: %u partial sums exceed the scratch of %d", nb,
+               kMaxRedBlocks);
+    double *p = nullptr;
+    QA_CHECK(scratch_address(&p));
+    RedScratch S;
+    S.partials = p;
+    S.res = p + 2 * kMaxRedBlocks;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(reduce_kernel, dim3(nb), dim3(256), 0, s, 0, n, (const double2 *)diff,
+                       (const double2 *)state, atol, rtol, (int64_t)1, S);
+    return finish_reduction(S, nb, 1, n, result, 1, s);
 }
 
 int qa_norm2_z(int64_t n, const void *x, double *result, void *stream) {
