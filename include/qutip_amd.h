@@ -233,6 +233,44 @@ int qa_rk_destroy(qa_rk_solver *s);
 /* out = A x with the engine's generator (device vectors; waits for the
  * result).  For the dense engine this is LindbladMatrixForm.matmul_data. */
 int qa_rk_apply(qa_rk_solver *s, const void *x, void *out);
+/* The same at time t: out = A(t) x for a time-dependent generator
+ * (QobjEvo.matmul_data(t, state), qutip/core/cy/qobjevo.pyx:1103-1116). */
+int qa_rk_apply_at(qa_rk_solver *s, double t, const void *x, void *out);
+
+/* ---- time-dependent generators ---------------------------------------------
+ * QuTiP's list form A(t) = A_0 + sum_e c_e(t) A_e (QobjEvo elements,
+ * qutip/core/cy/_element.pyx:388-446; coefficients, qutip/core/cy/
+ * coefficient.pyx:88) with every c_e a program the device evaluates itself:
+ * a stack machine over complex128 values, instruction = op | (arg << 8):
+ *   0 push consts[arg], 1 push t, 2 add, 3 sub, 4 mul, 5 div, 6 pow, 7 neg,
+ *   8 function arg (0 sin, 1 cos, 2 tan, 3 exp, 4 log, 5 log10, 6 sqrt,
+ *     7 abs, 8 real, 9 imag, 10 conj, 11 norm = |z|^2, 12 arg, 13 sinh,
+ *     14 cosh, 15 tanh, 16 asin, 17 acos, 18 atan, 19 asinh, 20 acosh,
+ *     21 atanh, 22 erf of the real part),
+ *   9 push the interpolation table at consts[arg] evaluated at t
+ *     (InterCoefficient, coefficient.pyx:412-560: consts[arg] = (npts,
+ *     rows), consts[arg + 1] = (dt or 0, t0), npts times (real parts), then
+ *     `rows` rows of npts polynomial coefficients, the constant term last),
+ *   10..15 compare real parts (<, <=, >, >=, ==, !=) to 1 or 0,
+ *   16 select (c, a, b -> re c != 0 ? a : b).
+ * Program e is code[start[e] .. start[e + 1]) (ncoef <= 8, start[0] = 0,
+ * start[ncoef] = ncode); consts: nconst complex128 (host arrays). */
+
+/* Make a coded SELL-C engine (qa_rk_create_coded) time-dependent: entries
+ * [vstart[e], vstart[e + 1]) of its value table are element e + 1's values
+ * and are multiplied by c_e(t) at every stage time t_prev + c_i dt
+ * (explicit_rk.pyx:358-389, 399-410); the other entries are A_0's.  The
+ * stage kernels evaluate the programs themselves, so the step loop stays on
+ * the device. */
+int qa_rk_set_coefficients(qa_rk_solver *s, int ncoef, const int *start, const int *vstart,
+                           int ncode, const int *code, int nconst, const double *consts);
+
+/* out_dev[e * ntimes + k] = c_e(times_dev[k]) (complex128, device), the
+ * programs as above: the device evaluation of Coefficient.__call__
+ * (coefficient.pyx:88-175) at many times. */
+int qa_coef_eval(int ncoef, const int *start, int ncode, const int *code, int nconst,
+                 const double *consts, int ntimes, const double *times_dev, void *out_dev,
+                 void *stream);
 void *qa_rk_stream(qa_rk_solver *s);
 
 /* Output functionals evaluated at every output time: functional f has

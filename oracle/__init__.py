@@ -9,7 +9,7 @@ Contents and how each function is pinned (a function without a pin says so):
 
 * ``tableaus``   published Verner-7 / Tsitouras-5 coefficients.  Pinned by the
                  order conditions and by measured convergence order
-                 (tests/test_oracle_rk.py).
+                 (tests/test_oracle.py).
 * ``rk``         numpy restatement of qutip/solver/integrator/explicit_rk.pyx
                  (step control, error norm, dense output).  Pinned by closed
                  forms (dy/dt = lambda y, the analytic system of
@@ -21,9 +21,16 @@ Contents and how each function is pinned (a function without a pin says so):
                  Hermiticity preservation) and by an independent dense
                  construction on small sizes, plus the closed-form decay of
                  qutip/tests/solver/test_mesolve.py:84-95.
+                 ``matrix_form_rhs`` (LindbladMatrixForm, the A + A^dag
+                 form) is pinned against that Liouvillian on Hermitian
+                 states of random systems.
 * ``exact``      vec(rho(t)) = expm(L t) vec(rho0) through
                  scipy.sparse.linalg.expm_multiply -- a library routine, the
                  FP64 reference the parity tests compare against.
+                 ``mesolve_expect_rk`` (``rk`` on ``matrix_form_rhs`` at
+                 rtol 1e-10, for systems whose superoperator is too large)
+                 is pinned against ``mesolve_expect`` on small random
+                 systems.
 * ``mc``         numpy restatement of the mcsolve jump loop
                  (qutip/solver/mcsolve.py:245-406).  Pinned statistically
                  (ensemble mean within 3 sigma of the ``exact`` master-equation

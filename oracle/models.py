@@ -97,6 +97,28 @@ def heff(H, c_ops):
     return Hf
 
 
+def matrix_form_rhs(H, c_ops):
+    """The matrix-form Lindblad right-hand side on vec(rho), following
+    LindbladMatrixForm.matmul_data with assume_hermitian_state
+    (qutip/core/cy/lindblad_matrix_form.pyx:170-186): H_nh = H - i/2 sum
+    c^dag c (:92-97), A = -i H_nh rho + 1/2 sum c rho c^dag, drho/dt =
+    A + A^dag.  Dense numpy arrays; returns f(t, y) for oracle/rk.py."""
+    H = np.asarray(H.toarray() if sp.issparse(H) else H, dtype=CD)
+    cs = [np.asarray(c.toarray() if sp.issparse(c) else c, dtype=CD) for c in c_ops]
+    Hnh = H.copy()
+    for c in cs:
+        Hnh = Hnh - 0.5j * (c.conj().T @ c)
+    d = H.shape[0]
+
+    def f(t, y):
+        rho = y.reshape((d, d), order="F")
+        A = -1j * (Hnh @ rho)
+        for c in cs:
+            A = A + 0.5 * ((c @ rho) @ c.conj().T)
+        return (A + A.conj().T).ravel(order="F")
+    return f
+
+
 def vec(rho):
     """Column stacking, superoperator.py:291-302."""
     return np.asarray(rho).ravel(order="F")

@@ -308,28 +308,51 @@ def expect(oper, state):
 # ---------------------------------------------------------------------------
 # random objects used by the serving workload (qutip/core/random_objects.py)
 # ---------------------------------------------------------------------------
+def _generator(seed):
+    """A numpy Generator from a seed, or the Generator itself
+    (random_objects.py:67-83)."""
+    if isinstance(seed, np.random.Generator):
+        return seed
+    return np.random.default_rng(seed)
+
+
 def rand_herm(N, density=0.3, seed=None):
-    """Random sparse Hermitian matrix with about density*N^2 entries (eigenvalues
-    not fixed).  Same distributional family as random_objects.py:198-283
-    (complex Gaussian entries, symmetrised); the exact draws of the reference
-    are not reproduced."""
-    rng = np.random.default_rng(seed)
-    nvals = max(1, int(N * N * density / 2))
-    r = rng.integers(0, N, nvals)
-    c = rng.integers(0, N, nvals)
-    v = (rng.standard_normal(nvals) + 1j * rng.standard_normal(nvals)) / 2
-    M = sp.coo_matrix((v, (r, c)), shape=(N, N)).tocsr()
-    M = M + M.conj().T
-    M.sum_duplicates()
+    """Random Hermitian operator, the "fill" distribution of
+    random_objects.py:198-290: H = (X + X^dag) / 2 with the entries of X
+    uniform in [-1, 1] + i [-1, 1].  Below density 0.5 X is sparse with the
+    entry count of _rand_herm_sparse (:286-308: positions drawn without
+    replacement); from 0.5 up X is dense and entries are removed in
+    symmetric pairs (_rand_herm_dense, :311-327).  The draws are made in
+    the same order from the same numpy Generator."""
+    gen = _generator(seed)
+    if density < 0.5:
+        target = 1 - (1 - density) ** 0.5
+        num = int(max((N * N - 0.666 * N) * target + 0.666 * N * density, 1))
+        data = (2 * gen.random(num) - 1) + (2 * gen.random(num) - 1) * 1j
+        rows, cols = np.divmod(gen.choice(N * N, num, replace=False), N)
+        M = sp.coo_matrix((data, (rows, cols)), shape=(N, N), dtype=complex).tocsr()
+        M = (0.5 * (M + M.conj().T)).tocsr()
+        M.sort_indices()
+    else:
+        M = (2 * gen.random((N, N)) - 1) + 1j * (2 * gen.random((N, N)) - 1)
+        M = 0.5 * (M + M.conj().T)
+        num = int(max(N * (N - 0.666) * (1 - density ** 0.5) + 0.666 * N * (1 - density), 1))
+        for index in gen.choice(N * N, num, replace=False):
+            r, c = divmod(int(index), N)
+            M[c, r] = 0
+            M[r, c] = 0
     return Qobj(M, dims=[[N], [N]], isherm=True)
 
 
 def rand_dm(N, seed=None):
-    """Random density matrix from a Ginibre matrix, rho = G G^dag / tr
-    (random_objects.py:508-604, 'ginibre' distribution, full rank)."""
-    rng = np.random.default_rng(seed)
-    G = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
-    rho = G @ G.conj().T
+    """Random density matrix, the full-rank "ginibre" distribution of
+    random_objects.py:508-643 (_rand_dm_ginibre): X with standard complex
+    normal entries (_randnz with norm 1, :86-106: one normal(size=(N, N, 2))
+    draw, real and imaginary parts last), rho = X X^dag / tr(X X^dag)."""
+    gen = _generator(seed)
+    g = gen.normal(size=(N, N, 2))
+    X = g[..., 0] + 1j * g[..., 1]
+    rho = X @ X.conj().T
     rho /= np.trace(rho)
     return Qobj(rho, dims=[[N], [N]], isherm=True)
 

@@ -20,6 +20,9 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) {
     return make_double2(a.x + b.x, a.y + b.y);
 }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) {
+    return make_double2(a.x - b.x, a.y - b.y);
+}
 // acc += a * b (complex, four fused multiply-adds)
 __device__ __forceinline__ void cfma(double2 &acc, double2 a, double2 b) {
     acc.x = fma(a.x, b.x, acc.x);

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "qa_api_internal.h"
+#include "qa_coef.h"
 #include "qa_common.h"
 
 using namespace qa;
@@ -196,6 +197,7 @@ struct EpiArgs {
     double *partials;
     RKCtl *ctl;
     int stage, run_phase, input, epi, nterms;
+    double cstage;                 // the stage's time is t_prev + cstage dt
     double coef_self, ecoef_self;  // coefficients of this stage's own k
     // the earlier stages with a non-zero coefficient, compacted: k_{tj[t]}
     // enters the next input / y_front with tc[t] and the error with te[t]
@@ -210,13 +212,16 @@ struct StageArgs : EpiArgs {
     const int *col;
     const V *vals;
     int nslices;
-    // coded operator (rk_stage_kernel<.., .., true>): 16-bit codes packed
+    // coded operator (FORM_CODED, FORM_CODED_UNIFORM): 16-bit codes packed
     // four to a word, the value and column-offset tables
     const uint2 *code;
     const double2 *vtab;
     const int *otab;
     int vbits;
     int uwords;  // > 0: every slice uwords words per lane (FORM_CODED_UNIFORM)
+    // time-dependent generator (rk_stage_kernel<.., .., .., true>): the
+    // coefficient programs and their value-table ranges
+    CoefSet td;
 };
 
 // Operator forms of rk_stage_kernel.
@@ -307,9 +312,10 @@ __device__ __forceinline__ double epi_post(const EpiArgs &P, const RKCtl *ctl, c
     return 0.0;
 }
 
-template <typename V, int L, int FORM>
+template <typename V, int L, int FORM, bool TD>
 __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
     constexpr bool CODED = FORM != FORM_PLAIN;
+    static_assert(CODED || !TD, "a time-dependent generator needs the coded form");
     RKCtl *ctl = P.ctl;
     __shared__ double2 vt[CODED ? kCodeTabMax : 1];
     __shared__ int ot[CODED ? kCodeTabMax : 1];
@@ -336,8 +342,16 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
     }
     const int phase = ctl->phase;
     const StageVecs v = stage_vecs(P, ctl);
+    double t_stage = 0.0;
+    if constexpr (TD) t_stage = ctl->t_prev + P.cstage * v.dt;
     if constexpr (CODED) __syncthreads();
     if (phase != P.run_phase) return;  // uniform over the whole grid
+    if constexpr (TD) {
+        // A(t) = A_0 + sum_e c_e(t) A_e: scale each element's table entries
+        // by its coefficient at this stage's time
+        __shared__ double2 cf[kMaxCoef];
+        coef_scale_table(P.td, t_stage, vt, cf);
+    }
     const int base = sp0, steps = (sp1 - sp0) / QA_WAVE;
     double part = 0.0;
     if (slice < P.nslices) {
@@ -363,6 +377,38 @@ __global__ __launch_bounds__(kStepBlocks) void rk_stage_kernel(StageArgs<V> P) {
         const double s = block_sum(part, smem);
         if (threadIdx.x == 0) P.partials[blockIdx.x] = s;
     }
+}
+
+// The value table of a time-dependent coded generator at time t: vt_out =
+// vtab with each element's entries scaled by its coefficient (one
+// workgroup; for the ungated products at a given time -- the first-step
+// estimate and apply).
+__global__ __launch_bounds__(256) void coef_table_kernel(CoefSet S, double t,
+                                                         const double2 *__restrict__ vtab,
+                                                         double2 *__restrict__ vt_out, int nv) {
+    __shared__ double2 cf[kMaxCoef];
+    if ((int)threadIdx.x < S.ncoef) cf[threadIdx.x] = coef_eval(S, threadIdx.x, t);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+        double2 v = vtab[i];
+        if (i >= S.vstart[0] && i < S.vstart[S.ncoef]) {
+            int e = 0;
+            while (e + 1 < S.ncoef && i >= S.vstart[e + 1]) ++e;
+            v = cmul(v, cf[e]);
+        }
+        vt_out[i] = v;
+    }
+}
+
+// Coefficient programs at many times: out[e * ntimes + k] = c_e(times[k])
+// (data-layer evaluation of a compiled coefficient set).
+__global__ __launch_bounds__(256) void coef_eval_kernel(CoefSet S, int ntimes,
+                                                        const double *__restrict__ times,
+                                                        double2 *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)S.ncoef * ntimes) return;
+    const int e = (int)(i / ntimes), k = (int)(i % ntimes);
+    out[i] = coef_eval(S, e, times[k]);
 }
 
 // ---- banded (DIA) Lindblad right-hand side ----------------------------------
@@ -469,13 +515,14 @@ __global__ __launch_bounds__(256) void dia_rhs_kernel(DiaOp D, const double2 *__
 // Cd_c = c^dag, followed by the stage epilogue.  Complex products use the
 // real 4M decomposition on 16x16 MFMA tiles (one wavefront per tile):
 // FP32 inputs and accumulation (v_mfma_f32_16x16x4f32), or each FP32 input
-// split into bf16 hi + lo with the three significant products hi*hi +
-// hi*lo + lo*hi (v_mfma_f32_16x16x32_bf16, FP32 accumulation).
+// split into bf16 pieces on v_mfma_f32_16x16x32_bf16 with FP32 accumulation:
+// hi + lo with the three significant products (bf16 x3, about 16 bits), or
+// hi + mid + lo with six (bf16 x6, FP32 accuracy).
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 struct DenseOp {
-    int N, nc, prec;  // prec 0: FP32 MFMA, 1: bf16 x3 split
+    int N, nc, prec;  // prec 0: FP32 MFMA, 1: bf16 x3 split, 2: bf16 x6 split
     int nsplit;       // waves per 16 x 16 tile (split K)
     int tiled;        // 1: 32 x 32 workgroup tiles staged through LDS (large N)
     const float2 *Hm, *Hp, *C, *Cd;  // N x N each; C / Cd: nc blocks
@@ -535,18 +582,48 @@ __device__ __forceinline__ void zmma_f32(v4f &re, v4f &im, const TA *__restrict_
     im += im2;
 }
 
-__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
-    hi = (__bf16)x;
-    lo = (__bf16)(x - (float)hi);
+// x as NP bf16 pieces x = p_0 + p_1 + ... (+ a remainder below the last
+// piece): each piece rounds what the previous ones left, and the remainder
+// x - (p_0 + ... + p_q) is exact in FP32, so two pieces keep 16 bits of
+// x's mantissa and three keep all 24 (Henry, Tang & Heinecke, ARITH 2019).
+template <int NP>
+__device__ __forceinline__ void split_bf16(float x, int u, v8bf (&p)[NP]) {
+    float r = x;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const __bf16 h = (__bf16)r;
+        p[q][u] = h;
+        r -= (float)h;
+    }
 }
 
-// Same product with bf16 x3 splitting (32-wide K blocks; lane (r, kq)
-// holds k = kk + 8 kq + u, u < 8, so its B piece is contiguous).  The next
-// block's operands are prefetched as in zmma_f32.
-template <typename TA, typename TB>
-__device__ __forceinline__ void zmma_bf16x3(v4f &re, v4f &im, const TA *__restrict__ A,
-                                            const TB *__restrict__ B, int N, int i0, int j0,
-                                            int lane, int k0, int k1) {
+template <int NP>
+__device__ __forceinline__ void negate_bf16(const v8bf (&p)[NP], v8bf (&n)[NP]) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) n[q] = -p[q];
+}
+
+// acc += sum of the piece products a_i b_j with i + j < NP, the largest
+// terms first: NP = 2 is bf16 x3 (hi hi, hi lo, lo hi), NP = 3 is bf16 x6
+// (the dropped products are below 2^-24 relative: FP32 accuracy).
+template <int NP>
+__device__ __forceinline__ v4f mma_pieces(const v8bf (&a)[NP], const v8bf (&b)[NP], v4f acc) {
+#pragma unroll
+    for (int s = 0; s < NP; ++s)
+#pragma unroll
+        for (int i = 0; i <= s; ++i)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[s - i], acc, 0, 0, 0);
+    return acc;
+}
+
+// Same product as zmma_f32 with the inputs split into NP bf16 pieces
+// (32-wide K blocks; lane (r, kq) holds k = kk + 8 kq + u, u < 8, so its B
+// piece is contiguous).  The next block's operands are prefetched as in
+// zmma_f32.
+template <int NP, typename TA, typename TB>
+__device__ __forceinline__ void zmma_bf16(v4f &re, v4f &im, const TA *__restrict__ A,
+                                          const TB *__restrict__ B, int N, int i0, int j0,
+                                          int lane, int k0, int k1) {
     const int r = lane & 15, kq = lane >> 4;
     const TA *pa = A + (size_t)(8 * kq) * N + i0 + r;
     const TB *pb = B + (size_t)(j0 + r) * N + 8 * kq;
@@ -565,38 +642,20 @@ __device__ __forceinline__ void zmma_bf16x3(v4f &re, v4f &im, const TA *__restri
             an[u] = tof(pa[(size_t)(kn + u) * N]);
             bn[u] = tof(pb[kn + u]);
         }
-        v8bf arh, arl, aih, ail, naih, nail, brh, brl, bih, bil;
+        v8bf ar[NP], ai[NP], nai[NP], br[NP], bi[NP];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            __bf16 h, l;
-            split_bf16(a[u].x, h, l);
-            arh[u] = h;
-            arl[u] = l;
-            split_bf16(a[u].y, h, l);
-            aih[u] = h;
-            ail[u] = l;
-            naih[u] = -h;
-            nail[u] = -l;
-            split_bf16(b[u].x, h, l);
-            brh[u] = h;
-            brl[u] = l;
-            split_bf16(b[u].y, h, l);
-            bih[u] = h;
-            bil[u] = l;
+            split_bf16<NP>(a[u].x, u, ar);
+            split_bf16<NP>(a[u].y, u, ai);
+            split_bf16<NP>(b[u].x, u, br);
+            split_bf16<NP>(b[u].y, u, bi);
         }
-        // re += ar br - ai bi ; im += ar bi + ai br   (hi*hi + hi*lo + lo*hi each)
-        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brh, re, 0, 0, 0);
-        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brl, re, 0, 0, 0);
-        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, brh, re, 0, 0, 0);
-        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bih, re2, 0, 0, 0);
-        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bil, re2, 0, 0, 0);
-        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail, bih, re2, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bih, im, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bil, im, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, bih, im, 0, 0, 0);
-        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brh, im2, 0, 0, 0);
-        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brl, im2, 0, 0, 0);
-        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, brh, im2, 0, 0, 0);
+        negate_bf16<NP>(ai, nai);
+        // re += ar br - ai bi ; im += ar bi + ai br
+        re = mma_pieces<NP>(ar, br, re);
+        re2 = mma_pieces<NP>(nai, bi, re2);
+        im = mma_pieces<NP>(ar, bi, im);
+        im2 = mma_pieces<NP>(ai, br, im2);
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             a[u] = an[u];
@@ -620,7 +679,9 @@ template <typename TA, typename TB>
 __device__ __forceinline__ void zmma(const DenseOp &D, v4f &re, v4f &im, const TA *A, const TB *B,
                                      int i0, int j0, int lane, int k0, int k1) {
     if (D.prec == 1)
-        zmma_bf16x3(re, im, A, B, D.N, i0, j0, lane, k0, k1);
+        zmma_bf16<2>(re, im, A, B, D.N, i0, j0, lane, k0, k1);
+    else if (D.prec == 2)
+        zmma_bf16<3>(re, im, A, B, D.N, i0, j0, lane, k0, k1);
     else
         zmma_f32(re, im, A, B, D.N, i0, j0, lane, k0, k1);
 }
@@ -767,48 +828,41 @@ __device__ __forceinline__ void gt_store(GemmTile &L, int buf, int tid, const fl
     }
 }
 
+// The wave's 16 x 16 tile += A_chunk B_chunk from LDS buffer `buf`, with
+// the inputs split into NP bf16 pieces (32-wide K steps).
+template <int NP>
+__device__ __forceinline__ void gt_mma_bf16(const GemmTile &L, int buf, int wi, int wj, int lane,
+                                            v4f &re, v4f &im, v4f &re2, v4f &im2) {
+    const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int st = 0; st < kGK / 32; ++st) {
+        v8bf ar[NP], ai[NP], nai[NP], br[NP], bi[NP];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const float2 a = L.a[buf][32 * st + 8 * kq + u][16 * wi + r];
+            const float2 b = L.b[buf][16 * wj + r][32 * st + 8 * kq + u];
+            split_bf16<NP>(a.x, u, ar);
+            split_bf16<NP>(a.y, u, ai);
+            split_bf16<NP>(b.x, u, br);
+            split_bf16<NP>(b.y, u, bi);
+        }
+        negate_bf16<NP>(ai, nai);
+        re = mma_pieces<NP>(ar, br, re);
+        re2 = mma_pieces<NP>(nai, bi, re2);
+        im = mma_pieces<NP>(ar, bi, im);
+        im2 = mma_pieces<NP>(ai, br, im2);
+    }
+}
+
 // The wave's 16 x 16 tile += A_chunk B_chunk from LDS buffer `buf`.
 __device__ __forceinline__ void gt_mma(const GemmTile &L, int buf, int prec, int wi, int wj,
                                        int lane, v4f &re, v4f &im) {
     const int r = lane & 15, kq = lane >> 4;
     v4f re2 = {0.f, 0.f, 0.f, 0.f}, im2 = {0.f, 0.f, 0.f, 0.f};
     if (prec == 1) {
-#pragma unroll
-        for (int st = 0; st < kGK / 32; ++st) {
-        v8bf arh, arl, aih, ail, naih, nail, brh, brl, bih, bil;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const float2 a = L.a[buf][32 * st + 8 * kq + u][16 * wi + r];
-            const float2 b = L.b[buf][16 * wj + r][32 * st + 8 * kq + u];
-            __bf16 h, l;
-            split_bf16(a.x, h, l);
-            arh[u] = h;
-            arl[u] = l;
-            split_bf16(a.y, h, l);
-            aih[u] = h;
-            ail[u] = l;
-            naih[u] = -h;
-            nail[u] = -l;
-            split_bf16(b.x, h, l);
-            brh[u] = h;
-            brl[u] = l;
-            split_bf16(b.y, h, l);
-            bih[u] = h;
-            bil[u] = l;
-        }
-        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brh, re, 0, 0, 0);
-        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, brl, re, 0, 0, 0);
-        re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, brh, re, 0, 0, 0);
-        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bih, re2, 0, 0, 0);
-        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, bil, re2, 0, 0, 0);
-        re2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail, bih, re2, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bih, im, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arh, bil, im, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(arl, bih, im, 0, 0, 0);
-        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brh, im2, 0, 0, 0);
-        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, brl, im2, 0, 0, 0);
-        im2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, brh, im2, 0, 0, 0);
-        }
+        gt_mma_bf16<2>(L, buf, wi, wj, lane, re, im, re2, im2);
+    } else if (prec == 2) {
+        gt_mma_bf16<3>(L, buf, wi, wj, lane, re, im, re2, im2);
     } else {
 #pragma unroll
         for (int st = 0; st < kGK / 4; ++st) {
@@ -1295,6 +1349,12 @@ struct qa_rk_solver {
     const void *code = nullptr;
     const double2 *vtab = nullptr;
     const int *otab = nullptr;
+    // time-dependent coded generator: coefficient programs (td.code and
+    // td.consts in td_block) and a value table at a given time (vtab_t)
+    int td_on = 0;
+    CoefSet td = {};
+    void *td_block = nullptr;
+    double2 *vtab_t = nullptr;
     // tableau
     int order = 0, rk_step = 0, rk_extra = 0, dorder = 0, interpolate = 1, adaptive = 1;
     int first_dense_from_front = 0;
@@ -1347,6 +1407,7 @@ void fill_epi(EpiArgs &P, qa_rk_solver *S, int i) {
     P.partials = S->partials;
     P.ctl = S->ctl;
     P.stage = i;
+    P.cstage = i < (int)S->c.size() ? S->c[i] : 0.0;
     double coef[kMaxStages], ecoef[kMaxStages];
     for (int j = 0; j < kMaxStages; ++j) coef[j] = ecoef[j] = 0.0;
     const int ne = S->rk_extra;
@@ -1409,6 +1470,7 @@ StageArgs<V> stage_args(qa_rk_solver *S, int i) {
     P.otab = S->otab;
     P.vbits = S->vbits;
     P.uwords = S->uwords;
+    P.td = S->td;
     return P;
 }
 
@@ -1417,15 +1479,15 @@ int stage_blocks(const qa_rk_solver *S) {
     return (int)((threads + kStepBlocks - 1) / kStepBlocks);
 }
 
-template <typename V, int FORM>
+template <typename V, int FORM, bool TD = false>
 int launch_sparse_stage(qa_rk_solver *S, int i) {
     StageArgs<V> P = stage_args<V>(S, i);
     const dim3 grid((unsigned)stage_blocks(S)), block(kStepBlocks);
     switch (S->lanes) {
-        case 1: hipLaunchKernelGGL((rk_stage_kernel<V, 1, FORM>), grid, block, 0, S->stream, P); break;
-        case 2: hipLaunchKernelGGL((rk_stage_kernel<V, 2, FORM>), grid, block, 0, S->stream, P); break;
-        case 4: hipLaunchKernelGGL((rk_stage_kernel<V, 4, FORM>), grid, block, 0, S->stream, P); break;
-        case 8: hipLaunchKernelGGL((rk_stage_kernel<V, 8, FORM>), grid, block, 0, S->stream, P); break;
+        case 1: hipLaunchKernelGGL((rk_stage_kernel<V, 1, FORM, TD>), grid, block, 0, S->stream, P); break;
+        case 2: hipLaunchKernelGGL((rk_stage_kernel<V, 2, FORM, TD>), grid, block, 0, S->stream, P); break;
+        case 4: hipLaunchKernelGGL((rk_stage_kernel<V, 4, FORM, TD>), grid, block, 0, S->stream, P); break;
+        case 8: hipLaunchKernelGGL((rk_stage_kernel<V, 8, FORM, TD>), grid, block, 0, S->stream, P); break;
         default:
             qa::set_error("lanes per row must be 1, 2, 4 or 8");
             return QA_ERR_ARG;
@@ -1469,6 +1531,9 @@ int launch_stage(qa_rk_solver *S, int i) {
         QA_CHECK_LAUNCH();
         return QA_OK;
     }
+    if (S->coded && S->td_on)
+        return S->uwords ? launch_sparse_stage<double2, FORM_CODED_UNIFORM, true>(S, i)
+                         : launch_sparse_stage<double2, FORM_CODED, true>(S, i);
     if (S->coded)
         return S->uwords ? launch_sparse_stage<double2, FORM_CODED_UNIFORM>(S, i)
                          : launch_sparse_stage<double2, FORM_CODED>(S, i);
@@ -1476,8 +1541,9 @@ int launch_stage(qa_rk_solver *S, int i) {
                         : launch_sparse_stage<double2, FORM_PLAIN>(S, i);
 }
 
-// out = A x (ungated): the derivative for the first-step estimate.
-int engine_rhs(qa_rk_solver *S, const double2 *x, double2 *out) {
+// out = A(t) x (ungated): the derivative for the first-step estimate and
+// apply (t only matters for a time-dependent generator).
+int engine_rhs(qa_rk_solver *S, const double2 *x, double2 *out, double t = 0.0) {
     if (S->kind == 2) {
         hipLaunchKernelGGL(dia_rhs_kernel, dim3((unsigned)S->nparts), dim3(256), 0, S->stream,
                            S->dia, x, out);
@@ -1505,6 +1571,15 @@ int engine_rhs(qa_rk_solver *S, const double2 *x, double2 *out) {
         hipLaunchKernelGGL(dense_rhs_kernel, dim3(nt, nt), block, 0, S->stream, S->dop, x, out);
         QA_CHECK_LAUNCH();
         return QA_OK;
+    }
+    if (S->coded && S->td_on) {
+        const int nv = 1 << S->vbits;
+        hipLaunchKernelGGL(coef_table_kernel, dim3(1), dim3(256), 0, S->stream, S->td, t, S->vtab,
+                           S->vtab_t, nv);
+        QA_CHECK_LAUNCH();
+        return qa_sell_coded_matvec_z(S->n, S->n, S->lanes, S->nslices, S->slice_ptr, S->code,
+                                      S->vtab_t, S->otab, S->vbits, x, out, 1.0, 0.0, 0,
+                                      S->stream);
     }
     if (S->coded)
         return qa_sell_coded_matvec_z(S->n, S->n, S->lanes, S->nslices, S->slice_ptr, S->code,
@@ -1615,7 +1690,8 @@ void free_all(qa_rk_solver *S) {
     if (S->graph) (void)hipGraphDestroy(S->graph);
     if (S->event) (void)hipEventDestroy(S->event);
     void *ptrs[] = {S->block, S->dense_block, S->ctl, S->bi_dev, S->times_dev, S->fun_out_dev, S->fptr_dev,
-                    S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev};
+                    S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev, S->td_block,
+                    S->vtab_t};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (void *p : S->retired) (void)hipFree(p);
@@ -1939,8 +2015,9 @@ int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, co
     QA_REQUIRE(out != nullptr, "qa_rk_create_dense: null output");
     *out = nullptr;
     QA_REQUIRE(N >= 16 && N % 16 == 0, "qa_rk_create_dense: N must be a positive multiple of 16");
-    QA_REQUIRE(precision == 0 || precision == 1, "qa_rk_create_dense: precision 0 (fp32) or 1 (bf16x3)");
-    QA_REQUIRE(precision == 0 || N % 32 == 0, "qa_rk_create_dense: bf16x3 needs N % 32 == 0");
+    QA_REQUIRE(precision >= 0 && precision <= 2,
+               "qa_rk_create_dense: precision 0 (fp32), 1 (bf16x3) or 2 (bf16x6)");
+    QA_REQUIRE(precision == 0 || N % 32 == 0, "qa_rk_create_dense: the bf16 forms need N % 32 == 0");
     QA_REQUIRE(nc >= 0 && nc <= 64 && H_nh && (nc == 0 || c_ops), "qa_rk_create_dense: operators");
     int rc = check_tableau(rk_step, rk_extra, dorder, a, b, c, bi, nsteps);
     if (rc) return rc;
@@ -1983,9 +2060,9 @@ int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, co
     S->dop.nc = nc;
     S->dop.prec = precision;
     // split K over up to kMaxSplit waves while every wave keeps whole K steps
-    // (16 for FP32, 32 for bf16 x3)
+    // (16 for FP32, 32 for the bf16 splits)
     {
-        const int kstep = precision == 1 ? 32 : 16;
+        const int kstep = precision >= 1 ? 32 : 16;
         int ns = kMaxSplit;
         while (ns > 1 && N % (ns * kstep) != 0) ns /= 2;
         S->dop.nsplit = ns;
@@ -2006,11 +2083,119 @@ int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, co
 }
 
 // out = L(x) with the engine's generator (device vectors of n entries).
-int qa_rk_apply(qa_rk_solver *S, const void *x, void *out) {
+int qa_rk_apply(qa_rk_solver *S, const void *x, void *out) { return qa_rk_apply_at(S, 0.0, x, out); }
+
+int qa_rk_apply_at(qa_rk_solver *S, double t, const void *x, void *out) {
     QA_REQUIRE(S && x && out, "qa_rk_apply: null argument");
-    int rc = engine_rhs(S, (const double2 *)x, (double2 *)out);
+    int rc = engine_rhs(S, (const double2 *)x, (double2 *)out, t);
     if (rc) return rc;
     QA_HIP(hipStreamSynchronize(S->stream));
+    return QA_OK;
+}
+
+namespace {
+// Host-side checks of a coefficient set: program bounds, the constants an
+// instruction names, the stack depth, and value-table ranges inside a table
+// of nv entries.  Fills the CoefSet's host fields.
+int check_coefficients(int ncoef, const int *start, const int *vstart, int ncode, const int *code,
+                       int nconst, const double *consts, int nv, CoefSet &T) {
+    QA_REQUIRE(ncoef >= 1 && ncoef <= kMaxCoef, "coefficients: 1 <= ncoef <= %d", kMaxCoef);
+    QA_REQUIRE(start && code && start[0] == 0 && start[ncoef] == ncode,
+               "coefficients: start[0] must be 0 and start[ncoef] the code length");
+    QA_REQUIRE(nconst >= 0 && (nconst == 0 || consts), "coefficients: constants");
+    for (int e = 0; e < ncoef; ++e) {
+        QA_REQUIRE(start[e + 1] > start[e], "coefficients: program %d is empty", e);
+        int depth = 0;
+        for (int pc = start[e]; pc < start[e + 1]; ++pc) {
+            const int op = code[pc] & 0xff, arg = code[pc] >> 8;
+            QA_REQUIRE(op >= CO_CONST && op <= CO_SELECT, "coefficients: bad op %d", op);
+            if (op == CO_CONST) QA_REQUIRE(arg >= 0 && arg < nconst, "coefficients: constant %d", arg);
+            if (op == CO_FN) QA_REQUIRE(arg >= FN_SIN && arg <= FN_ERF, "coefficients: function %d", arg);
+            if (op == CO_INTERP) {
+                QA_REQUIRE(arg >= 0 && arg + 2 <= nconst, "coefficients: table at %d", arg);
+                const int npts = (int)consts[2 * arg], nrow = (int)consts[2 * arg + 1];
+                QA_REQUIRE(npts >= 1 && nrow >= 1 && nrow <= 8 &&
+                               (long long)arg + 2 + npts + (long long)nrow * npts <= nconst,
+                           "coefficients: table at %d out of range", arg);
+            }
+            if (op == CO_CONST || op == CO_T || op == CO_INTERP) ++depth;
+            else if (op == CO_SELECT) depth -= 2;
+            else if (op != CO_NEG && op != CO_FN) depth -= 1;
+            QA_REQUIRE(depth >= 1 && depth < kCoefStack, "coefficients: program %d stack", e);
+        }
+        QA_REQUIRE(depth == 1, "coefficients: program %d leaves %d values", e, depth);
+    }
+    T.ncoef = ncoef;
+    for (int e = 0; e <= kMaxCoef; ++e) {
+        T.start[e] = e <= ncoef ? start[e] : ncode;
+        T.vstart[e] = 0;
+    }
+    if (vstart) {
+        for (int e = 0; e <= ncoef; ++e) {
+            QA_REQUIRE(vstart[e] >= 0 && vstart[e] <= nv && (e == 0 || vstart[e] >= vstart[e - 1]),
+                       "coefficients: value ranges must be increasing inside the table");
+            T.vstart[e] = vstart[e];
+        }
+        for (int e = ncoef + 1; e <= kMaxCoef; ++e) T.vstart[e] = vstart[ncoef];
+    }
+    return QA_OK;
+}
+
+int upload_coefficients(CoefSet &T, int ncode, const int *code, int nconst, const double *consts,
+                        void **block, hipStream_t s) {
+    const size_t cb = ((sizeof(int) * (size_t)ncode + 255) / 256) * 256;
+    const size_t kb = sizeof(double2) * (size_t)std::max(nconst, 1);
+    QA_HIP(hipMalloc(block, cb + kb));
+    QA_HIP(hipMemcpyAsync(*block, code, sizeof(int) * ncode, hipMemcpyHostToDevice, s));
+    if (nconst)
+        QA_HIP(hipMemcpyAsync((char *)*block + cb, consts, sizeof(double2) * nconst,
+                              hipMemcpyHostToDevice, s));
+    QA_HIP(hipStreamSynchronize(s));
+    T.code = (const int *)*block;
+    T.consts = (const double2 *)((char *)*block + cb);
+    return QA_OK;
+}
+}  // namespace
+
+int qa_rk_set_coefficients(qa_rk_solver *S, int ncoef, const int *start, const int *vstart,
+                           int ncode, const int *code, int nconst, const double *consts) {
+    QA_REQUIRE(S != nullptr, "qa_rk_set_coefficients: null solver");
+    QA_REQUIRE(S->kind == 0 && S->coded, "qa_rk_set_coefficients: needs a coded SELL-C engine");
+    QA_REQUIRE(vstart != nullptr, "qa_rk_set_coefficients: null value ranges");
+    CoefSet T = {};
+    QA_CHECK(check_coefficients(ncoef, start, vstart, ncode, code, nconst, consts, 1 << S->vbits, T));
+    QA_HIP(hipStreamSynchronize(S->stream));
+    if (S->td_block) S->retired.push_back(S->td_block);
+    S->td_block = nullptr;
+    QA_CHECK(upload_coefficients(T, ncode, code, nconst, consts, &S->td_block, S->stream));
+    if (!S->vtab_t) {
+        void *p = nullptr;
+        QA_HIP(hipMalloc(&p, sizeof(double2) * kCodeTabMax));
+        S->vtab_t = (double2 *)p;
+    }
+    S->td = T;
+    S->td_on = 1;
+    S->graph_dirty = true;
+    return QA_OK;
+}
+
+int qa_coef_eval(int ncoef, const int *start, int ncode, const int *code, int nconst,
+                 const double *consts, int ntimes, const double *times_dev, void *out_dev,
+                 void *stream) {
+    QA_REQUIRE(ntimes >= 0 && (ntimes == 0 || (times_dev && out_dev)), "qa_coef_eval: buffers");
+    CoefSet T = {};
+    QA_CHECK(check_coefficients(ncoef, start, nullptr, ncode, code, nconst, consts, 0, T));
+    if (ntimes == 0) return QA_OK;
+    hipStream_t s = (hipStream_t)stream;
+    void *block = nullptr;
+    QA_CHECK(upload_coefficients(T, ncode, code, nconst, consts, &block, s));
+    const int64_t total = (int64_t)ncoef * ntimes;
+    hipLaunchKernelGGL(coef_eval_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       T, ntimes, times_dev, (double2 *)out_dev);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(block);
+    if (e != hipSuccess) return qa::hip_fail(e, "coef_eval_kernel", __FILE__, __LINE__);
     return QA_OK;
 }
 
@@ -2107,7 +2292,7 @@ int qa_rk_set_state(qa_rk_solver *S, double t0, const void *y0_dev) {
         double norm = sqrt(norm2);
         const double tol = S->atol + norm * S->rtol;
         double2 *k0 = S->K, *k1 = S->K + S->npad;
-        rc = engine_rhs(S, S->y1, k0);
+        rc = engine_rhs(S, S->y1, k0, t0);
         if (rc) return rc;
         if (norm <= S->atol) norm = 1.0;
         rc = qa_norm2_z(S->n, k0, &kn2, s);
@@ -2124,7 +2309,7 @@ int qa_rk_set_state(qa_rk_solver *S, double t0, const void *y0_dev) {
         const double cf[1] = {dt1 / 100};
         rc = qa_lincomb_z(S->npad, S->y1, 1, vecs, cf, S->tmp, s);
         if (rc) return rc;
-        rc = engine_rhs(S, S->tmp, k1);
+        rc = engine_rhs(S, S->tmp, k1, t0 + dt1 / 100);
         if (rc) return rc;
         rc = qa_norm2_z(S->n, k1, &kn2, s);
         if (rc) return rc;

@@ -353,18 +353,27 @@ UNIFORM_WORDS_MAX = 6     # words per lane of the uniform-width coded form
 UNIFORM_PAD_MAX = 1.25    # uniform when it stores at most this many times the words
 
 
-def code_dictionary(A, fp32=False):
+def code_dictionary(A, fp32=False, elem=None):
     """The value and column-offset dictionaries of a sorted CSR matrix:
     (value index per entry, offset index per entry, vtab complex128[2^vbits],
     otab int32[2^(16 - vbits)], vbits, pad code) or None when no 16-bit
     split fits.  Values are told apart by their bits (-0.0 and 0.0 stay
     distinct; ``fp32``: rounded to complex64 first); value 0 and offset 0
-    are always present (the padding entry, code ``pad``)."""
+    are always present (the padding entry, code ``pad``).
+
+    ``elem`` (one small int per stored entry): the value table is keyed on
+    (element, value) and ordered by element, so the entries of element e
+    own one contiguous range of the table (element 0 first, with the
+    padding value); a seventh item, the range starts per element plus the
+    end, is returned."""
     n = A.shape[0]
     vals = A.data.astype(np.complex128)
     if fp32:
         vals = vals.astype(np.complex64).astype(np.complex128)
     bits = np.concatenate([np.zeros((1, 2), np.uint64), vals.view(np.uint64).reshape(-1, 2)])
+    if elem is not None:
+        el = np.concatenate([[0], np.asarray(elem, dtype=np.uint64)])
+        bits = np.concatenate([el[:, None], bits], axis=1)
     ubits, vinv = np.unique(bits, axis=0, return_inverse=True)
     rn = np.diff(A.indptr).astype(np.int64)
     rows = np.repeat(np.arange(n, dtype=np.int64), rn)
@@ -376,14 +385,19 @@ def code_dictionary(A, fp32=False):
     vinv = vinv.reshape(-1).astype(np.int64)
     oinv = oinv.reshape(-1).astype(np.int64)
     vtab = np.zeros(1 << vbits, dtype=np.complex128)
-    vtab[:len(ubits)] = np.ascontiguousarray(ubits).view(np.complex128).reshape(-1)
+    vbits_only = np.ascontiguousarray(ubits[:, -2:])
+    vtab[:len(ubits)] = vbits_only.view(np.complex128).reshape(-1)
     otab = np.zeros(1 << (16 - vbits), dtype=np.int32)
     otab[:len(uoffs)] = uoffs
     pad = int(vinv[0]) | (int(oinv[0]) << vbits)
-    return vinv[1:], oinv[1:], vtab, otab, vbits, pad
+    if elem is None:
+        return vinv[1:], oinv[1:], vtab, otab, vbits, pad
+    nel = int(np.max(elem)) + 1 if len(elem) else 1
+    vstart = np.searchsorted(ubits[:, 0], np.arange(nel + 1, dtype=np.uint64)).astype(np.int32)
+    return vinv[1:], oinv[1:], vtab, otab, vbits, pad, vstart
 
 
-def sell_coded_arrays(A, lanes=1, fp32=False, uniform="auto"):
+def sell_coded_arrays(A, lanes=1, fp32=False, uniform="auto", elem=None):
     """Coded SELL-C layout of a CSR matrix, or None when its distinct values
     and distinct column offsets (column - row) fit no 16-bit split
     (``code_split``: up to 1024 values with 64 offsets ... 64 values with
@@ -404,17 +418,22 @@ def sell_coded_arrays(A, lanes=1, fp32=False, uniform="auto"):
     lane holds at most 6 words and the padding stores at most 1.25 times
     the words.
 
+    ``elem``: element of every stored entry (A in the order given, rows
+    sorted, duplicates kept), for the time-dependent form: see
+    code_dictionary; then the value-table ranges are returned last.
+
     Returns (slice_ptr int32, codes uint16, vtab complex128[2^vbits],
-    otab int32[2^(16 - vbits)], vbits, npad)."""
+    otab int32[2^(16 - vbits)], vbits, npad[, vstart])."""
     if lanes not in (1, 2, 4, 8):
         raise ValueError("lanes must be 1, 2, 4 or 8")
-    A = sp.csr_matrix(A)
-    A.sort_indices()
+    if elem is None:
+        A = sp.csr_matrix(A)
+        A.sort_indices()
     n = A.shape[0]
-    dic = code_dictionary(A, fp32)
+    dic = code_dictionary(A, fp32, elem)
     if dic is None:
         return None
-    vinv, oinv, vtab, otab, vbits, pad_code = dic
+    vinv, oinv, vtab, otab, vbits, pad_code = dic[:6]
     rn = np.diff(A.indptr).astype(np.int64)
     rows = np.repeat(np.arange(n, dtype=np.int64), rn)
     C = 64 // lanes
@@ -438,12 +457,41 @@ def sell_coded_arrays(A, lanes=1, fp32=False, uniform="auto"):
     if total_words >= 2 ** 31:
         raise ValueError("coded SELL storage exceeds int32 word offsets")
     codes = np.full(4 * max(total_words, 1), pad_code, dtype=np.uint16)
-    j = np.arange(A.nnz, dtype=np.int64) - np.repeat(A.indptr[:-1].astype(np.int64), rn)
+    nnz = len(A.data)
+    j = np.arange(nnz, dtype=np.int64) - np.repeat(A.indptr[:-1].astype(np.int64), rn)
     u, q = j // lanes, j % lanes
     lane = (rows % C) * lanes + q
     word = slice_ptr[rows // C] + (u // 4) * 64 + lane
     codes[4 * word + u % 4] = (vinv | (oinv << vbits)).astype(np.uint16)
-    return slice_ptr.astype(np.int32), codes, vtab, otab, vbits, npad
+    out = (slice_ptr.astype(np.int32), codes, vtab, otab, vbits, npad)
+    return out if elem is None else out + (dic[6],)
+
+
+def stacked_elements(mats):
+    """The stored entries of several n x n CSR matrices (the elements of a
+    time-dependent generator, element 0 the constant part) as one CSR whose
+    rows hold every element's entries, duplicates kept (element by element,
+    columns sorted inside each), and the element of every entry."""
+    mats = [sp.csr_matrix(M) for M in mats]
+    n = mats[0].shape[0]
+    rows, cols, vals, elem = [], [], [], []
+    for e, M in enumerate(mats):
+        M = M.copy()
+        M.sum_duplicates()
+        M.sort_indices()
+        rn = np.diff(M.indptr)
+        rows.append(np.repeat(np.arange(n, dtype=np.int64), rn))
+        cols.append(M.indices.astype(np.int64))
+        vals.append(M.data.astype(np.complex128))
+        elem.append(np.full(M.nnz, e, dtype=np.int64))
+    rows, cols, vals, elem = map(np.concatenate, (rows, cols, vals, elem))
+    order = np.lexsort((cols, elem, rows))
+    rows, cols, vals, elem = rows[order], cols[order], vals[order], elem[order]
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(indptr, rows + 1, 1)
+    indptr = np.cumsum(indptr)
+    A = sp.csr_matrix((vals, cols.astype(np.int32), indptr.astype(np.int32)), shape=(n, n))
+    return A, elem
 
 
 class SellMatrix:

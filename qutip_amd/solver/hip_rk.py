@@ -19,10 +19,12 @@ values, FP64 state, accumulation and error norm), ``operator_format``
 ('auto' | 'coded' | 'sell': the fused engine's operator as 2-byte codes into
 value and column-offset tables when it has at most 256 of each -- the same
 result bits as the plain SELL-C values and columns, a tenth of the bytes)
-and ``dense_precision`` ('auto' | 'fp32' | 'bf16x3' | 'fp64_banded': the
-matrix-form Lindblad RHS as MFMA products, or in FP64 from the operators'
-diagonals; 'auto' takes the banded form when H_nh and the collapse
-operators have few diagonals).
+and ``dense_precision`` ('auto' | 'fp32' | 'bf16x6' | 'bf16x3' |
+'fp64_banded': the matrix-form Lindblad RHS as MFMA products -- FP32, or
+FP32 inputs split into three bf16 pieces (six products, FP32 accuracy) or
+two (three products, about 16 bits: outside the 1e-6 error bar on long
+solves) -- or in FP64 from the operators' diagonals; 'auto' takes the
+banded form when H_nh and the collapse operators have few diagonals).
 """
 import ctypes as C
 
@@ -61,6 +63,8 @@ def _ip(a):
 
 
 DIA_MAX_TERMS = 96  # products of diagonals the banded kernel takes (kDiaMaxTerms)
+# MFMA forms of the matrix-form RHS -> qa_rk_create_dense precision codes
+DENSE_PRECISIONS = {"fp32": 0, "bf16x3": 1, "bf16x6": 2}
 
 
 def banded_operators(Hnh, cops):
@@ -98,8 +102,8 @@ class RKEngine:
                  dense=None, dense_precision="fp32", coded="auto", uniform="auto"):
         """``op``: the sparse generator A (dy/dt = A y), or None with
         ``dense=(H_nh, [c_ops])`` (numpy arrays) for the matrix-form
-        Lindblad right-hand side: on MFMA (``dense_precision`` 'fp32' or
-        'bf16x3'), or from the operators' diagonals in FP64 ('fp64_banded';
+        Lindblad right-hand side: on MFMA (``dense_precision`` 'fp32',
+        'bf16x6' or 'bf16x3'), or from the operators' diagonals in FP64 ('fp64_banded';
         'auto' takes it when the operators are banded, else 'fp32').  ``coded``: store A as coded
         SELL-C (2-byte codes into 256-entry value and column-offset tables,
         the same result bits) -- "auto" when A fits the tables, True to
@@ -147,7 +151,7 @@ class RKEngine:
             cf = np.concatenate([np.asfortranarray(c, dtype=np.complex128).ravel(order="F")
                                  for c in cops]) if cops else np.zeros(1, np.complex128)
             self._dense_host = (Hf, cf)
-            prec = {"fp32": 0, "bf16x3": 1}[dense_precision]
+            prec = DENSE_PRECISIONS[dense_precision]
             rc = L.qa_rk_create_dense(
                 C.byref(h), N, len(cops), _dp(Hf.ravel(order="F").view(np.float64)),
                 _dp(np.ascontiguousarray(cf).view(np.float64)), prec, *tab_args)
@@ -724,5 +728,6 @@ class IntegratorHipRK(Integrator):
             raise ValueError("operator_precision must be 'fp64' or 'fp32'")
         if self._options.get("operator_format") not in ("auto", "coded", "sell"):
             raise ValueError("operator_format must be 'auto', 'coded' or 'sell'")
-        if self._options.get("dense_precision") not in ("auto", "fp32", "bf16x3", "fp64_banded"):
-            raise ValueError("dense_precision must be 'auto', 'fp32', 'bf16x3' or 'fp64_banded'")
+        if self._options.get("dense_precision") not in ("auto", "fp64_banded", *DENSE_PRECISIONS):
+            raise ValueError("dense_precision must be 'auto', 'fp32', 'bf16x6', 'bf16x3' or "
+                             "'fp64_banded'")

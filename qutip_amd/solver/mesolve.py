@@ -15,7 +15,8 @@ __all__ = ["mesolve", "MESolver"]
 class MESolver(Solver):
     """With ``options={"matrix_form": True}`` the state stays an N x N matrix
     and the right-hand side is LindbladMatrixForm (mesolve.py:251-277): the
-    dense path: MFMA products (``dense_precision`` 'fp32' or 'bf16x3') or,
+    dense path: MFMA products (``dense_precision`` 'fp32', 'bf16x6' or
+    'bf16x3') or,
     for banded operators, FP64 products of their diagonals ('fp64_banded';
     'auto', the default, picks it when it applies)."""
     name = "mesolve"

@@ -2,11 +2,12 @@
 the GPU.
 
 * LindbladMatrixForm.matmul_data against the dense formula in numpy (FP32
-  products: relative error ~1e-6 of the operator norms; bf16 x3 the same
-  order, the split keeps ~16 mantissa bits per factor);
+  and bf16 x6 products: relative error ~1e-6 of the operator norms; bf16 x3
+  keeps ~16 mantissa bits per factor);
 * mesolve(matrix_form=True) against the superoperator path and the FP64
-  fixture of the driven Kerr oscillator (N=256); the error per precision is
-  printed and bounded;
+  fixture of the driven Kerr oscillator (N=256) over the whole window
+  t in [0, 30]: FP32, bf16 x6 and the banded form within 1e-6, bf16 x3
+  within its stated 2e-4;
 * the banded form (FP64 products of the operators' diagonals) against the
   numpy formula to rounding (1e-13 relative, ragged N) and the Kerr fixture
   to 1e-6."""
@@ -28,11 +29,11 @@ def _dense_rhs(H, cs, rho):
 
 
 @pytest.mark.parametrize("N", [16, 32, 64, 128, 384, 512])
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x6", "bf16x3"])
 def test_matrix_form_rhs_vs_numpy(N, prec):
     # N >= 384 runs the 32 x 32 LDS-tiled kernels, smaller N the one-tile kernels
-    if prec == "bf16x3" and N % 32:
-        pytest.skip("bf16x3 needs N % 32 == 0")
+    if prec.startswith("bf16") and N % 32:
+        pytest.skip("the bf16 forms need N % 32 == 0")
     print(f"dense rhs N={N} prec={prec}", flush=True)
     rng = np.random.default_rng(N)
     H = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
@@ -45,14 +46,42 @@ def test_matrix_form_rhs_vs_numpy(N, prec):
     want = _dense_rhs(H, cs, rho)
     rel = np.abs(got - want).max() / np.abs(want).max()
     print(f"  relative max error {rel:.2e}", flush=True)
-    # FP32: 24-bit products; bf16 x3 drops the lo*lo term: ~2^-16 per factor
-    assert rel < (2e-5 if prec == "fp32" else 1e-4)
+    # FP32 and bf16 x6: 24-bit inputs, FP32 sums; bf16 x3 keeps ~16 bits
+    # of each input (its dropped lo*lo term is 2^-16 relative)
+    assert rel < {"fp32": 2e-5, "bf16x6": 2e-5, "bf16x3": 1e-4}[prec]
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
-def test_kerr_matrix_form_vs_golden(golden, prec):
-    print(f"kerr matrix form prec={prec}", flush=True)
-    g = golden("kerr_mesolve.npz")
+def test_bf16x6_is_as_accurate_as_fp32():
+    """The three-piece split reproduces the FP32 inputs exactly, so its RHS
+    error against complex128 is that of FP32 (within a factor for the
+    different FP32 summation order), and 100x below bf16 x3's."""
+    N = 256
+    rng = np.random.default_rng(3)
+    H = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    H = (H + H.conj().T) / 2
+    cs = [(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) * 0.3]
+    r = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    rho = r @ r.conj().T / N
+    want = _dense_rhs(H, cs, rho)
+    rel = {}
+    for prec in ("fp32", "bf16x6", "bf16x3"):
+        L = qa.LindbladMatrixForm(qa.Qobj(H), [qa.Qobj(c) for c in cs], precision=prec)
+        got = L.matmul_data(0.0, qa.HipDense.from_array(rho)).to_array()
+        rel[prec] = np.abs(got - want).max() / np.abs(want).max()
+    print(f"relative RHS errors {rel}", flush=True)
+    assert rel["bf16x6"] < 3 * rel["fp32"]
+    assert rel["bf16x6"] * 30 < rel["bf16x3"]
+
+
+# the metric's bar: <O>(t) within 1e-6 of the FP64 solution over the whole
+# window (t in [0, 30], 150 points); bf16 x3 is outside it and stated
+KERR_BAR = {"fp32": 1e-6, "bf16x6": 1e-6, "bf16x3": 2e-4, "fp64_banded": 1e-6}
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x6", "bf16x3", "fp64_banded"])
+def test_kerr_matrix_form_full_window(golden, prec):
+    print(f"kerr matrix form prec={prec}, t in [0, 30]", flush=True)
+    g = golden("kerr_full_mesolve.npz")
     s = systems.kerr(256)
     res = qa.mesolve(s["H"], s["psi0"], g["tlist"], s["c_ops"], e_ops=s["e_ops"],
                      options={"matrix_form": True, "dense_precision": prec,
@@ -60,7 +89,7 @@ def test_kerr_matrix_form_vs_golden(golden, prec):
     errs = [np.abs(np.asarray(res.expect[k]) - g["expect"][k]).max() for k in range(2)]
     print(f"  <n> err {errs[0]:.2e} (max <n> {g['expect'][0].max():.2f}), parity err "
           f"{errs[1]:.2e}; stats {res.stats}", flush=True)
-    assert errs[0] < 1e-3 * max(1.0, g["expect"][0].max()) and errs[1] < 1e-3
+    assert max(errs) < KERR_BAR[prec]
 
 
 def test_matrix_form_matches_superoperator_small():
@@ -80,7 +109,7 @@ def test_matrix_form_matches_superoperator_small():
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x6", "bf16x3"])
 def test_matrix_form_decay_closed_form_large(prec):
     """The whole tiled stage path (N = 384: 32 x 32 LDS tiles) on a solve with
     a closed form: H = 0, c = sqrt(g) a, rho0 = |5><5| gives
@@ -95,7 +124,7 @@ def test_matrix_form_decay_closed_form_large(prec):
                               "atol": 1e-10})
     err = np.abs(np.asarray(res.expect[0]) - 5 * np.exp(-g * tl)).max()
     print(f"decay N={N} prec={prec}: max err {err:.2e}", flush=True)
-    assert err < (1e-5 if prec == "fp32" else 5e-4)
+    assert err < (5e-4 if prec == "bf16x3" else 1e-5)
 
 
 def _banded_system(N, rng):

@@ -285,3 +285,40 @@ def test_solver_functionals_kept_per_e_ops_list():
     s._state_is_ket, s._state_shape = True, (6, 1)  # a ket: quadratic forms
     f4 = s._functionals([B, A])
     assert same(f4, fresh(s, [B, A])) and (f4[0] == 1).all()
+
+
+@pytest.mark.parametrize("N,density", [(40, 0.1), (64, 0.3), (24, 0.7)])
+def test_rand_herm_fill_distribution(N, density):
+    """rand_herm follows the "fill" distribution of random_objects.py:198-327:
+    Hermitian, entries of (X + X^dag)/2 with X uniform in the [-1,1]^2 box,
+    and the reference's entry count (sparse: that many distinct positions;
+    dense: that many symmetric pairs removed)."""
+    H = qa.rand_herm(N, density=density, seed=3).full()
+    assert np.array_equal(H, H.conj().T)
+    assert np.abs(H.real).max() <= 1.0 and np.abs(H.imag).max() <= 1.0
+    nz = np.count_nonzero(H)
+    if density < 0.5:
+        num = int(max((N * N - 0.666 * N) * (1 - (1 - density) ** 0.5) + 0.666 * N * density, 1))
+        # each drawn position and its mirror; a diagonal or mirrored pair
+        # coincides with itself
+        assert num <= nz <= 2 * num
+    else:
+        num = int(max(N * (N - 0.666) * (1 - density ** 0.5) + 0.666 * N * (1 - density), 1))
+        assert N * N - 2 * num <= nz <= N * N - num
+    # the same seed gives the same matrix; another seed another one
+    assert np.array_equal(H, qa.rand_herm(N, density=density, seed=3).full())
+    assert not np.array_equal(H, qa.rand_herm(N, density=density, seed=4).full())
+
+
+def test_rand_dm_ginibre():
+    """rand_dm is a full-rank Ginibre density matrix (random_objects.py:
+    612-643): Hermitian, unit trace, positive definite; X has standard
+    complex normal entries, so E tr(X X^dag) = 2 N^2."""
+    N = 48
+    rho = qa.rand_dm(N, seed=9).full()
+    assert np.abs(rho - rho.conj().T).max() < 1e-15
+    assert abs(np.trace(rho) - 1) < 1e-12
+    assert np.linalg.eigvalsh(rho).min() > 0
+    g = np.random.default_rng(9).normal(size=(N, N, 2))
+    X = g[..., 0] + 1j * g[..., 1]
+    assert abs(np.sum(np.abs(X) ** 2) / (2 * N * N) - 1) < 0.05

@@ -238,3 +238,37 @@ def test_rk_oracle_matches_golden_sesolve(golden):
     vals = np.array([[models.expect_ket(O, y).real / np.vdot(y, y).real for y in ys]
                      for O in m["e_ops"]])
     assert np.abs(vals - g["expect"]).max() < 1e-7
+
+
+@pytest.mark.parametrize("d,nc", [(5, 1), (12, 3)])
+def test_matrix_form_rhs_vs_liouvillian(d, nc):
+    """The A + A^dag matrix form equals the Liouvillian on Hermitian rho
+    (a dropped 1/2 on the dissipator or a wrong sign of H fails it)."""
+    rng = np.random.default_rng(d)
+    H = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+    H = (H + H.conj().T) / 2
+    cs = [rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d)) for _ in range(nc)]
+    Lm = models.liouvillian(sp.csr_matrix(H), [sp.csr_matrix(c) for c in cs])
+    r = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+    rho = r @ r.conj().T
+    f = models.matrix_form_rhs(H, cs)
+    got = f(0.0, models.vec(rho))
+    assert np.abs(got - Lm @ models.vec(rho)).max() < 1e-12 * np.abs(got).max()
+    assert np.allclose(models.unvec(got, d), models.unvec(got, d).conj().T, atol=1e-12)
+
+
+def test_mesolve_expect_rk_vs_expm():
+    """The RK FP64 reference used for large dense systems agrees with the
+    matrix exponential of the Liouvillian on a small random system."""
+    d = 8
+    rng = np.random.default_rng(4)
+    H = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+    H = (H + H.conj().T) / 2
+    cs = [0.3 * (rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d)))]
+    psi0 = np.zeros(d, complex)
+    psi0[0] = 1
+    tl = np.linspace(0, 3, 13)
+    e = [sp.csr_matrix(np.diag(np.arange(d) * 1.0 + 0j)), sp.csr_matrix(H)]
+    a = exact.mesolve_expect_rk(H, cs, psi0, tl, e)
+    b = exact.mesolve_expect(sp.csr_matrix(H), [sp.csr_matrix(c) for c in cs], psi0, tl, e)
+    assert np.abs(a - b).max() < 1e-9
