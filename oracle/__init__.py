@@ -23,7 +23,12 @@ Contents and how each function is pinned (a function without a pin says so):
                  qutip/tests/solver/test_mesolve.py:84-95.
                  ``matrix_form_rhs`` (LindbladMatrixForm, the A + A^dag
                  form) is pinned against that Liouvillian on Hermitian
-                 states of random systems.
+                 states of random systems.  ``td_rhs`` / ``str_coefficient``
+                 (the list-form QobjEvo with string coefficients) are
+                 pinned, through ``rk``, by closed forms: a qubit driven by
+                 f(t) sigma_x (P = sin^2 F(t), F the integral of f), with
+                 sigma_x dephasing (<sigma_z> = cos 2F e^{-2 gamma t}), and
+                 by scipy's solve_ivp on a small driven chain.
 * ``exact``      vec(rho(t)) = expm(L t) vec(rho0) through
                  scipy.sparse.linalg.expm_multiply -- a library routine, the
                  FP64 reference the parity tests compare against.

@@ -119,6 +119,45 @@ def matrix_form_rhs(H, c_ops):
     return f
 
 
+# The names a string coefficient may use (qutip/core/coefficient.py:367-395).
+STR_ENV = {
+    "sin": np.sin, "cos": np.cos, "tan": np.tan, "asin": np.arcsin, "acos": np.arccos,
+    "atan": np.arctan, "pi": np.pi, "sinh": np.sinh, "cosh": np.cosh, "tanh": np.tanh,
+    "asinh": np.arcsinh, "acosh": np.arccosh, "atanh": np.arctanh, "exp": np.exp,
+    "log": np.log, "log10": np.log10, "sqrt": np.sqrt, "real": np.real, "imag": np.imag,
+    "conj": np.conj, "abs": np.abs, "norm": lambda x: np.abs(x) ** 2, "arg": np.angle,
+    "np": np,
+}
+
+
+def str_coefficient(base, args=None):
+    """A string coefficient as StrFunctionCoefficient evaluates it
+    (qutip/core/cy/coefficient.pyx:292-370): the expression in ``t`` and the
+    arguments, run by Python in the environment above; returns f(t) ->
+    complex."""
+    args = dict(args or {})
+    env = dict(STR_ENV)
+
+    def f(t):
+        return complex(eval(base, env, {"t": t, **args}))
+    return f
+
+
+def td_rhs(A0, parts):
+    """dy/dt = (A0 + sum_e c_e(t) A_e) y: the list-form QobjEvo's
+    matmul_data (qutip/core/cy/qobjevo.pyx:1103-1116 over _EvoElement,
+    _element.pyx:117-182, 388-446), every coefficient a function of t."""
+    A0 = sp.csr_matrix(A0, dtype=CD)
+    parts = [(sp.csr_matrix(A, dtype=CD), c) for A, c in parts]
+
+    def f(t, y):
+        out = A0 @ y
+        for A, c in parts:
+            out = out + c(t) * (A @ y)
+        return out
+    return f
+
+
 def vec(rho):
     """Column stacking, superoperator.py:291-302."""
     return np.asarray(rho).ravel(order="F")

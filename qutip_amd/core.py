@@ -270,8 +270,21 @@ def lindblad_dissipator(a, b=None):
 
 
 def liouvillian(H=None, c_ops=()):
-    """-i (spre(H) - spost(H)) + sum_c D[c] (superoperator.py:51-142)."""
+    """-i (spre(H) - spost(H)) + sum_c D[c] (superoperator.py:51-142).  A
+    time-dependent H (QobjEvo in list form) gives a QobjEvo: the constant
+    Liouvillian plus -i (spre(H_e) - spost(H_e)) with each element's
+    coefficient, as the reference's linear maps of QobjEvo do
+    (superoperator.py:120-134)."""
     c_ops = list(c_ops or [])
+    if any(isinstance(c, QobjEvo) and c.elements for c in c_ops):
+        raise NotImplementedError("time-dependent collapse operators are not supported")
+    c_ops = [c.qobj if isinstance(c, QobjEvo) else c for c in c_ops]
+    if isinstance(H, QobjEvo):
+        if H.elements:
+            out = QobjEvo(liouvillian(H.qobj, c_ops))
+            out.elements = [(liouvillian(q), c) for q, c in H.elements]
+            return out
+        H = H.qobj
     if H is not None and H.issuper:
         L = H.data.copy()
         ref = H
@@ -358,26 +371,66 @@ def rand_dm(N, seed=None):
 
 
 # ---------------------------------------------------------------------------
-# QobjEvo (constant systems)
+# QobjEvo: constant operators and QuTiP's list form
 # ---------------------------------------------------------------------------
 class QobjEvo:
-    """Constant operator with the data-layer entry points the solvers call
-    (qutip/core/cy/qobjevo.pyx:1018-1131).  ``matmul_data`` and
-    ``expect_data`` take device data (HipDense) and run on the GPU; the
-    device copy of the operator is made on first use."""
+    """An operator in QuTiP's list form (qutip/core/cy/qobjevo.pyx:31-552):
+    ``QobjEvo(Q)`` for a constant Qobj, or ``QobjEvo([H0, [H1, c1], ...],
+    args=..., tlist=...)`` for A(t) = H0 + sum_e c_e(t) H_e, each c_e a
+    string, array (with ``tlist``), function or Coefficient
+    (coefficient.py:63-229).  ``qobj`` is the constant part (the sum of the
+    constant elements) and ``elements`` the time-dependent ones as
+    (Qobj, Coefficient) pairs (_EvoElement, _element.pyx:388-446).
 
-    def __init__(self, Q):
+    ``matmul_data(t, state)`` and ``expect_data`` take device data
+    (HipDense) and run on the GPU: sum_e c_e(t) H_e @ state, the
+    coefficients evaluated on the host at t (the plain form; the fused
+    engine evaluates them on the device from their programs)."""
+
+    def __init__(self, Q, args=None, tlist=None, order=3, boundary_conditions=None):
+        from . import coefficient as _coef
+        self.elements = []
         if isinstance(Q, QobjEvo):
-            Q = Q.qobj
-        if not isinstance(Q, Qobj):
-            raise TypeError("QobjEvo needs a Qobj")
-        self.qobj = Q
-        self.dims = Q.dims
-        self.shape = Q.shape
+            self.qobj = Q.qobj
+            self.elements = [(q, c.replace_arguments(args) if args else c) for q, c in Q.elements]
+        elif isinstance(Q, Qobj):
+            self.qobj = Q
+        elif isinstance(Q, (list, tuple)):
+            const = None
+            for item in Q:
+                if isinstance(item, Qobj):
+                    const = item if const is None else const + item
+                elif isinstance(item, QobjEvo):
+                    const = item.qobj if const is None else const + item.qobj
+                    self.elements += item.elements
+                elif isinstance(item, (list, tuple)) and len(item) == 2:
+                    q, c = item
+                    if not isinstance(q, Qobj):
+                        raise TypeError("a list element must be [Qobj, coefficient]")
+                    self.elements.append((q, _coef.coefficient(
+                        c, tlist=tlist, args=args, order=order,
+                        boundary_conditions=boundary_conditions)))
+                else:
+                    raise TypeError("QobjEvo list items are Qobj or [Qobj, coefficient]")
+            if const is None:
+                ref = self.elements[0][0]
+                const = Qobj(sp.csr_matrix(ref.shape, dtype=np.complex128), dims=ref.dims,
+                             superrep=ref.superrep)
+            self.qobj = const
+        else:
+            raise TypeError("QobjEvo needs a Qobj or a list")
+        self.dims = self.qobj.dims
+        self.shape = self.qobj.shape
+        for q, _ in self.elements:
+            if q.shape != self.shape:
+                raise ValueError("the elements of a QobjEvo must have one shape")
         self._dev = None
         self._dev_fp32 = None
+        self._dev_el = None
 
-    isconstant = True
+    @property
+    def isconstant(self):
+        return not self.elements
 
     @property
     def issuper(self):
@@ -388,7 +441,37 @@ class QobjEvo:
         return self.qobj.isoper
 
     def __call__(self, t=0.0, args=None):
-        return self.qobj
+        """The operator at time t (qobjevo.pyx:437-480)."""
+        out = self.qobj
+        for q, c in self.elements:
+            cc = c.replace_arguments(args) if args else c
+            out = out + complex(cc(t)) * q
+        return out
+
+    def to_list(self):
+        """The list form (qobjevo.pyx:869-890)."""
+        return [self.qobj] + [[q, c] for q, c in self.elements]
+
+    def replace_arguments(self, args=None, **kwargs):
+        args = {**(args or {}), **kwargs}
+        out = QobjEvo(self.qobj)
+        out.elements = [(q, c.replace_arguments(args)) for q, c in self.elements]
+        return out
+
+    def arguments(self, args=None, **kwargs):
+        """Replace the coefficients' arguments in place (qobjevo.pyx:
+        560-590)."""
+        args = {**(args or {}), **kwargs}
+        if args:
+            self.elements = [(q, c.replace_arguments(args)) for q, c in self.elements]
+
+    def linear_map(self, f):
+        """f applied to every element's operator, coefficients kept
+        (qobjevo.pyx:818-867; f linear)."""
+        out = QobjEvo(f(self.qobj))
+        out.elements = [(f(q), c) for q, c in self.elements]
+        out.dims, out.shape = out.qobj.dims, out.qobj.shape
+        return out
 
     def device_data(self, fp32=False):
         if fp32:
@@ -399,32 +482,72 @@ class QobjEvo:
             self._dev = _data.HipCSR(self.qobj.to_csr())
         return self._dev
 
+    def device_elements(self):
+        if self._dev_el is None:
+            self._dev_el = [_data.HipCSR(q.to_csr()) for q, _ in self.elements]
+        return self._dev_el
+
     def matmul_data(self, t, state, out=None):
-        """out += self(t) @ state  (qobjevo.pyx:1103-1116)."""
-        return _data.matmul(self.device_data(), state, 1.0, out=out)
+        """out += self(t) @ state  (qobjevo.pyx:1103-1116: the constant part,
+        then every element scaled by its coefficient at t)."""
+        out = _data.matmul(self.device_data(), state, 1.0, out=out)
+        for (q, c), d in zip(self.elements, self.device_elements()):
+            out = _data.matmul(d, state, complex(c(t)), out=out)
+        return out
 
     def expect_data(self, t, state):
-        return _data.expect(self.device_data(), state)
+        if not self.elements:
+            return _data.expect(self.device_data(), state)
+        return _data.expect(_data.HipCSR(self(t).to_csr()), state)
+
+    def _binary(self, other, sign):
+        out = QobjEvo(self)
+        if isinstance(other, QobjEvo):
+            out.qobj = self.qobj + sign * other.qobj
+            out.elements = list(self.elements) + [(sign * q, c) for q, c in other.elements]
+        else:
+            out.qobj = self.qobj + sign * other
+        out._dev = out._dev_fp32 = out._dev_el = None
+        return out
 
     def __add__(self, other):
-        other = other.qobj if isinstance(other, QobjEvo) else other
-        return QobjEvo(self.qobj + other)
+        return self._binary(other, 1)
+
+    __radd__ = __add__
 
     def __sub__(self, other):
-        other = other.qobj if isinstance(other, QobjEvo) else other
-        return QobjEvo(self.qobj - other)
+        return self._binary(other, -1)
 
     def __mul__(self, other):
-        return QobjEvo(self.qobj * other)
+        if not isinstance(other, numbers.Number):
+            return NotImplemented
+        out = QobjEvo(self.qobj * other)
+        out.elements = [(q * other, c) for q, c in self.elements]
+        return out
 
     __rmul__ = __mul__
 
+    def __neg__(self):
+        return self * -1
+
     def dag(self):
-        return QobjEvo(self.qobj.dag())
+        out = QobjEvo(self.qobj.dag())
+        out.elements = [(q.dag(), c.conj()) for q, c in self.elements]
+        return out
 
     def __matmul__(self, other):
-        other = other.qobj if isinstance(other, QobjEvo) else other
-        return QobjEvo(self.qobj @ other)
+        if isinstance(other, QobjEvo):
+            if other.elements:
+                raise NotImplementedError("products of two time-dependent operators")
+            other = other.qobj
+        out = QobjEvo(self.qobj @ other)
+        out.elements = [(q @ other, c) for q, c in self.elements]
+        return out
+
+    def __rmatmul__(self, other):
+        out = QobjEvo(other @ self.qobj)
+        out.elements = [(other @ q, c) for q, c in self.elements]
+        return out
 
 
 class LindbladMatrixForm:

@@ -372,7 +372,7 @@ def code_dictionary(A, fp32=False, elem=None):
         vals = vals.astype(np.complex64).astype(np.complex128)
     bits = np.concatenate([np.zeros((1, 2), np.uint64), vals.view(np.uint64).reshape(-1, 2)])
     if elem is not None:
-        el = np.concatenate([[0], np.asarray(elem, dtype=np.uint64)])
+        el = np.concatenate([np.zeros(1, np.uint64), np.asarray(elem, dtype=np.uint64)])
         bits = np.concatenate([el[:, None], bits], axis=1)
     ubits, vinv = np.unique(bits, axis=0, return_inverse=True)
     rn = np.diff(A.indptr).astype(np.int64)
@@ -497,18 +497,25 @@ def stacked_elements(mats):
 class SellMatrix:
     """A SELL-C copy of a CSR matrix in HBM (see sell_arrays), or with
     ``coded=True`` the coded form (sell_coded_arrays; ValueError when the
-    matrix does not fit the 256-entry tables)."""
+    matrix does not fit the 256-entry tables).  ``elem``: the element of
+    every stored entry of a stacked time-dependent generator
+    (stacked_elements); the coded value table is then split by element and
+    ``vstart`` holds the ranges."""
 
-    def __init__(self, A, fp32=False, lanes=1, coded=False, uniform="auto"):
+    def __init__(self, A, fp32=False, lanes=1, coded=False, uniform="auto", elem=None):
         self.shape = A.shape
         self.lanes = lanes
         self.fp32 = fp32
-        self.coded = bool(coded)
+        self.coded = bool(coded) or elem is not None
+        self.vstart = None
         if self.coded:
-            arrs = sell_coded_arrays(A, lanes, fp32, uniform)
+            arrs = sell_coded_arrays(A, lanes, fp32, uniform, elem=elem)
             if arrs is None:
                 raise ValueError("the operator's distinct values and column offsets fit no "
                                  "16-bit code split: no coded SELL-C form")
+            if elem is not None:
+                self.vstart = arrs[-1]
+                arrs = arrs[:-1]
             sptr, codes, vtab, otab, self.vbits, npad = arrs
             self.nslices = len(sptr) - 1
             self.npad = npad

@@ -33,6 +33,7 @@ import scipy.sparse as sp
 
 from .. import _lib, autotune
 from .. import data as _data
+from ..coefficient import DeviceProgram
 from . import small as _small
 from .integrator import Integrator, IntegratorException
 from .tableaus import TABLEAUS
@@ -63,6 +64,7 @@ def _ip(a):
 
 
 DIA_MAX_TERMS = 96  # products of diagonals the banded kernel takes (kDiaMaxTerms)
+MAX_COEF = 8  # time-dependent elements of one generator on the device (kMaxCoef)
 # MFMA forms of the matrix-form RHS -> qa_rk_create_dense precision codes
 DENSE_PRECISIONS = {"fp32": 0, "bf16x3": 1, "bf16x6": 2}
 
@@ -99,7 +101,7 @@ class RKEngine:
 
     def __init__(self, op, tableau="vern7", rtol=1e-6, atol=1e-8, nsteps=1000, first_step=0.0,
                  min_step=0.0, max_step=0.0, interpolate=True, fp32=False, lanes=None,
-                 dense=None, dense_precision="fp32", coded="auto", uniform="auto"):
+                 dense=None, dense_precision="fp32", coded="auto", uniform="auto", td=None):
         """``op``: the sparse generator A (dy/dt = A y), or None with
         ``dense=(H_nh, [c_ops])`` (numpy arrays) for the matrix-form
         Lindblad right-hand side: on MFMA (``dense_precision`` 'fp32',
@@ -108,7 +110,12 @@ class RKEngine:
         SELL-C (2-byte codes into 256-entry value and column-offset tables,
         the same result bits) -- "auto" when A fits the tables, True to
         require it, False for the plain form; ``uniform``: the coded slices'
-        layout (see data.sell_coded_arrays)."""
+        layout (see data.sell_coded_arrays).  ``td``: a time-dependent
+        generator A(t) = A_0 + sum_e c_e(t) A_e as (A_0, [(A_e,
+        DeviceProgram of c_e), ...]) with ``op`` None: the elements go into
+        one coded operator whose value table is split by element, and the
+        stage kernels evaluate the programs at every stage time
+        (ValueError when the stacked operator has no coded form)."""
         tab = TABLEAUS[tableau] if isinstance(tableau, str) else tableau
         self.tableau = tab
         self._a = np.ascontiguousarray(tab["a"], dtype=np.float64)
@@ -156,6 +163,30 @@ class RKEngine:
                 C.byref(h), N, len(cops), _dp(Hf.ravel(order="F").view(np.float64)),
                 _dp(np.ascontiguousarray(cf).view(np.float64)), prec, *tab_args)
             _lib.check(rc, "qa_rk_create_dense")
+        elif td is not None:
+            A0, parts = td
+            if not 1 <= len(parts) <= MAX_COEF:
+                raise ValueError(f"1 to {MAX_COEF} time-dependent elements")
+            A, elem = _data.stacked_elements([A0] + [M for M, _ in parts])
+            self.n = A.shape[0]
+            if lanes is None:
+                lanes = autotune.lanes_for(A, False, True)
+            self.lanes = lanes
+            self.coded = True
+            self.sell = _data.SellMatrix(A, lanes=lanes, uniform=uniform, elem=elem)
+            sptr, cols, vals = self.sell.ptrs
+            rc = L.qa_rk_create_coded(
+                C.byref(h), self.n, lanes, self.sell.nslices, C.c_void_p(sptr),
+                C.c_void_p(cols), C.c_void_p(vals), C.c_void_p(self.sell.otab_ptr),
+                self.sell.vbits, *tab_args)
+            _lib.check(rc, "qa_rk_create_coded")
+            self.handle = h
+            progs = DeviceProgram.concat([p for _, p in parts])
+            start, code, consts = progs.arrays()
+            vstart = np.ascontiguousarray(self.sell.vstart[1:], dtype=np.int32)
+            self._td_host = (start, vstart, code, consts)
+            _lib.call("qa_rk_set_coefficients", h, len(parts), _ip(start), _ip(vstart),
+                      len(code), _ip(code), len(consts), _dp(consts.view(np.float64)))
         else:
             if isinstance(op, _data.HipCSR):
                 A = op.to_scipy()
@@ -252,10 +283,10 @@ class RKEngine:
     def iterate(self, count):
         _lib.call("qa_rk_iterate", self.handle, int(count))
 
-    def apply(self, x, out=None):
-        """out = A x (device vectors of n entries)."""
+    def apply(self, x, out=None, t=0.0):
+        """out = A(t) x (device vectors of n entries)."""
         out = _data.HipDense(x.shape) if out is None else out
-        _lib.call("qa_rk_apply", self.handle, C.c_void_p(x.ptr), C.c_void_p(out.ptr))
+        _lib.call("qa_rk_apply_at", self.handle, float(t), C.c_void_p(x.ptr), C.c_void_p(out.ptr))
         return out
 
 
@@ -507,6 +538,26 @@ def constant_operator(derivative):
     return sp.csr_matrix(np.asarray(d))
 
 
+def time_dependent_operator(derivative):
+    """(A_0, [(A_e, DeviceProgram), ...]) when ``derivative`` is the bound
+    ``matmul_data`` of a time-dependent QobjEvo of this package whose every
+    coefficient has a device program (at most MAX_COEF elements); None
+    otherwise (constant generators, coefficients that only run on the
+    host, QuTiP's own QobjEvo whose elements are not visible from Python:
+    the plain form evaluates those on the host)."""
+    owner = getattr(derivative, "__self__", None)
+    elements = getattr(owner, "elements", None)
+    if not elements or not hasattr(owner, "qobj") or len(elements) > MAX_COEF:
+        return None
+    parts = []
+    for q, c in elements:
+        prog = c.device_program() if hasattr(c, "device_program") else None
+        if prog is None:
+            return None
+        parts.append((sp.csr_matrix(q.to_csr()), prog))
+    return sp.csr_matrix(owner.qobj.to_csr()), parts
+
+
 class IntegratorHipRK(Integrator):
     """Explicit RK (vern7 / tsit5) on the GPU.  Usable with
     ``method="hip_rk"``."""
@@ -539,7 +590,21 @@ class IntegratorHipRK(Integrator):
         owner = getattr(self.derivative, "__self__", None)
         dense = matrix_form_operators(owner) if o["fused"] else None
         op = constant_operator(self.derivative) if (o["fused"] and dense is None) else None
-        if dense is not None:
+        td = (time_dependent_operator(self.derivative)
+              if (o["fused"] and dense is None and op is None
+                  and o["operator_precision"] == "fp64" and o["operator_format"] != "sell")
+              else None)
+        if td is not None:
+            try:
+                self._engine = RKEngine(None, o["tableau"], o["rtol"], o["atol"], o["nsteps"],
+                                        o["first_step"], o["min_step"], o["max_step"],
+                                        o["interpolate"], td=td)
+                self.name = "hip_rk (fused, time-dependent coded operator)"
+            except ValueError:
+                self._engine = None  # no coded form: the plain form below
+        if self._engine is not None:
+            pass
+        elif dense is not None:
             self._engine = RKEngine(None, o["tableau"], o["rtol"], o["atol"], o["nsteps"],
                                     o["first_step"], o["min_step"], o["max_step"],
                                     o["interpolate"], dense=dense,

@@ -25,8 +25,12 @@ class MESolver(Solver):
     solver_options = {**Solver.solver_options, "matrix_form": False}
 
     def __init__(self, H, c_ops=None, *, options=None):
-        if isinstance(H, QobjEvo):
+        if isinstance(H, (list, tuple)):
+            H = QobjEvo(H)
+        if isinstance(H, QobjEvo) and not H.elements:
             H = H.qobj
+        if any(isinstance(c, QobjEvo) and c.elements for c in (c_ops or [])):
+            raise NotImplementedError("time-dependent collapse operators are not supported")
         c_ops = [c.qobj if isinstance(c, QobjEvo) else c for c in (c_ops or [])]
         for c in c_ops:
             if not isinstance(c, Qobj):
@@ -34,6 +38,8 @@ class MESolver(Solver):
         self._num_collapse = len(c_ops)
         self._matrix_form = bool((options or {}).get("matrix_form", False))
         if self._matrix_form:
+            if isinstance(H, QobjEvo):
+                raise NotImplementedError("matrix_form with a time-dependent H")
             if H.issuper or any(c.issuper for c in c_ops):
                 raise ValueError("matrix_form=True cannot be used with superoperator H")
             self._vectorize_state = False
@@ -43,7 +49,7 @@ class MESolver(Solver):
             for c in c_ops:
                 if c.issuper:
                     rhs = rhs + c
-            rhs = QobjEvo(rhs)
+            rhs = rhs if isinstance(rhs, QobjEvo) else QobjEvo(rhs)
         super().__init__(rhs, options=options)
 
 
@@ -54,6 +60,8 @@ def mesolve(H, rho0, tlist, c_ops=None, e_ops=None, args=None, options=None, **k
     """Master-equation evolution; a ket without collapse operators goes to
     sesolve like the reference (mesolve.py:155-162)."""
     c_ops = list(c_ops or [])
+    if isinstance(H, (list, tuple)):
+        H = QobjEvo(H, args=args, tlist=tlist)
     if not c_ops and isinstance(rho0, Qobj) and rho0.isket and not H.issuper:
         return sesolve(H, rho0, tlist, e_ops=e_ops, args=args, options=options)
     solver = MESolver(H, c_ops, options=options)

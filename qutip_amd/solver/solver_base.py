@@ -238,6 +238,7 @@ class Solver:
     # -- run ------------------------------------------------------------------
     def run(self, state0, tlist, *, e_ops=None, args=None):
         t_start = time.time()
+        self._argument(args)
         tlist = np.asarray(tlist, dtype=np.float64)
         ops = self._eop_list(e_ops)
         self._check_eops(ops, state0.dims[0])
@@ -315,6 +316,13 @@ class Solver:
     # -- step interface (solver_base.py:228-280) -------------------------------
     def start(self, state0, t0):
         self._integrator.set_state(t0, self._prepare_state(state0))
+
+    def _argument(self, args):
+        """Replace the coefficients' arguments of the rhs and rebuild the
+        integrator's device programs (solver_base.py:456-460)."""
+        if args and getattr(self.rhs, "elements", None):
+            self.rhs.arguments(args)
+            self._integrator = self._get_integrator()
 
     def step(self, t, *, args=None, copy=True):
         _, state = self._integrator.integrate(t, copy=False)

@@ -154,3 +154,39 @@ def test_functionals_check():
         _lib.host_call("qa_functionals_check", 4, 1, i32(1), i32(0, 1), i32(-1), i32(0))
     with pytest.raises(ValueError, match="fptr"):
         _lib.host_call("qa_functionals_check", 4, 1, i32(0), i32(1, 1), i32(0), i32(0))
+
+
+def _coef_check(progs, ntimes=0):
+    """qa_coef_eval with no times: only the host-side program checks run."""
+    import numpy as np
+    from qutip_amd import coefficient as qc
+    start, code, consts = qc.DeviceProgram.concat(progs).arrays()
+    L = _lib.load()
+    return L.qa_coef_eval(len(progs), start.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                          len(code), code.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                          len(consts),
+                          consts.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                          ntimes, None, None, None)
+
+
+def test_coefficient_programs_pass_the_abi_checks():
+    """Every compiled test expression and interpolation table is accepted
+    by the library's program checks; malformed programs are rejected
+    before anything reaches the device."""
+    import numpy as np
+    from qutip_amd import coefficient as qc
+    from tests.test_coefficient import EXPRESSIONS
+    progs = [qc.coefficient(e, args=a).device_program() for e, a in EXPRESSIONS]
+    tl = np.linspace(0, 1, 6)
+    progs += [qc.coefficient(tl ** 2, tlist=tl, order=k).device_program() for k in (0, 1, 3)]
+    for g0 in range(0, len(progs), 8):
+        assert _coef_check(progs[g0:g0 + 8]) == 0, _lib.load().qa_last_error()
+    assert _coef_check(progs[:9]) == _lib.QA_ERR_ARG  # more than 8 programs
+    bad = [qc.DeviceProgram([qc.CO_ADD], [])]  # pops an empty stack
+    assert _coef_check(bad) == _lib.QA_ERR_ARG
+    bad = [qc.DeviceProgram([qc.CO_CONST | (3 << 8)], [1.0])]  # constant out of range
+    assert _coef_check(bad) == _lib.QA_ERR_ARG
+    bad = [qc.DeviceProgram([qc.CO_T, qc.CO_T], [])]  # leaves two values
+    assert _coef_check(bad) == _lib.QA_ERR_ARG
+    bad = [qc.DeviceProgram([qc.CO_INTERP], [complex(50, 2), 0j])]  # table past the end
+    assert _coef_check(bad) == _lib.QA_ERR_ARG

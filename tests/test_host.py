@@ -322,3 +322,65 @@ def test_rand_dm_ginibre():
     g = np.random.default_rng(9).normal(size=(N, N, 2))
     X = g[..., 0] + 1j * g[..., 1]
     assert abs(np.sum(np.abs(X) ** 2) / (2 * N * N) - 1) < 0.05
+
+
+def test_qobjevo_list_form():
+    """QobjEvo([H0, [H1, c]]) evaluates to H0 + c(t) H1 (qobjevo.pyx:437-480);
+    algebra keeps the coefficients; dag conjugates them."""
+    sx, sz = qa.sigmax(), qa.sigmaz()
+    H = qa.QobjEvo([sz, [sx, "cos(w * t)"], [sz, np.array([0.0, 1.0, 2.0])]],
+                   args={"w": 2.0}, tlist=np.array([0.0, 1.0, 2.0]))
+    assert not H.isconstant and len(H.elements) == 2
+    for t in (0.0, 0.4, 1.7):
+        want = sz.full() * (1 + t) + np.cos(2 * t) * sx.full()
+        assert np.allclose(H(t).full(), want)
+    G = (2.0 * H - sz).dag()
+    assert np.allclose(G(0.3).full(), (2 * H(0.3) - sz).full().conj().T)
+    H.arguments(w=3.0)
+    assert np.allclose(H(0.5).full(), sz.full() * 1.5 + np.cos(1.5) * sx.full())
+    assert isinstance(H.to_list()[1], list)
+
+
+def test_liouvillian_of_time_dependent_h():
+    """liouvillian of a list-form H: the constant Liouvillian plus -i(spre -
+    spost) of each element with its coefficient; at any t it equals the
+    oracle's Liouvillian of H(t)."""
+    a = qa.destroy(4)
+    H = qa.QobjEvo([a.dag() @ a, [a + a.dag(), "sin(t)"]])
+    L = qa.liouvillian(H, [np.sqrt(0.2) * a])
+    for t in (0.0, 0.9):
+        want = models.liouvillian(H(t).to_csr(), [np.sqrt(0.2) * a.to_csr()])
+        assert abs(L(t).to_csr() - want).max() < 1e-14
+
+
+def test_time_dependent_routing_on_the_host():
+    """Coefficients with device programs give the fused time-dependent
+    engine its operator; a Python function coefficient leaves it to the
+    plain form."""
+    from qutip_amd.solver.hip_rk import time_dependent_operator
+    sx, sz = qa.sigmax(), qa.sigmaz()
+    rhs = -1j * qa.QobjEvo([sz, [sx, "cos(t)"]])
+    A0, parts = time_dependent_operator(rhs.matmul_data)
+    assert abs(A0 - (-1j * sz.to_csr())).max() == 0 and len(parts) == 1
+    assert parts[0][1].evaluate(0.3) == pytest.approx(np.cos(0.3))
+    rhs_f = -1j * qa.QobjEvo([sz, [sx, lambda t: np.cos(t)]])
+    assert time_dependent_operator(rhs_f.matmul_data) is None
+    assert time_dependent_operator(qa.QobjEvo(sz).matmul_data) is None
+
+
+def test_stacked_elements_code_ranges():
+    """The stacked operator keeps every element's entries apart and its
+    coded value table is split by element (ranges in order)."""
+    rng = np.random.default_rng(1)
+    mats = [sp.random(40, 40, density=0.1, random_state=rng, format="csr") * (k + 1)
+            for k in range(3)]
+    A, elem = qdata.stacked_elements(mats)
+    assert A.nnz == sum(m.nnz for m in mats)
+    assert abs(A - sum(mats)).max() < 1e-15
+    arrs = qdata.sell_coded_arrays(A, 1, elem=elem)
+    vstart = arrs[-1]
+    assert list(vstart) == sorted(vstart) and vstart[0] == 0 and len(vstart) == 4
+    vtab = arrs[2]
+    for k, m in enumerate(mats):
+        vals = set(np.round(vtab[vstart[k]:vstart[k + 1]], 12)) - {0}
+        assert vals == set(np.round(m.data, 12)) - {0}

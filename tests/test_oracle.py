@@ -272,3 +272,55 @@ def test_mesolve_expect_rk_vs_expm():
     a = exact.mesolve_expect_rk(H, cs, psi0, tl, e)
     b = exact.mesolve_expect(sp.csr_matrix(H), [sp.csr_matrix(c) for c in cs], psi0, tl, e)
     assert np.abs(a - b).max() < 1e-9
+
+
+def _gauss_F(t, A, t0, s):
+    """Integral over [0, t] of A exp(-(u - t0)^2 / (2 s^2))."""
+    from scipy.special import erf
+    c = s * np.sqrt(2)
+    return A * s * np.sqrt(np.pi / 2) * (erf((t - t0) / c) + erf(t0 / c))
+
+
+def test_td_rhs_driven_qubit_closed_form():
+    """H(t) = f(t) sigma_x commutes with itself at all times: from |up>,
+    <sigma_z>(t) = cos 2F(t); with sigma_x dephasing at rate gamma the y-z
+    components also decay: cos 2F(t) e^{-2 gamma t}.  A stage time off by
+    c_i dt, or a coefficient applied to the wrong element, fails this."""
+    sx = models.sigmax()
+    sz = models.sigmaz()
+    args = {"A": 2.0, "t0": 1.5, "s": 0.4}
+    f = models.str_coefficient("A * exp(-(t - t0) ** 2 / (2 * s ** 2))", args)
+    tl = np.linspace(0, 3, 31)
+    F = _gauss_F(tl, **args)
+    psi0 = np.array([1, 0], complex)
+    rhs = models.td_rhs(sp.csr_matrix((2, 2)), [(-1j * sx, f)])
+    ys, _ = rk.solve(rhs, psi0, tl, tableaus.TABLEAUS["vern7"], rtol=1e-10, atol=1e-12)
+    ez = np.array([np.vdot(y, sz @ y).real for y in ys])
+    assert np.abs(ez - np.cos(2 * F)).max() < 1e-8
+    g = 0.3
+    L0 = models.liouvillian(sp.csr_matrix((2, 2), dtype=complex), [np.sqrt(g) * sx])
+    L1 = models.liouvillian(sx, [])
+    rhs = models.td_rhs(L0, [(L1, f)])
+    ys, _ = rk.solve(rhs, models.vec(models.ket2dm(psi0)), tl, tableaus.TABLEAUS["vern7"],
+                     rtol=1e-10, atol=1e-12)
+    ez = np.array([models.expect_dm(sz, models.unvec(y, 2)).real for y in ys])
+    assert np.abs(ez - np.cos(2 * F) * np.exp(-2 * g * tl)).max() < 1e-8
+
+
+def test_td_rhs_vs_solve_ivp():
+    """A three-site chain with a driven field against scipy's DOP853 at
+    tight tolerances (an independent integrator)."""
+    from scipy.integrate import solve_ivp
+    L = 3
+    zz = sum(models.site_op(models.sigmaz(), i, L) @ models.site_op(models.sigmaz(), i + 1, L)
+             for i in range(L - 1))
+    xs = sum(models.site_op(models.sigmax(), i, L) for i in range(L))
+    f = models.str_coefficient("0.8 * cos(w * t) + 0.2j * sin(t)", {"w": 2.0})
+    rhs = models.td_rhs(-1j * zz, [(-1j * xs, f)])
+    psi0 = np.zeros(2 ** L, complex)
+    psi0[0] = 1
+    tl = np.linspace(0, 4, 9)
+    ys, _ = rk.solve(rhs, psi0, tl, tableaus.TABLEAUS["vern7"], rtol=1e-11, atol=1e-13)
+    ref = solve_ivp(lambda t, y: rhs(t, y), (0, 4), psi0, method="DOP853", t_eval=tl,
+                    rtol=1e-12, atol=1e-14)
+    assert np.abs(np.array(ys).T - ref.y).max() < 1e-8
