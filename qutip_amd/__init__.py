@@ -12,6 +12,8 @@ from .core import (Qobj, QobjEvo, LindbladMatrixForm, basis, ket2dm, destroy, cr
                    lindblad_dissipator, expect, rand_herm, rand_dm)
 from .data import HipDense, HipCSR
 from .solver import (sesolve, mesolve, mcsolve, SESolver, MESolver, MCSolver, McResult, Solver,
-                     Result, Integrator, IntegratorException, IntegratorHipRK)
+                     Result, Integrator, IntegratorException, IntegratorHipRK, propagator,
+                     Propagator)
+from . import coefficient  # the module: coefficient.coefficient(...) builds one
 
 __version__ = "0.1.0"

@@ -58,6 +58,7 @@ class SolveRequest:
         self.options = {"rtol": 1e-6, "atol": 1e-8, "nsteps": 1000, **(options or {})}
         # filled by the server
         self.result = None
+        self.stderr = None
         self.t_admit = None
         self.t_done = None
         self.error = None
@@ -247,6 +248,8 @@ class _Running:
             self.req.error = f"trajectory status {sorted(set(info[:, 0].tolist()))}"
         else:
             self.req.result = ex.mean(axis=0).T
+            # standard error of the ensemble mean (real parts)
+            self.req.stderr = ex.real.std(axis=0).T / np.sqrt(max(ex.shape[0], 1))
         _lib.call("qa_event_destroy", self.event)
         _lib.call("qa_stream_destroy", self.stream)
         self.done = True

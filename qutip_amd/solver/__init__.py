@@ -7,5 +7,6 @@ from .solver_base import Solver
 from .sesolve import sesolve, SESolver
 from .mesolve import mesolve, MESolver
 from .mcsolve import mcsolve, MCSolver, McResult
+from .propagator import propagator, Propagator
 
 Solver.add_integrator(IntegratorHipRK, "hip_rk")

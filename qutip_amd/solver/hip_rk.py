@@ -587,6 +587,8 @@ class IntegratorHipRK(Integrator):
         self._small_final = None   # (t, device state) after a whole-solve run
         self._small_stats = None
         self._engine_pending = False  # the engine still waits for _small_start
+        self._src = None    # ("op", A) or ("td", (A0, parts)): what a block engine is built from
+        self._block = None  # (m, engine) for a block of m state columns
         owner = getattr(self.derivative, "__self__", None)
         dense = matrix_form_operators(owner) if o["fused"] else None
         op = constant_operator(self.derivative) if (o["fused"] and dense is None) else None
@@ -600,6 +602,7 @@ class IntegratorHipRK(Integrator):
                                         o["first_step"], o["min_step"], o["max_step"],
                                         o["interpolate"], td=td)
                 self.name = "hip_rk (fused, time-dependent coded operator)"
+                self._src = ("td", td)
             except ValueError:
                 self._engine = None  # no coded form: the plain form below
         if self._engine is not None:
@@ -619,6 +622,7 @@ class IntegratorHipRK(Integrator):
                                     coded={"auto": "auto", "coded": True,
                                            "sell": False}[o["operator_format"]])
             self.name = "hip_rk (fused, coded operator)" if self._engine.coded else "hip_rk (fused)"
+            self._src = ("op", op)
             if op.shape[0] <= _small.MAX_N:
                 # whole solves (Solver.run) of a small system in one workgroup
                 self._small = _small.SmallSolve(
@@ -630,6 +634,10 @@ class IntegratorHipRK(Integrator):
                                   o["nsteps"], o["first_step"], o["min_step"], o["max_step"],
                                   o["interpolate"])
             self.name = "hip_rk (plain)"
+        # the engine of a single state column; set_state switches to a
+        # block engine for several columns and back
+        self._engine_base = self._engine
+        self._small_base = self._small
 
     @property
     def fused(self):
@@ -641,18 +649,54 @@ class IntegratorHipRK(Integrator):
         arr = state.full() if hasattr(state, "full") else np.asarray(state)
         return _data.HipDense.from_array(arr)
 
+    def _block_engine(self, m):
+        """The fused engine for a block of m state columns (a propagator:
+        Solver.run of an operator state, propagator.py:110-200): the
+        generator kron(I_m, A) acting on the column-major buffer, so the
+        whole matrix is one state with one step size and one error norm,
+        as the reference integrates it."""
+        if self._block is not None and self._block[0] == m:
+            return self._block[1]
+        self._block = None
+        o = self.options
+        kind, src = self._src
+        args = (o["tableau"], o["rtol"], o["atol"], o["nsteps"], o["first_step"], o["min_step"],
+                o["max_step"], o["interpolate"])
+        eye = sp.identity(m, dtype=np.complex128, format="csr")
+        if kind == "op":
+            eng = RKEngine(sp.kron(eye, src, format="csr"), *args,
+                           fp32=(o["operator_precision"] == "fp32"),
+                           coded={"auto": "auto", "coded": True,
+                                  "sell": False}[o["operator_format"]])
+        else:
+            A0, parts = src
+            eng = RKEngine(None, *args, td=(sp.kron(eye, A0, format="csr"),
+                                            [(sp.kron(eye, M, format="csr"), p)
+                                             for M, p in parts]))
+        self._block = (m, eng)
+        return eng
+
     def set_state(self, t, state):
         y = self._to_device(state)
         self._shape = y.shape
-        if self._engine is not None and y.size != self._engine.n:
-            # a block of columns (e.g. a propagator): run the plain form,
-            # whose derivative handles matrices
-            o = self.options
-            self._engine = None
-            self._plain = PlainRK(self.derivative, o["tableau"], o["rtol"], o["atol"],
-                                  o["nsteps"], o["first_step"], o["min_step"], o["max_step"],
-                                  o["interpolate"])
-            self.name = "hip_rk (plain)"
+        base = self._engine_base
+        if base is not None:
+            self._engine, self._small = base, self._small_base
+        if base is not None and y.size != base.n:
+            n0 = base.n
+            if self._src is not None and y.shape[0] == n0 and y.size % n0 == 0:
+                # a block of columns (a propagator) on the fused engine
+                self._engine = self._block_engine(y.size // n0)
+                self._small = None
+            else:
+                # any other block (the matrix form): the plain form, whose
+                # derivative handles matrices
+                o = self.options
+                self._engine = None
+                self._plain = PlainRK(self.derivative, o["tableau"], o["rtol"], o["atol"],
+                                      o["nsteps"], o["first_step"], o["min_step"],
+                                      o["max_step"], o["interpolate"])
+                self.name = "hip_rk (plain)"
         self._engine_pending = False
         if self._engine is not None and self._small is not None:
             # the whole-solve kernel starts from _small_start; the engine

@@ -132,14 +132,18 @@ class Solver:
             self._options.get("normalize_output", False)
             and abs(norm - 1) <= NORM_ATOL)
         arr = state.full()
-        if self._vectorize_state and self.rhs.issuper:
+        # an operator evolves as vec(rho) under a superoperator generator; a
+        # superoperator state (a propagator) stays a matrix (solver_base.py:
+        # 87-95)
+        self._vectorized = bool(self._vectorize_state and self.rhs.issuper and not state.issuper)
+        if self._vectorized:
             vec = arr.ravel(order="F").reshape(-1, 1)
             return _data.HipDense.from_array(vec)
         return _data.HipDense.from_array(arr)
 
     def _restore_state(self, data, copy=True):
         arr = data.to_array()
-        if self._vectorize_state and self.rhs.issuper:
+        if self._vectorized:
             arr = arr.reshape(self._state_shape, order="F")
         arr = arr.reshape(self._state_shape)
         q = Qobj(arr, dims=self._state_dims, copy=False)
@@ -281,7 +285,7 @@ class Solver:
         return res
 
     def _state_view(self, st):
-        if self._vectorize_state and self.rhs.issuper:
+        if self._vectorized:
             d = self._state_shape[0]
             return st.reshape((d, d))
         return st

@@ -50,13 +50,20 @@ def kerr(N=256, delta=0.5, K=0.1, F=2.0, kappa=1.0):
                 e_ops=[a.dag() @ a, parity], tlist=np.linspace(0, 30, 150))
 
 
+SERVING_OPTIONS = {"rtol": 1e-8, "atol": 1e-10}
+
+
 def serving_trace(n, rate=200.0, seed=7, dims=(16, 32, 64, 128, 256, 512), tf=10.0, npts=50,
-                  kinds=("se", "me", "mc"), kind_p=(0.4, 0.4, 0.2), ntraj=(64, 512)):
+                  kinds=("se", "me", "mc"), kind_p=(0.4, 0.4, 0.2), ntraj=(64, 512),
+                  options=None):
     """The serving workload: Poisson arrivals (rate per second), d uniform in
     ``dims``, H = rand_herm(d, density=0.1), 1-4 collapse operators
     sqrt(gamma) rand_dm(d) with gamma ~ U[0.01, 0.1], 40% sesolve / 40% mesolve
     / 20% mcsolve (ntraj ~ U{64..512}), t in [0, 10] with 50 outputs,
-    psi0 = basis(d, 0), e_ops = {|0><0|, H}; every draw from default_rng(seed)."""
+    psi0 = basis(d, 0), e_ops = {|0><0|, H}; every draw from default_rng(seed).
+    Solver options: ``options`` (default rtol 1e-8, atol 1e-10, the
+    tolerances the <O> error bar of 1e-6 is quoted at)."""
+    options = dict(SERVING_OPTIONS if options is None else options)
     from qutip_amd.serve import SolveRequest
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -74,5 +81,6 @@ def serving_trace(n, rate=200.0, seed=7, dims=(16, 32, 64, 128, 256, 512), tf=10
         proj = qa.basis(d, 0) @ qa.basis(d, 0).dag()
         reqs.append(SolveRequest(kind, H, qa.basis(d, 0), np.linspace(0, tf, npts),
                                  c_ops=c_ops if kind != "se" else [], e_ops=[proj, H],
-                                 ntraj=nt if kind == "mc" else 0, seed=s_op, arrival=t, rid=i))
+                                 ntraj=nt if kind == "mc" else 0, seed=s_op, arrival=t, rid=i,
+                                 options=options))
     return reqs
