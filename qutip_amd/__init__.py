@@ -11,9 +11,10 @@ from .core import (Qobj, QobjEvo, LindbladMatrixForm, basis, ket2dm, destroy, cr
                    sigmaz, sigmam, sigmap, tensor, spre, spost, liouvillian,
                    lindblad_dissipator, expect, rand_herm, rand_dm)
 from .data import HipDense, HipCSR
+from .fileio import file_data_store, file_data_read, qsave, qload
 from .solver import (sesolve, mesolve, mcsolve, SESolver, MESolver, MCSolver, McResult, Solver,
                      Result, Integrator, IntegratorException, IntegratorHipRK, propagator,
-                     Propagator)
+                     Propagator, steadystate)
 from . import coefficient  # the module: coefficient.coefficient(...) builds one
 
 __version__ = "0.1.0"

@@ -8,5 +8,6 @@ from .sesolve import sesolve, SESolver
 from .mesolve import mesolve, MESolver
 from .mcsolve import mcsolve, MCSolver, McResult
 from .propagator import propagator, Propagator
+from .steadystate import steadystate
 
 Solver.add_integrator(IntegratorHipRK, "hip_rk")

@@ -57,37 +57,88 @@ def _vp(p):
 
 
 class McResult:
-    """Ensemble result with the attributes of qutip's McResult used here."""
+    """Ensemble result with the attributes of qutip's McResult used here.
+    The per-time sums of <O> and <O>^2 over the trajectories are kept
+    (_TrajectorySum, multitrajresult.py:1062-1175), so results of disjoint
+    seed sets can be merged."""
 
     def __init__(self, tlist, e_ops, runs_expect, info, col_times, col_which, seeds, stats,
                  keep_runs):
         self.times = list(tlist)
         self.e_ops = e_ops
-        self.seeds = seeds
+        self.seeds = list(seeds) if seeds is not None else []
         self.stats = stats
         n = runs_expect.shape[0]
-        self.num_trajectories = n
-        self.ntraj = n
         vals = []
         for j, O in enumerate(e_ops):
             v = runs_expect[:, :, j]
             if O.isherm:
                 v = v.real
             vals.append(v)
-        self._runs = vals
-        self.average_expect = [v.mean(axis=0) for v in vals]
-        self.std_expect = [np.sqrt(np.abs((v ** 2).mean(axis=0) - np.abs(v.mean(axis=0)) ** 2))
-                           for v in vals]
-        self.expect = self.average_expect
+        self._sum_expect = [v.sum(axis=0) for v in vals]
+        self._sum2_expect = [(v ** 2).sum(axis=0) for v in vals]
+        self._set_averages(n)
         self.runs_expect = [v.copy() for v in vals] if keep_runs else None
         nj = info[:, 1]
         self.col_times = [list(col_times[i, :min(nj[i], col_times.shape[1])]) for i in range(n)]
         self.col_which = [list(col_which[i, :min(nj[i], col_which.shape[1])]) for i in range(n)]
         self.num_jumps = nj.copy()
 
+    def _set_averages(self, n):
+        """Mean and standard deviation from the sums (multitrajresult.py:
+        261-279): std = sqrt(|mean(x^2) - |mean(x)|^2|)."""
+        self.num_trajectories = n
+        self.ntraj = n
+        self.average_expect = [s / n for s in self._sum_expect]
+        self.std_expect = [np.sqrt(np.abs(s2 / n - np.abs((s / n) ** 2)))
+                           for s, s2 in zip(self._sum_expect, self._sum2_expect)]
+        self.expect = self.average_expect
+
     @property
     def average_e_data(self):
         return dict(enumerate(self.average_expect))
+
+    def merge(self, other, p=None):
+        """The ensemble p rho + (1 - p) rho' of two results on the same
+        e_ops and times; p defaults to N / (N + N'), every trajectory
+        weighing the same (MultiTrajResult.merge, multitrajresult.py:
+        935-1037: the sums are combined with weights p / p_equal and
+        (1 - p) / (1 - p_equal))."""
+        if not isinstance(other, McResult):
+            return NotImplemented
+        if len(self.e_ops) != len(other.e_ops) or not all(
+                a is b or np.array_equal(a.full(), b.full())
+                for a, b in zip(self.e_ops, other.e_ops)):
+            raise ValueError("Shared `e_ops` is required to merge results")
+        if list(self.times) != list(other.times):
+            raise ValueError("Shared `times` are is required to merge results")
+        n = self.num_trajectories + other.num_trajectories
+        p_equal = self.num_trajectories / n
+        if p is None:
+            p = p_equal
+        w1, w2 = p / p_equal, (1 - p) / (1 - p_equal)
+        new = McResult.__new__(McResult)
+        new.times = list(self.times)
+        new.e_ops = self.e_ops
+        new.seeds = self.seeds + other.seeds
+        new.stats = dict(self.stats)
+        new.stats["run time"] = self.stats.get("run time", 0.0) + other.stats.get("run time", 0.0)
+        new.stats["end_condition"] = "Merged results"
+        new._sum_expect = [w1 * a + w2 * b for a, b in zip(self._sum_expect, other._sum_expect)]
+        new._sum2_expect = [w1 * a + w2 * b
+                            for a, b in zip(self._sum2_expect, other._sum2_expect)]
+        new._set_averages(n)
+        new.runs_expect = ([np.concatenate([a, b]) for a, b in
+                            zip(self.runs_expect, other.runs_expect)]
+                           if self.runs_expect is not None and other.runs_expect is not None
+                           else None)
+        new.col_times = self.col_times + other.col_times
+        new.col_which = self.col_which + other.col_which
+        new.num_jumps = np.concatenate([self.num_jumps, other.num_jumps])
+        return new
+
+    def __add__(self, other):
+        return self.merge(other, p=None)
 
     def __repr__(self):
         return f"<McResult ntraj={self.num_trajectories} ntimes={len(self.times)}>"
@@ -340,7 +391,7 @@ class MCSolver:
         if isinstance(e_ops, dict):
             e_ops = list(e_ops.values())
         e_ops = [] if e_ops is None else ([e_ops] if isinstance(e_ops, Qobj) else list(e_ops))
-        e_ops = [O.qobj(0) if isinstance(O, QobjEvo) else O for O in e_ops]
+        e_ops = [O(0.0) if isinstance(O, QobjEvo) else O for O in e_ops]
         if not state.isket:
             raise NotImplementedError("the batched kernel runs ket initial states")
         tlist = np.asarray(tlist, dtype=np.float64)
