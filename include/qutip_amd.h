@@ -417,6 +417,36 @@ int qa_rk_small_run(int n, int nsys, double t0, int ntimes, const double *times_
                     double max_step, int fp32, void *fun_out_dev, void *y_final_dev,
                     int *info_dev, int grid, void *stream);
 
+/* ---- batched stochastic trajectories (ssesolve / smesolve) ---------------
+ * Replaces the per-trajectory loop of MultiTrajSolver.run around
+ * SIntegrator (qutip/solver/stochastic.py:587-900, sode/sode.py:138-260)
+ * with the systems of sode/ssystem.pyx:119-250 and the Euler / Platen steps
+ * of sode/_sode.pyx:17-120: all ntraj trajectories in one launch, one
+ * workgroup per trajectory at a time.
+ *   open == 0 (ssesolve): the state is psi (n = d); L = -iH - 1/2 sum c^dag
+ *     c and M_i = c_i; drift = L psi + sum_i (e_i / 2 M_i psi - e_i^2 / 8
+ *     psi), e_i = <psi|c_i + c_i^dag|psi>; diffusion_i = M_i psi - e_i / 2
+ *     psi.
+ *   open != 0 (smesolve): the state is vec(rho) (n = d^2, column stacking);
+ *     L the Liouvillian of H, sc_ops and c_ops; M_i = spre(c_i) +
+ *     spost(c_i^dag); drift = L x, diffusion_i = M_i x - tr(M_i rho) x.
+ * method 0 Euler, 1 Platen.  L (width lw) and the M_i (width mw, block i at
+ * i mw n entries) are column-major ELL on the device: entry u of row r at
+ * u n + r, widths multiples of 4, padding entries value 0 with a column in
+ * [0, n).  counts_dev (nout - 1 ints): steps of every output interval (nsteps
+ * in total, time step dt); dW_dev [ntraj][nsteps][nsc]: the Wiener
+ * increments; x0_dev: the initial state.  e_ops: ne <= 16 CSR operators with
+ * absolute row pointers, one block of rows + 1 per operator (rows = n for
+ * ssesolve: <psi|O|psi>; rows = d for smesolve: tr(O rho)).  out_dev
+ * [ntraj][nout][ne] complex128; info_dev [ntraj]: 0, or -1 when a state
+ * stopped being finite.  n <= 1024, 1 <= nsc <= 4; grid <= 0: one workgroup
+ * per trajectory. */
+int qa_sde_run(int n, int d, int open, int nsc, int method, int ntraj, int nout,
+               const int *counts_dev, double dt, long long nsteps, int lw, const int *lcol,
+               const void *lval, int mw, const int *mcol, const void *mval, const double *dW_dev,
+               const void *x0_dev, int ne, const int *erow, const int *ecol, const void *ew,
+               void *out_dev, int *info_dev, int grid, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

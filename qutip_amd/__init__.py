@@ -14,7 +14,8 @@ from .data import HipDense, HipCSR
 from .fileio import file_data_store, file_data_read, qsave, qload
 from .solver import (sesolve, mesolve, mcsolve, SESolver, MESolver, MCSolver, McResult, Solver,
                      Result, Integrator, IntegratorException, IntegratorHipRK, propagator,
-                     Propagator, steadystate)
+                     Propagator, steadystate, ssesolve, smesolve, SSESolver, SMESolver,
+                     StochasticResult)
 from . import coefficient  # the module: coefficient.coefficient(...) builds one
 
 __version__ = "0.1.0"

@@ -13,7 +13,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 INCLUDE = HERE.parent / "include"
 LIB = HERE / "libqutip_amd.so"
-SOURCES = ["runtime.hip", "linalg.hip", "rk.hip", "mc.hip"]
+SOURCES = ["runtime.hip", "linalg.hip", "rk.hip", "mc.hip", "sde.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",

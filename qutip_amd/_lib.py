@@ -94,6 +94,8 @@ SIGNATURES = {
     "qa_mc_pack_tableau": [vp, i32, i32, i32, i32, P_f64, P_f64, P_f64, P_f64, P_f64, i32],
     "qa_rk_small_run": [i32, i32, f64, i32, vp, vp, i32, vp, vp, C.c_longlong, i32, vp, vp, vp,
                         vp, vp, vp, f64, f64, i32, f64, f64, f64, i32, vp, vp, vp, i32, vp],
+    "qa_sde_run": [i32, i32, i32, i32, i32, i32, i32, vp, f64, C.c_longlong, i32, vp, vp, i32, vp,
+                   vp, vp, vp, i32, vp, vp, vp, vp, vp, i32, vp],
     "qa_mc_run": [i32, i32, i32, vp, vp, i32, vp, vp, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp,
                   i32, f64, f64, i32, f64, f64, f64, i32, f64, f64, f64, f64, i32, vp, vp, vp,
                   i32, vp, i32, vp],

@@ -9,5 +9,6 @@ from .mesolve import mesolve, MESolver
 from .mcsolve import mcsolve, MCSolver, McResult
 from .propagator import propagator, Propagator
 from .steadystate import steadystate
+from .stochastic import ssesolve, smesolve, SSESolver, SMESolver, StochasticResult
 
 Solver.add_integrator(IntegratorHipRK, "hip_rk")
