@@ -348,3 +348,211 @@ int qto_rk_solve(int n, const int *rowptr, const int *col, const double *vals,
     free(s.k); free(s.y); free(s.y_prev); free(s.y_front); free(s.y_temp); free(s.k_fsal);
     return st;
 }
+
+/* ---- mcsolve trajectories ------------------------------------------------
+ * One trajectory of MCIntegrator (qutip/solver/mcsolve.py:245-406) on the
+ * RK above in its integrate(t, step=True) form (qutip_integrator.py:84-87):
+ * the target norm is the first draw (set_state :276-279); each mcstep
+ * that crosses the target is followed by the log-ratio secant search of
+ * _find_collapse_time (:321-369) and the jump of _do_collapse (:371-406:
+ * the channel by <c^dag c> with one draw when there are several channels,
+ * renormalisation, a fresh target draw, and a new RK started at the jump).
+ * The draws are numpy's uniform stream of the trajectory's generator,
+ * passed in.  Output: <O> of the normalised state at every time of tlist.
+ * stats[0..3] = RHS calls, attempts, jumps, draws used.  Returns 0, the RK
+ * status when negative, -10 when the collapse time is not found, -11 when
+ * the draws run out. */
+static void rk_setup(rk_t *s, int n, const int *rowptr, const int *col, const double *vals,
+                     int order, int rk_step, int rk_extra, int dorder, const double *a,
+                     const double *b, const double *c, const double *e, const double *bi,
+                     double rtol, double atol, int nsteps)
+{
+    memset(s, 0, sizeof(*s));
+    s->n = n; s->rowptr = rowptr; s->col = col; s->vals = vals;
+    s->order = order; s->rk_step = rk_step; s->rk_extra = rk_extra; s->dorder = dorder;
+    s->a = a; s->b = b; s->c = c; s->e = e; s->bi = bi;
+    s->adaptive = e != NULL;
+    s->interpolate = bi != NULL;
+    s->rtol = rtol; s->atol = atol;
+    s->min_step = 1e-15;
+    s->max_numsteps = nsteps;
+    s->fsal = 1;
+    for (int j = 0; j < rk_step; j++) {
+        double x = a[(size_t)(rk_step - 1) * rk_extra + j], yv = b[j];
+        if (fabs(x - yv) > 1e-14 + 1e-5 * fabs(yv)) s->fsal = 0;
+    }
+    int nk = s->interpolate ? rk_extra : rk_step;
+    s->k = (cplx **)malloc(sizeof(cplx *) * (size_t)nk);
+    for (int i = 0; i < nk; i++) s->k[i] = (cplx *)calloc((size_t)n, sizeof(cplx));
+    s->y = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    s->y_prev = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    s->y_front = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    s->y_temp = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    s->k_fsal = (cplx *)calloc((size_t)n, sizeof(cplx));
+}
+
+static void rk_free(rk_t *s)
+{
+    int nk = s->interpolate ? s->rk_extra : s->rk_step;
+    for (int i = 0; i < nk; i++) free(s->k[i]);
+    free(s->k); free(s->y); free(s->y_prev); free(s->y_front); free(s->y_temp);
+    free(s->k_fsal);
+}
+
+/* set_initial_value (explicit_rk.pyx:204-230) */
+static void rk_set_state(rk_t *s, double t, const cplx *y0)
+{
+    int n = s->n;
+    copy_to(y0, s->y, n);
+    copy_to(y0, s->y_prev, n);
+    copy_to(y0, s->y_front, n);
+    s->t = s->t_prev = s->t_front = t;
+    s->dt_int = 0;
+    s->status = NORMAL;
+    s->dt_safe = s->atol;
+    if (s->fsal) rhs(s, s->y, s->k_fsal);
+    s->dt_safe = s->first_step ? s->first_step : estimate_first_step(s, s->y);
+}
+
+static double nrm2(const cplx *v, int n)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += v[i].re * v[i].re + v[i].im * v[i].im;
+    return s;
+}
+
+/* <v|O|v> for a CSR O */
+static cplx quad(int n, const int *rp, const int *cl, const cplx *w, const cplx *v)
+{
+    cplx acc = {0.0, 0.0};
+    for (int r = 0; r < n; r++) {
+        double re = 0.0, im = 0.0;
+        for (int p = rp[r]; p < rp[r + 1]; p++) {
+            cplx a = w[p], x = v[cl[p]];
+            re += a.re * x.re - a.im * x.im;
+            im += a.re * x.im + a.im * x.re;
+        }
+        acc.re += v[r].re * re + v[r].im * im;
+        acc.im += v[r].re * im - v[r].im * re;
+    }
+    return acc;
+}
+
+int qto_mc_traj(int d, const int *h_rowptr, const int *h_col, const double *h_vals,
+                int nchan, const int *c_rowptr, const int *c_col, const double *c_vals,
+                const int *n_rowptr, const int *n_col, const double *n_vals,
+                int neops, const int *e_rowptr, const int *e_col, const double *e_vals,
+                const double *psi0, int ntimes, const double *tlist,
+                int order, int rk_step, int rk_extra, int dorder,
+                const double *a, const double *b, const double *c,
+                const double *e, const double *bi, double rtol, double atol, int nsteps,
+                int norm_steps, double norm_t_tol, double norm_tol, double norm_min_step,
+                double mc_corr_eps, const double *draws, int max_draws,
+                double *expect_out, long *stats)
+{
+    rk_t s;
+    rk_setup(&s, d, h_rowptr, h_col, h_vals, order, rk_step, rk_extra, dorder, a, b, c, e, bi,
+             rtol, atol, nsteps);
+    cplx *tmp = (cplx *)malloc(sizeof(cplx) * (size_t)d);
+    cplx *st = (cplx *)malloc(sizeof(cplx) * (size_t)d);
+    cplx *out = (cplx *)expect_out;
+    int used = 0, jumps = 0, rc = 0;
+#define DRAW() (used < max_draws ? draws[used++] : -1.0)
+    double target = DRAW();
+    rk_set_state(&s, tlist[0], (const cplx *)psi0);
+    const cplx *ev = (const cplx *)e_vals;
+    double nn0 = nrm2(s.y, d);
+    for (int k = 0; k < neops; k++) {
+        cplx q = quad(d, e_rowptr + (size_t)k * (d + 1), e_col, ev, s.y);
+        out[k].re = q.re / nn0;
+        out[k].im = q.im / nn0;
+    }
+    for (int it = 1; it < ntimes && rc == 0; it++) {
+        double t = tlist[it];
+        double t_old = s.t;
+        copy_to(s.y, st, d);
+        double norm_old = nrm2(st, d);
+        while (t_old < t) {
+            integrate(&s, t, 1);
+            if (s.status < 0) { rc = s.status; break; }
+            double t_step = s.t;
+            double nrm = nrm2(s.y, d);
+            if (nrm <= target) {
+                /* _find_collapse_time */
+                double t_prev = t_old, t_final = t_step, norm = nrm, t_guess = t_final;
+                int tries = 0;
+                while (tries < norm_steps) {
+                    tries++;
+                    if (t_final - t_prev < norm_t_tol) { t_guess = t_final; break; }
+                    double dt = t_final - t_prev;
+                    double ratio = log(norm_old / target) / log(norm_old / norm);
+                    if (ratio < norm_min_step) ratio = norm_min_step;
+                    if (ratio > 1 - norm_min_step) ratio = 1 - norm_min_step;
+                    t_guess = t_prev + dt * ratio;
+                    if (t_guess - t_prev < norm_t_tol) t_guess = t_prev + norm_t_tol;
+                    integrate(&s, t_guess, 1);
+                    if (s.status < 0) { rc = s.status; break; }
+                    double ng = nrm2(s.y, d);
+                    if (fabs(target - ng) < norm_tol * target) break;
+                    else if (ng < target) { t_final = t_guess; norm = ng; }
+                    else { t_prev = t_guess; norm_old = ng; }
+                }
+                if (rc) break;
+                if (tries >= norm_steps) { rc = -10; break; }
+                /* _do_collapse at t_guess with the state s.y */
+                int which = 0;
+                if (nchan > 1) {
+                    double probs[64], tot = 0.0;
+                    for (int ch = 0; ch < nchan; ch++) {
+                        cplx q = quad(d, n_rowptr + (size_t)ch * (d + 1), n_col,
+                                      (const cplx *)n_vals, s.y);
+                        probs[ch] = q.re;
+                        tot += q.re;
+                    }
+                    double u = DRAW();
+                    if (u < 0) { rc = -11; break; }
+                    double tg = tot * u - probs[0];
+                    while (tg > 0 && which <= nchan) {
+                        which++;
+                        tg -= probs[which];
+                    }
+                }
+                zero(tmp, d);
+                qto_csr_matvec(d, c_rowptr + (size_t)which * (d + 1), c_col, c_vals,
+                               (const double *)s.y, 1.0, 0.0, (double *)tmp);
+                double nj = sqrt(nrm2(tmp, d));
+                if (nj < mc_corr_eps) {
+                    double ns = sqrt(nrm2(s.y, d));
+                    for (int i = 0; i < d; i++) { tmp[i].re = s.y[i].re / ns; tmp[i].im = s.y[i].im / ns; }
+                } else {
+                    for (int i = 0; i < d; i++) { tmp[i].re /= nj; tmp[i].im /= nj; }
+                    jumps++;
+                    target = DRAW();
+                    if (target < 0) { rc = -11; break; }
+                }
+                long nr = s.n_rhs, na = s.n_attempts;
+                rk_set_state(&s, t_guess, tmp);
+                s.n_rhs += nr;
+                s.n_attempts += na;
+                t_old = s.t;
+                norm_old = 1.0;
+            } else {
+                t_old = t_step;
+                norm_old = nrm;
+            }
+        }
+        if (rc) break;
+        double nn = nrm2(s.y, d);
+        for (int k = 0; k < neops; k++) {
+            cplx q = quad(d, e_rowptr + (size_t)k * (d + 1), e_col, ev, s.y);
+            out[(size_t)it * neops + k].re = q.re / nn;
+            out[(size_t)it * neops + k].im = q.im / nn;
+        }
+    }
+#undef DRAW
+    if (stats) { stats[0] = s.n_rhs; stats[1] = s.n_attempts; stats[2] = jumps; stats[3] = used; }
+    free(tmp);
+    free(st);
+    rk_free(&s);
+    return rc;
+}

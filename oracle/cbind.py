@@ -126,3 +126,71 @@ def rk_solve(A, y0, tlist, tableau, fun, rtol=1e-6, atol=1e-8, nsteps=1000,
         C.c_double(min_step), C.c_double(max_step), C.c_int(int(interpolate)),
         C.c_int(nfun), P(fptr), P(fidx), P(fw), P(out), P(yf), P(stats))
     return out, yf, {"n_rhs": int(stats[0]), "n_attempts": int(stats[1]), "status": int(st)}
+
+
+def _csr_blocks(ops, d):
+    """CSR operators back to back with absolute row pointers (one block of
+    d + 1 per operator)."""
+    rps, cis, dvs, off = [], [], [], 0
+    for O in ops:
+        A = sp.csr_matrix(O, dtype=np.complex128)
+        A.sort_indices()
+        rps.append(A.indptr.astype(np.int64) + off)
+        cis.append(A.indices)
+        dvs.append(A.data)
+        off += A.nnz
+    rp = np.ascontiguousarray(np.concatenate(rps), dtype=np.int32)
+    ci = np.ascontiguousarray(np.concatenate(cis) if off else np.zeros(1), dtype=np.int32)
+    dv = np.ascontiguousarray(np.concatenate(dvs) if off else np.zeros(1, np.complex128),
+                              dtype=np.complex128).view(np.float64)
+    return rp, ci, dv
+
+
+class McTrajectories:
+    """The C restatement of the mcsolve trajectory loop (qto_mc_traj) for
+    one system: ``run(psi0, tlist, draws)`` gives <O>(t) of one trajectory
+    from its uniform draws (numpy's stream of its generator)."""
+
+    def __init__(self, H, c_ops, e_ops, tableau, rtol=1e-6, atol=1e-8, nsteps=1000,
+                 norm_steps=25, norm_t_tol=1e-6, norm_tol=1e-4, norm_min_step=0.1,
+                 mc_corr_eps=1e-10):
+        H = sp.csr_matrix(H, dtype=np.complex128)
+        cs = [sp.csr_matrix(c, dtype=np.complex128) for c in c_ops]
+        ns = [(c.conj().T @ c).tocsr() for c in cs]
+        Heff = -1j * H
+        for n_ in ns:
+            Heff = Heff - 0.5 * n_
+        self.d = H.shape[0]
+        self.h = _csr_parts(sp.csr_matrix(Heff))
+        self.c = _csr_blocks(cs, self.d)
+        self.n = _csr_blocks(ns, self.d)
+        self.e = _csr_blocks(e_ops, self.d)
+        self.nchan, self.neops = len(cs), len(e_ops)
+        self.tab = [np.ascontiguousarray(tableau[k], dtype=np.float64)
+                    for k in ("a", "b", "c", "e", "bi")]
+        self.order = int(tableau["order"])
+        self.opts = (rtol, atol, nsteps, norm_steps, norm_t_tol, norm_tol, norm_min_step,
+                     mc_corr_eps)
+
+    def run(self, psi0, tlist, draws):
+        lib = oracle_lib()
+        P = lambda arr: arr.ctypes.data_as(C.c_void_p)
+        psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
+        tl = np.ascontiguousarray(tlist, dtype=np.float64)
+        draws = np.ascontiguousarray(draws, dtype=np.float64)
+        out = np.zeros((len(tl), max(self.neops, 1)), dtype=np.complex128)
+        stats = np.zeros(4, dtype=np.int64)
+        a, b, c, e, bi = self.tab
+        rtol, atol, nsteps, norm_steps, norm_t_tol, norm_tol, norm_min_step, eps = self.opts
+        rc = lib.qto_mc_traj(
+            C.c_int(self.d), P(self.h[0]), P(self.h[1]), P(self.h[2]), C.c_int(self.nchan),
+            P(self.c[0]), P(self.c[1]), P(self.c[2]), P(self.n[0]), P(self.n[1]),
+            P(self.n[2]), C.c_int(self.neops), P(self.e[0]), P(self.e[1]), P(self.e[2]),
+            P(psi0), C.c_int(len(tl)), P(tl), C.c_int(self.order), C.c_int(len(b)),
+            C.c_int(len(c)), C.c_int(bi.shape[1]), P(a), P(b), P(c), P(e), P(bi),
+            C.c_double(rtol), C.c_double(atol), C.c_int(nsteps), C.c_int(norm_steps),
+            C.c_double(norm_t_tol), C.c_double(norm_tol), C.c_double(norm_min_step),
+            C.c_double(eps), P(draws), C.c_int(len(draws)), P(out), P(stats))
+        return out[:, :self.neops], {"rc": int(rc), "n_rhs": int(stats[0]),
+                                     "n_attempts": int(stats[1]), "jumps": int(stats[2]),
+                                     "draws": int(stats[3])}

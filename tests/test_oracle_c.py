@@ -78,3 +78,21 @@ def test_c_rk_too_much_work_status():
     _, _, st = cbind.rk_solve(Lm, models.vec(models.ket2dm(m["psi0"])), [0.0, 25.0],
                               tableaus.vern7, fun, rtol=1e-8, atol=1e-10, nsteps=2)
     assert st["status"] == -1
+
+
+def test_c_mc_trajectories_match_the_numpy_oracle():
+    """The C trajectory loop (the C3 CPU baseline) against oracle/mc.py on
+    the same draws: the same jumps and <O>(t) within 1e-8."""
+    from oracle import mc as omc, models, tableaus
+    from oracle.cbind import McTrajectories
+    m = models.ising(4, 0.2, tf=4.0, npts=9)
+    cm = McTrajectories(m["H"], m["c_ops"], m["e_ops"], tableaus.TABLEAUS["vern7"])
+    for seed in range(6):
+        rng = np.random.default_rng(seed)
+        draws = np.random.default_rng(seed).random(64)
+        tr = omc.Trajectory(m["H"], m["c_ops"], tableaus.TABLEAUS["vern7"])
+        want = tr.run(m["psi0"], m["tlist"], rng, m["e_ops"])
+        got, st = cm.run(m["psi0"], m["tlist"], draws)
+        assert st["rc"] == 0 and st["jumps"] == len(tr.collapses)
+        assert st["draws"] == tr.draws_used
+        assert np.abs(got - want).max() < 1e-8
