@@ -72,20 +72,14 @@ def cpu_baseline(L_csr, rho0, tlist, e_ops_funcs):
     return dt, vals, st
 
 
-def mc_cpu_baseline(ntraj=40):
-    """The restated reference trajectory loop (numpy/scipy vern7, one core)
-    on the first ``ntraj`` seeds of the same workload: trajectories/s."""
-    from oracle import mc as omc, models, tableaus
-    m = models.ising(10, 0.05, tf=20.0, npts=100)
-    t0 = time.perf_counter()
-    for i in range(ntraj):
-        tr = omc.Trajectory(m["H"], m["c_ops"], tableaus.vern7)
-        tr.run(m["psi0"], m["tlist"], np.random.default_rng(np.random.SeedSequence(i)),
-               m["e_ops"])
-    dt = time.perf_counter() - t0
-    return {"value": round(ntraj / dt, 3), "unit": "traj/s", "cores": 1, "kind": "port",
-            "sample": f"{ntraj} trajectories (seeds 0..{ntraj - 1}) of the same workload, "
-                      f"{dt:.1f} s"}
+def mc_cpu_baseline():
+    """The C restatement of the reference trajectory loop (qto_mc_traj) on
+    the same workload: one core, and the seeds farmed over the host's cores
+    as map='parallel' farms them."""
+    from oracle import baselines
+    r = baselines.c3()
+    return {"value": r["all_cores"], "unit": "traj/s", "cores": r["cores"], "kind": "port",
+            "one_core": r["one_core"], "sample": r["sample"]}
 
 
 def bench_mcsolve(qa, systems, log, ntraj=2048, reps=3, cpu=True):
@@ -170,20 +164,15 @@ def bench_jc(qa, systems, log, reps=20, cpu=True):
            "path": solver._integrator.stats().get("path", "fused engine")}
     # a sweep: 2048 couplings g in [0.02, 0.09] x 2 pi, one launch, one
     # workgroup per solve
-    from oracle import models as _m
     from qutip_amd.solver.small import SmallSolve
     gs = 2 * np.pi * np.linspace(0.02, 0.09, 2048)
     Ls = []
     for gg in gs:
-        mm = _m.jaynes_cummings(g=gg)
-        Ls.append(_m.liouvillian(mm["H"], mm["c_ops"]))
-    mm = _m.jaynes_cummings()
+        sg = systems.jaynes_cummings(g=gg)
+        Ls.append(qa.liouvillian(sg["H"], sg["c_ops"]).to_csr())
     ss = SmallSolve(Ls, "vern7", rtol=1e-8, atol=1e-10)
-    from oracle import cbind as _cb
-    fptr, fidx, fw = _cb.functionals_from_ops(mm["e_ops"], 20)
-    ss.set_functionals(np.zeros(len(fptr) - 1, np.int32), fptr, np.zeros(len(fidx), np.int32),
-                       fidx, fw.view(np.complex128))
-    rho0 = np.tile(_m.vec(_m.ket2dm(mm["psi0"])), (len(gs), 1))
+    ss.set_functionals(*solver._functionals(s["e_ops"]))
+    rho0 = np.tile(qa.ket2dm(s["psi0"]).full().ravel(order="F"), (len(gs), 1))
     ss.run(0.0, rho0[:256], tl[1:])  # warm-up
     t0 = time.perf_counter()
     vals, _, info = ss.run(0.0, rho0, tl[1:])
@@ -193,28 +182,22 @@ def bench_jc(qa, systems, log, reps=20, cpu=True):
         "all_ok": bool((info[:, 0] >= 0).all()),
         "rk_attempts_total": int(info[:, 3].sum())}
     if cpu:
-        from oracle import cbind, models, tableaus
-        m = models.jaynes_cummings()
-        Lm = models.liouvillian(m["H"], m["c_ops"])
-        fun = cbind.functionals_from_ops(m["e_ops"], 20)
-        rho0 = models.vec(models.ket2dm(m["psi0"]))
-        n = 0
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < 2.0:
-            cbind.rk_solve(Lm, rho0, m["tlist"], tableaus.vern7, fun, rtol=1e-8, atol=1e-10)
-            n += 1
-        tc = (time.perf_counter() - t0) / n
-        out["cpu_baseline"] = {"value": round(1 / tc, 2), "unit": "solves/s", "cores": 1,
-                               "kind": "port", "sample": f"{n} full solves of the same workload"}
+        from oracle import baselines
+        r = baselines.c1()
+        out["cpu_baseline"] = {"value": r["all_cores"], "unit": "solves/s", "cores": r["cores"],
+                               "kind": "port", "one_core": r["one_core"],
+                               "sample": r["sample"]}
     log(f"jc: {out}")
     return out
 
 
-def bench_dense(qa, systems, log, reps=2, tf=None):
+def bench_dense(qa, systems, log, reps=2, tf=None, cpu=True):
     """Matrix-form mesolve of the driven-dissipative Kerr oscillator, N=256
     (dense rho 256 x 256), t in [0, 30] with 150 outputs, e_ops a^dag a and
-    parity, vern7 at rtol=1e-8 atol=1e-10, on MFMA in FP32 and in the bf16 x3
-    split (FP32 accumulate); error against the FP64 fixture.  Algorithmic
+    parity, vern7 at rtol=1e-8 atol=1e-10, on MFMA in FP32 and in the bf16 x6
+    and x3 splits (FP32 accumulate), and in FP64 from the diagonals; error
+    against the FP64 fixture; the CPU baseline is the reference's banded
+    (Dia) form restated in C on a window of the solve.  Algorithmic
     FLOP per RHS: the four complex 256^3 products Hm X, X Hp, T = c X and
     T c^dag (4 * 8 * 256^3)."""
     from qutip_amd import _lib
@@ -230,7 +213,7 @@ def bench_dense(qa, systems, log, reps=2, tf=None):
     out = {"workload": "mesolve Kerr oscillator N=256 matrix form (dense rho 256x256), "
                        f"t in [0,{tl[-1]:g}], {len(tl)} outputs, vern7 rtol=1e-8 atol=1e-10",
            "algorithmic_flop_per_rhs": flop_rhs}
-    for prec in ("fp32", "bf16x3", "fp64_banded"):
+    for prec in ("fp32", "bf16x6", "bf16x3", "fp64_banded"):
         solver = qa.MESolver(s["H"], s["c_ops"], options={
             "matrix_form": True, "dense_precision": prec, "rtol": 1e-8, "atol": 1e-10})
         ts = []
@@ -250,6 +233,7 @@ def bench_dense(qa, systems, log, reps=2, tf=None):
         out[prec] = {"solves_per_s": round(1 / best, 4), "s_per_solve": round(best, 3),
                      "err_vs_fp64": float(err), "rk_attempts": st["attempts"],
                      "rhs_evals": st["rhs_evals"], "stage_us": round(rhs_ms * 1e3, 2)}
+        out[prec]["meets_1e-6"] = bool(err <= 1e-6)
         if prec == "fp64_banded":
             # Dia products of the diagonals (the reference's own form for this
             # system): no MFMA; the state is read once per stage
@@ -258,6 +242,8 @@ def bench_dense(qa, systems, log, reps=2, tf=None):
             continue
         out[prec]["mfma_tflops"] = round(tf, 2)
         out[prec]["mfma_peak_tflops"] = 157.3 if prec == "fp32" else 2500.0
+        if prec.startswith("bf16"):
+            out[prec]["hardware_products_per_real_product"] = 3 if prec == "bf16x3" else 6
         log(f"dense {prec}: {1 / best:.3f} solves/s err {err:.2e} stage {rhs_ms * 1e3:.1f} us "
             f"({tf:.1f} TFLOP/s)")
     # the same solve through the sparse Liouvillian (the banded structure
@@ -273,15 +259,65 @@ def bench_dense(qa, systems, log, reps=2, tf=None):
         "solves_per_s": round(1 / dt, 4), "err_vs_fp64": float(err), "nnz": int(Lk.nnz),
         "rk_attempts": solver._integrator.stats()["attempts"]}
     log(f"kerr sparse superoperator: {1 / dt:.3f} solves/s err {err:.2e} (nnz {Lk.nnz})")
+    if cpu and tf is None:
+        from oracle import baselines
+        r = baselines.c4(out["fp64_banded"]["rhs_evals"], tf=2.0)
+        out["cpu_baseline"] = {"value": r["all_cores"], "unit": "solves/s", "cores": r["cores"],
+                               "kind": "port", "one_core": r["one_core"],
+                               "path": "the reference's banded (Dia) matrix-form products",
+                               "sample": r["sample"]}
+        log(f"kerr CPU baseline: {out['cpu_baseline']}")
     return out
 
 
-def bench_serving(systems, log, nreq=400, warm=40, policy="fifo"):
+def served_errors(done, log):
+    """The served <O>(t) of the first requests against FP64 references, as
+    tests/test_gpu_serve.py checks them: sesolve against the matrix
+    exponential (scipy), mesolve against the committed FP64 fixture,
+    mcsolve in standard errors of that master equation, the standard error
+    bounded below by 2 ||O|| sqrt(p (1 - p) / ntraj) with p the no-jump
+    probability (most requests of this mix see no jump, so their sample
+    standard error is 0)."""
+    import scipy.sparse.linalg as spla
+    g = np.load(ROOT / "tests" / "golden" / "serving_seed7.npz")
+    nref = int(g["nreq"])
+    ref = {int(i): e for i, e in zip(g["ids"], g["expect"])}
+    worst = {"se": 0.0, "me": 0.0, "mc_over_sigma": 0.0}
+    for r in done:
+        if r.id >= nref or r.error:
+            continue
+        got = np.asarray(r.result).real
+        if r.kind == "se":
+            psis = spla.expm_multiply(-1j * r.H.to_csr().tocsc(),
+                                      r.psi0.full().ravel().astype(complex),
+                                      start=r.tlist[0], stop=r.tlist[-1], num=len(r.tlist),
+                                      endpoint=True)
+            want = np.array([[np.vdot(p, O.to_csr() @ p).real for p in psis] for O in r.e_ops])
+            worst["se"] = max(worst["se"], float(np.abs(got - want).max()))
+        elif r.kind == "me":
+            worst["me"] = max(worst["me"], float(np.abs(got - ref[r.id]).max()))
+        else:
+            H = r.H.full()
+            heff = -1j * H - 0.5 * sum(c.full().conj().T @ c.full() for c in r.c_ops)
+            psis = spla.expm_multiply(heff, r.psi0.full().ravel().astype(complex), start=0.0,
+                                      stop=r.tlist[-1], num=len(r.tlist), endpoint=True)
+            p = np.clip(1 - np.sum(np.abs(psis) ** 2, axis=1), 0.0, 1.0)
+            onorm = np.array([1.0, np.linalg.norm(H, 2)])[:, None]
+            se = np.maximum(np.asarray(r.stderr), 2 * onorm * np.sqrt(p * (1 - p) / r.ntraj))
+            dev = np.abs(got - ref[r.id]) / (se + 1e-12)
+            worst["mc_over_sigma"] = max(worst["mc_over_sigma"], float(dev.max()))
+    log(f"served error (first {nref} requests): {worst}")
+    return {"requests_checked": nref, **{k: float(f"{v:.3e}") for k, v in worst.items()}}
+
+
+def bench_serving(systems, log, nreq=2040, warm=40, policy="sjf"):
     """The serving mix: Poisson arrivals at 200 req/s (seed 7), d uniform in
-    {16..512}, 40% sesolve / 40% mesolve / 20% mcsolve, admitted by HBM
+    {16..512}, 40% sesolve / 40% mesolve / 20% mcsolve at rtol 1e-8 /
+    atol 1e-10 (the tolerances of the <O> error bar), admitted by HBM
     budget and slots, overlapped on per-request streams.  Latency =
     completion - arrival; the first ``warm`` requests are excluded from the
-    percentiles."""
+    percentiles.  The served <O> of the first requests is checked against
+    FP64 references."""
     from qutip_amd.serve import SolveServer
     t0 = time.perf_counter()
     reqs = systems.serving_trace(nreq)
@@ -294,17 +330,29 @@ def bench_serving(systems, log, nreq=400, warm=40, policy="fifo"):
     wall = time.perf_counter() - t0
     done = sorted(done, key=lambda r: r.id)
     lat = np.array([r.latency() for r in done[warm:]])
+    service = np.array([r.t_done - r.t_admit for r in done[warm:]])
     errors = [r.id for r in done if r.error]
     kinds = {k: sum(1 for r in done if r.kind == k) for k in ("se", "me", "mc")}
-    out = {"workload": f"{nreq} requests, Poisson 200 req/s seed 7, d in {{16..512}}, "
-                       f"40/40/20 se/me/mc, t_final 10, 50 outputs",
+    by_kind = {}
+    for k in ("se", "me", "mc"):
+        sv = [r.t_done - r.t_admit for r in done[warm:] if r.kind == k]
+        if sv:
+            by_kind[k] = {"p50_service_s": round(float(np.percentile(sv, 50)), 4),
+                          "p99_service_s": round(float(np.percentile(sv, 99)), 4)}
+    out = {"workload": f"{nreq} requests ({warm} of warm-up), Poisson 200 req/s seed 7, "
+                       "d in {16..512}, 40/40/20 se/me/mc, t_final 10, 50 outputs, "
+                       "rtol 1e-8 atol 1e-10",
            "admission": policy,
            "solves_per_s": round(len(done) / wall, 2), "wall_s": round(wall, 3),
            "arrival_span_s": round(reqs[-1].arrival, 3),
            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
            "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+           "p50_service_s": round(float(np.percentile(service, 50)), 4),
+           "p99_service_s": round(float(np.percentile(service, 99)), 4),
+           "service_by_kind": by_kind,
            "peak_in_flight": srv.peak_in_flight, "errors": len(errors), "kinds": kinds,
-           "trace_build_s": round(tbuild, 2)}
+           "trace_build_s": round(tbuild, 2),
+           "served_error": served_errors(done, log)}
     log(f"serving: {out['solves_per_s']} solves/s p50 {out['p50_latency_s']} s "
         f"p99 {out['p99_latency_s']} s peak {srv.peak_in_flight}")
     return out
@@ -404,6 +452,26 @@ def stage_roofline(RKEngine, L, stats, lanes, coded, reps=20):
             "algorithmic_bytes_per_launch": int(avg_bytes)}
 
 
+def host_info():
+    """The CPU the baselines ran on (lscpu's model and core counts)."""
+    info = {"os_cpu_count": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "CPU(s)", "Thread(s) per core", "Core(s) per socket",
+                             "Socket(s)"):
+                info[k.strip()] = v.strip()
+    except Exception:  # lscpu missing: the counts above remain
+        pass
+    return info
+
+
 def _lib_call(*a):
     from qutip_amd import _lib
     _lib.call(*a)
@@ -435,7 +503,7 @@ def main():
     ap.add_argument("--no-mc", action="store_true", help="skip the batched mcsolve leg")
     ap.add_argument("--no-dense", action="store_true", help="skip the dense Kerr leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-requests", type=int, default=2040)
     ap.add_argument("--dense-tf", type=float, default=None,
                     help="end the Kerr solve early (profiling runs; default t=30)")
     ap.add_argument("--dense-reps", type=int, default=2)
@@ -548,11 +616,10 @@ def main():
     if not args.no_mc:
         out["mcsolve"] = bench_mcsolve(qa, systems, log, cpu=not args.no_cpu)
     if not args.no_dense:
-        out["dense_kerr"] = bench_dense(qa, systems, log, reps=args.dense_reps, tf=args.dense_tf)
+        out["dense_kerr"] = bench_dense(qa, systems, log, reps=args.dense_reps, tf=args.dense_tf,
+                                        cpu=not args.no_cpu)
     if not args.no_serve:
         out["serving"] = bench_serving(systems, log, nreq=args.serve_requests)
-        out["serving_shortest_first"] = bench_serving(systems, log, nreq=args.serve_requests,
-                                                      policy="sjf")
     if not args.no_plain:
         psolver = qa.MESolver(s["H"], s["c_ops"], options={**opts, "fused": False})
         t0 = time.perf_counter()
@@ -563,16 +630,20 @@ def main():
         out["plain_form"] = {"solves_per_s": round(1 / tp, 4), "err_vs_fp64": float(perr)}
         log(f"plain: {1 / tp:.3f} solves/s err {perr:.2e}")
     if not args.no_cpu:
-        from oracle import cbind, models
+        from oracle import baselines, cbind, models
         rho0 = models.vec(models.ket2dm(models.ising(8, 0.1)["psi0"]))
         fun = cbind.functionals_from_ops([O.to_csr() for O in s["e_ops"]], 256)
-        log("CPU baseline: C restatement, 1 core, the full solve")
+        log("CPU baseline: C restatement, the full solve on one core, then on every core")
         dt, vals, st = cpu_baseline(L, rho0, tlist, fun)
         cerr = np.abs((vals[:, :2] / vals[:, 2:3]).real.T - golden["expect"]).max()
-        out["cpu_baseline"] = {"value": round(1 / dt, 5), "unit": "solves/s", "cores": 1,
-                               "kind": "port",
-                               "sample": f"one full solve of the same workload ({dt:.1f} s, "
-                                         f"{st['n_rhs']} RHS, err {cerr:.1e})"}
+        allc = baselines.c2()
+        out["cpu_baseline"] = {"value": allc["all_cores"], "unit": "solves/s",
+                               "cores": allc["cores"], "kind": "port",
+                               "one_core": round(1 / dt, 5),
+                               "sample": f"{allc['sample']} ({allc['per_process_s']} s each); "
+                                         f"one core alone: {dt:.1f} s, {st['n_rhs']} RHS, "
+                                         f"err {cerr:.1e}"}
+        out["host"] = host_info()
     print(json.dumps(out), flush=True)
 
 

@@ -60,6 +60,9 @@ typedef struct {
     int n;
     const int *rowptr, *col;
     const double *vals;
+    /* a right-hand side other than the CSR product (the banded matrix form) */
+    void (*rhs_fn)(void *ctx, const cplx *y, cplx *out);
+    void *rhs_ctx;
     /* tableau */
     int order, rk_step, rk_extra, dorder, interpolate, adaptive, fsal;
     const double *a, *b, *c, *e, *bi;
@@ -95,6 +98,10 @@ static double frob(const cplx *v, int n)
 static void rhs(rk_t *s, const cplx *y, cplx *out)
 {
     s->n_rhs++;
+    if (s->rhs_fn) {
+        s->rhs_fn(s->rhs_ctx, y, out);
+        return;
+    }
     zero(out, s->n);
     qto_csr_matvec(s->n, s->rowptr, s->col, s->vals, (const double *)y, 1.0, 0.0,
                    (double *)out);
@@ -555,4 +562,124 @@ int qto_mc_traj(int d, const int *h_rowptr, const int *h_col, const double *h_va
     free(st);
     rk_free(&s);
     return rc;
+}
+
+/* ---- the matrix form with banded operators --------------------------------
+ * LindbladMatrixForm.matmul_data with assume_hermitian_state
+ * (qutip/core/cy/lindblad_matrix_form.pyx:170-186) when H_nh and the c_ops
+ * are Dia (diagonal storage, the products of matmul_dia_dense_dense,
+ * qutip/core/data/matmul.pyx:429-507): A = -i H_nh rho, then for every c
+ * T = 1/2 c rho and A += T c^dag, then drho/dt = A + A^dag.  rho is N x N
+ * column-major.  A diagonal is (offset o, values v[r] = M[r, r + o]); entries
+ * outside the matrix are ignored. */
+typedef struct {
+    int N, nh, nc;
+    const int *h_off;       /* nh offsets */
+    const cplx *h_val;      /* nh x N */
+    const int *c_ndiag;     /* nc */
+    const int *c_off;       /* all c diagonals back to back */
+    const cplx *c_val;
+    cplx *T;                /* N x N work */
+} dia_mf_t;
+
+static void dia_times_dense(int N, int nd, const int *off, const cplx *val, const cplx *x,
+                            cplx *out, double sre, double sim)
+{
+    /* out[r, j] += s sum_d v_d[r] x[r + o_d, j] */
+    for (int d = 0; d < nd; d++) {
+        int o = off[d];
+        const cplx *v = val + (size_t)d * N;
+        int r0 = o < 0 ? -o : 0, r1 = o > 0 ? N - o : N;
+        for (int j = 0; j < N; j++) {
+            const cplx *xc = x + (size_t)j * N;
+            cplx *oc = out + (size_t)j * N;
+            for (int r = r0; r < r1; r++) {
+                double ar = v[r].re * sre - v[r].im * sim, ai = v[r].re * sim + v[r].im * sre;
+                cplx xv = xc[r + o];
+                oc[r].re += ar * xv.re - ai * xv.im;
+                oc[r].im += ar * xv.im + ai * xv.re;
+            }
+        }
+    }
+}
+
+static void dense_times_dia_dag(int N, int nd, const int *off, const cplx *val, const cplx *T,
+                                cplx *out)
+{
+    /* out[r, j] += sum_d T[r, j + o_d] conj(c[j, j + o_d]) = (T c^dag)[r, j] */
+    for (int d = 0; d < nd; d++) {
+        int o = off[d];
+        const cplx *v = val + (size_t)d * N;
+        int j0 = o < 0 ? -o : 0, j1 = o > 0 ? N - o : N;
+        for (int j = j0; j < j1; j++) {
+            double cr = v[j].re, ci = -v[j].im;
+            const cplx *tc = T + (size_t)(j + o) * N;
+            cplx *oc = out + (size_t)j * N;
+            for (int r = 0; r < N; r++) {
+                oc[r].re += tc[r].re * cr - tc[r].im * ci;
+                oc[r].im += tc[r].re * ci + tc[r].im * cr;
+            }
+        }
+    }
+}
+
+static void dia_mf_rhs(void *ctx, const cplx *rho, cplx *out)
+{
+    dia_mf_t *m = (dia_mf_t *)ctx;
+    int N = m->N;
+    size_t NN = (size_t)N * N;
+    memset(out, 0, sizeof(cplx) * NN);
+    dia_times_dense(N, m->nh, m->h_off, m->h_val, rho, out, 0.0, -1.0);
+    int base = 0;
+    for (int k = 0; k < m->nc; k++) {
+        int nd = m->c_ndiag[k];
+        memset(m->T, 0, sizeof(cplx) * NN);
+        dia_times_dense(N, nd, m->c_off + base, m->c_val + (size_t)base * N, rho, m->T, 0.5, 0.0);
+        dense_times_dia_dag(N, nd, m->c_off + base, m->c_val + (size_t)base * N, m->T, out);
+        base += nd;
+    }
+    /* out += out^dag (iadd_adjoint_dense) */
+    for (int j = 0; j < N; j++)
+        for (int r = 0; r <= j; r++) {
+            cplx a = out[(size_t)j * N + r], b = out[(size_t)r * N + j];
+            cplx v = {a.re + b.re, a.im - b.im};
+            out[(size_t)j * N + r] = v;
+            out[(size_t)r * N + j].re = v.re;
+            out[(size_t)r * N + j].im = -v.im;
+        }
+}
+
+/* qto_rk_solve on the banded matrix form; t_end: stop after this time (a
+ * bounded sample of the window).  stats as qto_rk_solve. */
+int qto_rk_solve_dia(int N, int nh, const int *h_off, const double *h_val, int nc,
+                     const int *c_ndiag, const int *c_off, const double *c_val,
+                     const double *rho0, int ntimes, const double *tlist,
+                     int order, int rk_step, int rk_extra, int dorder,
+                     const double *a, const double *b, const double *c,
+                     const double *e, const double *bi, double rtol, double atol, int nsteps,
+                     int nfun, const int *fun_ptr, const int *fun_idx, const double *fun_w,
+                     double *fun_out, long *stats)
+{
+    rk_t s;
+    int n = N * N;
+    rk_setup(&s, n, NULL, NULL, NULL, order, rk_step, rk_extra, dorder, a, b, c, e, bi, rtol,
+             atol, nsteps);
+    dia_mf_t m = {N, nh, nc, h_off, (const cplx *)h_val, c_ndiag, c_off, (const cplx *)c_val,
+                  (cplx *)malloc(sizeof(cplx) * (size_t)n)};
+    s.rhs_fn = dia_mf_rhs;
+    s.rhs_ctx = &m;
+    rk_set_state(&s, tlist[0], (const cplx *)rho0);
+    cplx *fo = (cplx *)fun_out;
+    eval_functionals(s.y, nfun, fun_ptr, fun_idx, (const cplx *)fun_w, fo);
+    for (int it = 1; it < ntimes; it++) {
+        integrate(&s, tlist[it], 0);
+        if (s.status < 0) break;
+        eval_functionals(s.y, nfun, fun_ptr, fun_idx, (const cplx *)fun_w,
+                         fo + (size_t)it * nfun);
+    }
+    if (stats) { stats[0] = s.n_rhs; stats[1] = s.n_attempts; stats[2] = s.status; }
+    int st = s.status;
+    free(m.T);
+    rk_free(&s);
+    return st;
 }

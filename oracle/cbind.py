@@ -194,3 +194,57 @@ class McTrajectories:
         return out[:, :self.neops], {"rc": int(rc), "n_rhs": int(stats[0]),
                                      "n_attempts": int(stats[1]), "jumps": int(stats[2]),
                                      "draws": int(stats[3])}
+
+
+def diagonals(M, tol=0.0):
+    """(offsets, values [nd, N]) of a square matrix with values[d][r] =
+    M[r, r + o_d] (zero outside the matrix); the non-zero diagonals only."""
+    M = np.asarray(M.toarray() if sp.issparse(M) else M, dtype=np.complex128)
+    N = M.shape[0]
+    offs, vals = [], []
+    for o in range(-N + 1, N):
+        dg = np.diagonal(M, o)
+        if np.abs(dg).max() > tol:
+            v = np.zeros(N, dtype=np.complex128)
+            if o >= 0:
+                v[:N - o] = dg
+            else:
+                v[-o:] = dg
+            offs.append(o)
+            vals.append(v)
+    return np.asarray(offs, dtype=np.int32), np.asarray(vals).reshape(len(offs), N)
+
+
+def rk_solve_dia(H, c_ops, rho0, tlist, tableau, fun, rtol=1e-8, atol=1e-10, nsteps=10 ** 6):
+    """The C restatement of the RK on the banded matrix form (qto_rk_solve_dia):
+    H_nh = H - i/2 sum c^dag c and the c_ops as diagonals.  Returns
+    (values[ntimes, nfun], stats)."""
+    lib = oracle_lib()
+    H = np.asarray(H.toarray() if sp.issparse(H) else H, dtype=np.complex128)
+    cs = [np.asarray(c.toarray() if sp.issparse(c) else c, dtype=np.complex128) for c in c_ops]
+    Hnh = H - 0.5j * sum((c.conj().T @ c for c in cs), np.zeros_like(H))
+    ho, hv = diagonals(Hnh)
+    cd = [diagonals(c) for c in cs]
+    c_nd = np.asarray([len(o) for o, _ in cd] or [0], dtype=np.int32)
+    c_off = np.ascontiguousarray(np.concatenate([o for o, _ in cd]) if cd else np.zeros(1),
+                                 dtype=np.int32)
+    c_val = np.ascontiguousarray(np.concatenate([v for _, v in cd]) if cd else
+                                 np.zeros((1, H.shape[0]), np.complex128))
+    hv = np.ascontiguousarray(hv)
+    N = H.shape[0]
+    rho0 = np.ascontiguousarray(rho0, dtype=np.complex128)
+    tl = np.ascontiguousarray(tlist, dtype=np.float64)
+    a, b, c, e, bi = [np.ascontiguousarray(tableau[k], dtype=np.float64)
+                      for k in ("a", "b", "c", "e", "bi")]
+    fptr, fidx, fw = fun
+    nfun = len(fptr) - 1
+    out = np.zeros((len(tl), nfun), dtype=np.complex128)
+    stats = np.zeros(3, dtype=np.int64)
+    P = lambda arr: arr.ctypes.data_as(C.c_void_p)
+    st = lib.qto_rk_solve_dia(
+        C.c_int(N), C.c_int(len(ho)), P(ho), P(hv.view(np.float64)), C.c_int(len(cs)),
+        P(c_nd), P(c_off), P(c_val.view(np.float64)), P(rho0), C.c_int(len(tl)), P(tl),
+        C.c_int(tableau["order"]), C.c_int(len(b)), C.c_int(len(c)), C.c_int(bi.shape[1]),
+        P(a), P(b), P(c), P(e), P(bi), C.c_double(rtol), C.c_double(atol), C.c_int(nsteps),
+        C.c_int(nfun), P(fptr), P(fidx), P(fw), P(out), P(stats))
+    return out, {"n_rhs": int(stats[0]), "n_attempts": int(stats[1]), "status": int(st)}

@@ -10,11 +10,13 @@
 //   row by row (the reference does a copy plus one zaxpy per coefficient per
 //   stage, explicit_rk.pyx:432-437);
 // * the last stage's epilogue forms y_front, the embedded error vector and
-//   the weighted RMS norm partials (ode.pyx:27-52); the last workgroup to
-//   finish reduces them in a fixed order and runs the step controller on the
+//   the weighted RMS norm partials (ode.pyx:27-52); a one-workgroup control
+//   kernel reduces them in a fixed order and runs the step controller on the
 //   device, so the host never waits on a step;
 // * dense-output stages, the output-time interpolation and the expectation
 //   values ("functionals") are further kernels gated by the device phase;
+//   with one chunk per functional the last output workgroup to finish also
+//   advances the output counters, so no completion launch follows;
 // * one loop iteration (all of the above) is captured once into a hipGraph
 //   and replayed; kernels whose phase does not match return immediately.
 //
@@ -1117,6 +1119,10 @@ struct OutArgs {
     // array: a per-thread index into a kernel-argument array becomes one
     // dependent kernarg load per element)
     const double *bi;
+    // nchunks == 1: every workgroup of the output grid arrives here once and
+    // the last one completes the output (rk_output_finish_kernel's work), so
+    // no second launch follows; zero between launches
+    unsigned int *arrivals;
 };
 
 // Number of pending outputs (output times inside [t_prev, t_front]) and
@@ -1221,12 +1227,34 @@ __global__ __launch_bounds__(256) void rk_store_kernel(OutArgs A) {
     }
 }
 
+// Completion of an output (thread 0 of one workgroup): the time and status
+// of the last output, the counters, and the next phase (route).
+__device__ void finish_outputs(RKCtl *c, int ph, int nq, const int *front) {
+    if (nq == 0) {
+        // a non-finite step left no output time inside [t_prev, t_front]
+        c->status = ST_OUTSIDE_RANGE;
+        c->phase = PH_ERROR;
+        return;
+    }
+    if (ph == PH_DENSE) {
+        c->dense_ready = 1;
+        c->n_dense++;
+    }
+    c->t_last = c->times[c->out_idx + nq - 1];
+    c->status = front[nq - 1] ? ST_AT_FRONT : ST_INTERPOLATED;
+    c->n_outputs += nq;
+    c->out_idx += nq;
+    route(c);
+}
+
 // Functionals of the pending output states: linear f = sum w y[col] (a
 // vectorised density matrix: tr(O rho), tr(rho)) or quadratic
 // f = sum w conj(y[row]) y[col] (a ket: <psi|O|psi>, ||psi||^2).  Grid
 // (nchunks, nfun, kMaxPending): one workgroup per chunk, functional and
-// pending output time (layers past the pending count return at once); each
-// writes the partial sum of its chunk, which rk_output_finish_kernel adds up.
+// pending output time (layers past the pending count skip the sum).  With
+// several chunks each workgroup writes the partial sum of its chunk, which
+// rk_output_finish_kernel adds up; with one chunk the sums are the values
+// and the last workgroup to finish completes the output.
 __global__ __launch_bounds__(256) void rk_output_kernel(OutArgs A) {
     RKCtl *c = A.ctl;
     // the functional's range does not depend on the control block: it is
@@ -1235,29 +1263,56 @@ __global__ __launch_bounds__(256) void rk_output_kernel(OutArgs A) {
     const int fbeg = A.fptr[f], fend = A.fptr[f + 1];
     const int quad = A.fkind[f];
     const int ph = c->phase;
-    if (ph != PH_DENSE && ph != PH_OUTPUT) return;
+    // one chunk per functional: the values go straight to fun_out and the
+    // last workgroup to arrive completes the output (uniform over the grid)
+    const bool one_pass = A.nchunks == 1;
+    const bool active = ph == PH_DENSE || ph == PH_OUTPUT;
+    if (!active && !one_pass) return;
     __shared__ double w[kMaxPending][kMaxStages];
     __shared__ int front[kMaxPending];
     __shared__ int nq_s;
     __shared__ double smem[256 / QA_WAVE];
-    const int nq = pending_outputs(c, A, w, front, &nq_s);
-    const int q = blockIdx.z;  // one output time per workgroup layer
-    if (q >= nq) return;       // uniform over the workgroup
-    const double2 *yprev = c->cur ? A.y1 : A.y0;
-    const double2 *yfront = c->cur ? A.y0 : A.y1;
-    const int beg = fbeg + blockIdx.x * kOutChunk;
-    const int end = min(fend, beg + kOutChunk);
-    double2 acc = make_double2(0.0, 0.0);
-    for (int e = beg + threadIdx.x; e < end; e += blockDim.x) {
-        double2 v = cmul(A.fw[e], state_at(A, yprev, yfront, w[q], front[q], A.fcol[e]));
-        if (quad) {
-            const double2 l = state_at(A, yprev, yfront, w[q], front[q], A.frow[e]);
-            v = cmul(make_double2(l.x, -l.y), v);
+    int nq = 0;
+    if (active) {
+        nq = pending_outputs(c, A, w, front, &nq_s);
+        const int q = blockIdx.z;  // one output time per workgroup layer
+        if (q < nq) {              // uniform over the workgroup
+            const double2 *yprev = c->cur ? A.y1 : A.y0;
+            const double2 *yfront = c->cur ? A.y0 : A.y1;
+            const int beg = fbeg + blockIdx.x * kOutChunk;
+            const int end = min(fend, beg + kOutChunk);
+            double2 acc = make_double2(0.0, 0.0);
+            for (int e = beg + threadIdx.x; e < end; e += blockDim.x) {
+                double2 v = cmul(A.fw[e], state_at(A, yprev, yfront, w[q], front[q], A.fcol[e]));
+                if (quad) {
+                    const double2 l = state_at(A, yprev, yfront, w[q], front[q], A.frow[e]);
+                    v = cmul(make_double2(l.x, -l.y), v);
+                }
+                acc = cadd(acc, v);
+            }
+            const double2 sq = block_sum2(acc, smem);
+            if (threadIdx.x == 0) {
+                if (!one_pass)
+                    A.partials[((int64_t)q * A.nfun + f) * A.nchunks + blockIdx.x] = sq;
+                else if (f < A.nfun)
+                    c->fun_out[(int64_t)(c->out_idx + q) * A.nfun + f] = sq;
+            }
         }
-        acc = cadd(acc, v);
     }
-    const double2 sq = block_sum2(acc, smem);
-    if (threadIdx.x == 0) A.partials[((int64_t)q * A.nfun + f) * A.nchunks + blockIdx.x] = sq;
+    if (!one_pass) return;
+    // Every workgroup arrives once, after all its threads' reads of the
+    // control block, so the last one to arrive may change it.  It resets the
+    // counter for the next launch.
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned total = gridDim.x * gridDim.y * gridDim.z;
+        if (atomicAdd(A.arrivals, 1u) == total - 1) {
+            __threadfence();
+            *A.arrivals = 0u;
+            if (active) finish_outputs(c, ph, nq, front);
+        }
+    }
 }
 
 // Completes an output launch: sums the partial sums of every (output time,
@@ -1282,21 +1337,7 @@ __global__ __launch_bounds__(256) void rk_output_finish_kernel(OutArgs A) {
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    if (nq == 0) {
-        // a non-finite step left no output time inside [t_prev, t_front]
-        c->status = ST_OUTSIDE_RANGE;
-        c->phase = PH_ERROR;
-        return;
-    }
-    if (ph == PH_DENSE) {
-        c->dense_ready = 1;
-        c->n_dense++;
-    }
-    c->t_last = c->times[c->out_idx + nq - 1];
-    c->status = front[nq - 1] ? ST_AT_FRONT : ST_INTERPOLATED;
-    c->n_outputs += nq;
-    c->out_idx += nq;
-    route(c);
+    finish_outputs(c, ph, nq, front);
 }
 
 // Start of a run over new output times.
@@ -1368,6 +1409,7 @@ struct qa_rk_solver {
     double2 *y_last = nullptr, *tmp = nullptr;
     double *partials = nullptr;
     RKCtl *ctl = nullptr;
+    unsigned int *arrivals = nullptr;  // the output kernel's arrival counter
     double *bi_dev = nullptr;
     double *times_dev = nullptr;
     int times_cap = 0;
@@ -1609,6 +1651,7 @@ OutArgs out_args(qa_rk_solver *S) {
     A.partials = S->fpart_dev;
     A.nchunks = S->nchunks;
     A.y_last = S->y_last;
+    A.arrivals = S->arrivals;
     return A;
 }
 
@@ -1646,8 +1689,10 @@ int enqueue_iteration(qa_rk_solver *S) {
     hipLaunchKernelGGL(rk_output_kernel, dim3(S->nchunks, std::max(S->nfun, 1), kMaxPending),
                        dim3(256), 0, S->stream, A);
     QA_CHECK_LAUNCH();
-    hipLaunchKernelGGL(rk_output_finish_kernel, dim3(1), dim3(256), 0, S->stream, A);
-    QA_CHECK_LAUNCH();
+    if (S->nchunks > 1) {  // one chunk: rk_output_kernel completes the output itself
+        hipLaunchKernelGGL(rk_output_finish_kernel, dim3(1), dim3(256), 0, S->stream, A);
+        QA_CHECK_LAUNCH();
+    }
     return QA_OK;
 }
 
@@ -1691,7 +1736,7 @@ void free_all(qa_rk_solver *S) {
     if (S->event) (void)hipEventDestroy(S->event);
     void *ptrs[] = {S->block, S->dense_block, S->ctl, S->bi_dev, S->times_dev, S->fun_out_dev, S->fptr_dev,
                     S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev, S->td_block,
-                    S->vtab_t};
+                    S->vtab_t, S->arrivals};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (void *p : S->retired) (void)hipFree(p);
@@ -1774,6 +1819,10 @@ int rk_finish_create(qa_rk_solver *S, int order, int rk_step, int rk_extra, int 
     if (rc) return fail(rc);
     he = hipMalloc((void **)&S->ctl, sizeof(RKCtl));
     if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMalloc(ctl)", __FILE__, __LINE__));
+    he = hipMalloc((void **)&S->arrivals, 256);
+    if (he != hipSuccess) return fail(qa::hip_fail(he, "hipMalloc(arrivals)", __FILE__, __LINE__));
+    rc = qa_fill_zero(S->arrivals, 256, S->stream);
+    if (rc) return fail(rc);
     {
         // dense-output table, rows padded to kMaxDenseOrder (zeros without
         // interpolation: the output kernels then never read it)

@@ -7,12 +7,11 @@ import numpy as np
 import qutip_amd as qa
 
 
-def jaynes_cummings(N=10):
+def jaynes_cummings(N=10, g=0.05 * 2 * np.pi):
     a = qa.tensor(qa.destroy(N), qa.qeye(2))
     sm = qa.tensor(qa.qeye(N), qa.sigmam())
     sz = qa.tensor(qa.qeye(N), qa.sigmaz())
     wc = wq = 2 * np.pi
-    g = 0.05 * 2 * np.pi
     H = wc * a.dag() @ a + wq * 0.5 * sz + g * (a @ sm.dag() + a.dag() @ sm)
     c_ops = [np.sqrt(0.005) * a, np.sqrt(0.05) * sm]
     psi0 = qa.tensor(qa.basis(N, 0), qa.basis(2, 0))

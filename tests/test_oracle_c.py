@@ -96,3 +96,17 @@ def test_c_mc_trajectories_match_the_numpy_oracle():
         assert st["rc"] == 0 and st["jumps"] == len(tr.collapses)
         assert st["draws"] == tr.draws_used
         assert np.abs(got - want).max() < 1e-8
+
+
+def test_c_banded_matrix_form_vs_exact():
+    """The C banded matrix-form RK (the C4 CPU baseline) on a small driven
+    Kerr oscillator against the FP64 matrix exponential of the Liouvillian."""
+    from oracle import cbind, exact, models, tableaus
+    m = models.kerr(16)
+    tl = np.linspace(0, 3, 7)
+    rho0 = models.vec(models.ket2dm(m["psi0"]))
+    fun = cbind.functionals_from_ops(m["e_ops"], 16)
+    vals, st = cbind.rk_solve_dia(m["H"], m["c_ops"], rho0, tl, tableaus.TABLEAUS["vern7"], fun)
+    got = (vals[:, :2] / vals[:, 2:3]).real.T
+    ref = exact.mesolve_expect(m["H"], m["c_ops"], m["psi0"], tl, m["e_ops"])
+    assert st["status"] >= 0 and np.abs(got - ref).max() < 1e-7
