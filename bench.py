@@ -125,12 +125,32 @@ def bench_mcsolve(qa, systems, log, ntraj=2048, reps=3, cpu=True):
         width = solver._device_ops(prec == "fp32")[1][0]
         vb = 8 if prec == "fp32" else 16
         op_bytes = 10 * float(info[:, 3].sum()) * width * s["H"].shape[0] * (vb + 4)
+        # the algorithmic bytes of a batched RHS: every trajectory's state
+        # read and its derivative written (2 d 16 B, complex128; 2 d 8 B for
+        # complex64) plus the operator once per batch of ntraj; at least 10
+        # RHS per attempt (the vern7 step stages; dense-output stages add more)
+        rhs = 10 * float(info[:, 3].sum())
+        d = s["H"].shape[0]
+        alg = rhs * 2 * d * (vb) + rhs / ntraj * (s["H"].nnz * 20 + (d + 1) * 4)
+        achieved = alg / best / 1e9
         out[prec] = {"traj_per_s": round(ntraj / best, 1), "s_per_batch": round(best, 4),
                      "all_ok": ok, "jumps": int(info[:, 1].sum()),
                      "rk_attempts": int(info[:, 3].sum()),
                      "max_dev_over_sigma_vs_mesolve": round(worst_sigma, 3),
                      "ell_width": int(width),
-                     "operator_l2_read_gbs_lower_bound": round(op_bytes / best / 1e9, 1)}
+                     "operator_l2_read_gbs_lower_bound": round(op_bytes / best / 1e9, 1),
+                     "roofline": {
+                         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": read_mc_traffic(prec),
+                         "kernel": "mc_traj_kernel (one launch, every trajectory)",
+                         "avg_launch_us": round(best * 1e6, 1),
+                         "algorithmic_bytes_per_launch": int(alg),
+                         "algorithmic_unit": "per batched RHS: the states read and written "
+                                             "(2 d 16 B per trajectory) and the operator once; "
+                                             f"{int(rhs)} trajectory RHS (10 per attempt, a "
+                                             "lower bound), so achieved is a lower bound; the "
+                                             "launch time is the host-timed batch"}}
         log(f"mcsolve {prec}: {ntraj / best:.0f} traj/s ({best * 1e3:.1f} ms), worst "
             f"{worst_sigma:.2f} sigma, jumps {int(info[:, 1].sum())}")
     if cpu:
@@ -197,9 +217,10 @@ def bench_dense(qa, systems, log, reps=2, tf=None, cpu=True):
     parity, vern7 at rtol=1e-8 atol=1e-10, on MFMA in FP32 and in the bf16 x6
     and x3 splits (FP32 accumulate), and in FP64 from the diagonals; error
     against the FP64 fixture; the CPU baseline is the reference's banded
-    (Dia) form restated in C on a window of the solve.  Algorithmic
-    FLOP per RHS: the four complex 256^3 products Hm X, X Hp, T = c X and
-    T c^dag (4 * 8 * 256^3)."""
+    (Dia) form restated in C on a window of the solve.  Algorithmic FLOP
+    per RHS: the Hermitian form's three complex 256^3 products H_nh X, c X
+    and (c X) c^dag (3 * 8 * 256^3 = 402.7 MFLOP), which the reference's
+    A + A^dag computes for a Hermitian rho."""
     from qutip_amd import _lib
     s = systems.kerr(256)
     g = np.load(ROOT / "tests" / "golden" / "kerr_full_mesolve.npz")
@@ -209,7 +230,7 @@ def bench_dense(qa, systems, log, reps=2, tf=None, cpu=True):
         keep = tl <= tf
         tl, ref = tl[keep], ref[:, keep]
     N = 256
-    flop_rhs = 4 * 8 * N ** 3
+    flop_rhs = 3 * 8 * N ** 3
     out = {"workload": "mesolve Kerr oscillator N=256 matrix form (dense rho 256x256), "
                        f"t in [0,{tl[-1]:g}], {len(tl)} outputs, vern7 rtol=1e-8 atol=1e-10",
            "algorithmic_flop_per_rhs": flop_rhs}
@@ -233,6 +254,8 @@ def bench_dense(qa, systems, log, reps=2, tf=None, cpu=True):
         out[prec] = {"solves_per_s": round(1 / best, 4), "s_per_solve": round(best, 3),
                      "err_vs_fp64": float(err), "rk_attempts": st["attempts"],
                      "rhs_evals": st["rhs_evals"], "stage_us": round(rhs_ms * 1e3, 2)}
+        if prec != "fp64_banded":
+            out[prec]["hermitian_tiles"] = eng.hermitian
         out[prec]["meets_1e-6"] = bool(err <= 1e-6)
         if prec == "fp64_banded":
             # Dia products of the diagonals (the reference's own form for this
@@ -396,6 +419,50 @@ def pmc_workload():
         f"lanes 1, 2, 4, 8, plain then coded (nnz {L.nnz})")
 
 
+MC_TRAFFIC_FILE = ROOT / "profiles" / "mc_traj_kernel_ising10_traffic.txt"
+
+
+def read_mc_traffic(prec):
+    """HBM bytes of one batched mcsolve launch (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy calibration) from the committed counter summary."""
+    try:
+        for line in MC_TRAFFIC_FILE.read_text().splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == f"traffic_bytes_per_launch_{prec}":
+                return int(float(v.strip()))
+    except OSError:
+        pass
+    return None
+
+
+def mc_probe(reps=2):
+    """Only the batched mcsolve launches (TFIM L=10, 2048 trajectories,
+    complex128 then complex64), after a 64 MiB device copy used as the
+    counter calibration: the command the trajectory kernel's counters are
+    read on."""
+    from qutip_amd import _lib, data as qd
+    from qutip_amd.solver.mcsolve import MCSolver, _read_seeds, draws_for
+    from tests import systems
+    import qutip_amd as qa  # noqa: F401
+    a, b = qd.DeviceBuffer(COPY_BYTES), qd.DeviceBuffer(COPY_BYTES)
+    _lib.call("qa_fill_zero", C.c_void_p(a.ptr), C.c_size_t(COPY_BYTES), None)
+    for _ in range(3):
+        _lib.call("qa_memcpy_d2d", C.c_void_p(b.ptr), C.c_void_p(a.ptr), C.c_size_t(COPY_BYTES),
+                  None)
+    _lib.call("qa_device_sync")
+    s = systems.ising(10, 0.05, tf=20.0, npts=100)
+    tlist = np.linspace(0, 20, 100)
+    seeds = _read_seeds(list(range(2048)), 2048)
+    dev_draws = (qd.DeviceBuffer.from_array(draws_for(seeds, 64)), 2048, 64)
+    psi0 = s["psi0"].full().ravel()
+    for prec in ("fp64", "fp32"):
+        solver = MCSolver(s["H"], s["c_ops"], options={"precision": prec})
+        for _ in range(reps):
+            solver.run_batch(psi0, tlist, s["e_ops"], dev_draws)
+        _lib.call("qa_device_sync")
+        print(f"mc probe {prec}: {reps} launches", flush=True)
+
+
 def dense_rhs_probe(N, nc, prec="fp32"):
     """Time the matrix-form Lindblad RHS alone (stage kernels in solve order,
     qa_rk_profile_sequence) for a random N x N system with nc dense collapse
@@ -409,13 +476,16 @@ def dense_rhs_probe(N, nc, prec="fp32"):
           for _ in range(nc)]
     Hnh = H - 0.5j * sum(c.conj().T @ c for c in cs)
     eng = RKEngine(None, "vern7", 1e-6, 1e-8, dense=(Hnh, cs), dense_precision=prec)
+    eng.set_hermitian(True)  # mesolve's default for a Hermitian rho
     eng.set_state(0.0, (np.eye(N, dtype=complex) / N).ravel(order="F"))
     ts, td = C.c_double(), C.c_double()
     _lib.call("qa_rk_profile_sequence", eng.handle, 10, C.byref(ts), C.byref(td))
     us = ts.value * 1e3
-    flop = (2 + 2 * nc) * 8 * N ** 3
+    flop = (1 + 2 * nc) * 8 * N ** 3  # H_nh X, c X, (c X) c^dag: the Hermitian form
     print(json.dumps({"dense_rhs_probe": {"N": N, "nc": nc, "precision": prec,
+                                          "hermitian": eng.hermitian,
                                           "us_per_rhs": round(us, 2),
+                                          "algorithmic_flop_per_rhs": flop,
                                           "tflops": round(flop / (us * 1e-6) / 1e12, 2)}}),
           flush=True)
 
@@ -439,17 +509,30 @@ def stage_roofline(RKEngine, L, stats, lanes, coded, reps=20):
     avg_ms = (n_step * t_step.value + n_dense * t_dense.value) / (n_step + n_dense)
     avg_bytes = float((stats["attempts"] * byts[:10].sum() + stats["dense_preps"] * byts[10:].sum())
                       / (n_step + n_dense))
-    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    # the algorithmic bytes of one RHS of the CSR complex128 product (values
+    # and column indices of every non-zero, the row pointers, x read and
+    # y written once); one stage launch evaluates one RHS
+    rhs_bytes = nnz * 20 + (n + 1) * 4 + 2 * n * 16
+    achieved = rhs_bytes / (avg_ms * 1e-3) / 1e9
+    own = avg_bytes / (avg_ms * 1e-3) / 1e9
     log(f"stage kernel ({'coded' if coded else 'plain'}, lanes {lanes}) avg {avg_ms * 1e3:.2f} us "
-        f"(step stages {t_step.value * 1e3:.2f}, dense {t_dense.value * 1e3:.2f}), "
-        f"{avg_bytes / 1e6:.2f} MB -> {achieved:.0f} GB/s")
+        f"(step stages {t_step.value * 1e3:.2f}, dense {t_dense.value * 1e3:.2f}): "
+        f"{rhs_bytes / 1e6:.2f} MB per RHS -> {achieved:.0f} GB/s; the form's own "
+        f"{avg_bytes / 1e6:.2f} MB -> {own:.0f} GB/s")
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(lanes, coded),
             "kernel": f"rk_stage_kernel<double2, {lanes}, {form}>",
             "kernel_form": {0: "plain SELL-C", 1: "coded SELL-C",
                             2: "coded SELL-C, slices of one width"}[form],
             "avg_launch_us": round(avg_ms * 1e3, 3),
-            "algorithmic_bytes_per_launch": int(avg_bytes)}
+            "algorithmic_bytes_per_launch": int(rhs_bytes),
+            "algorithmic_unit": "one RHS of the CSR complex128 product: nnz*20 + (n+1)*4 + "
+                                "2*n*16 bytes; one RHS per launch",
+            "form_bytes_per_launch": int(avg_bytes),
+            "form_gbs": round(own, 1),
+            "form_bytes_note": "what this operator form and the fused stage combine read and "
+                               "write per launch (coded: 2-byte codes and two 256-entry "
+                               "tables instead of values and columns)"}
 
 
 def host_info():
@@ -511,7 +594,12 @@ def main():
                     help="only run the launches the HBM-traffic counters are read on")
     ap.add_argument("--dense-rhs-probe", type=int, nargs=2, metavar=("N", "NC"),
                     help="only time the matrix-form RHS at size N with NC collapse operators")
+    ap.add_argument("--mc-probe", action="store_true",
+                    help="only run the batched mcsolve launches the trajectory counters are read on")
     args = ap.parse_args()
+    if args.mc_probe:
+        mc_probe()
+        return
     if args.pmc:
         pmc_workload()
         return

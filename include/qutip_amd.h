@@ -228,6 +228,16 @@ int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, co
                        const double *a, const double *b, const double *c, const double *e,
                        const double *bi, double rtol, double atol, int nsteps, double first_step,
                        double min_step, double max_step, int interpolate);
+/* Whether the dense engine's states are Hermitian: the
+ * assume_hermitian_state switch of LindbladMatrixForm
+ * (lindblad_matrix_form.pyx:59-90, set from state.isherm by
+ * MESolver._prepare_state, mesolve.py:270-276).  On (and N % 32 == 0),
+ * each stage computes the tiles on and above the diagonal and writes each
+ * conjugate-transposed below it, so every RHS is exactly Hermitian, as the
+ * reference's A + A^dag (:175-189) is; off (the default), the full RHS.
+ * No effect on the other engines; get reports what the engine runs. */
+int qa_rk_set_hermitian(qa_rk_solver *s, int on);
+int qa_rk_get_hermitian(qa_rk_solver *s, int *on);
 int qa_rk_destroy(qa_rk_solver *s);
 
 /* out = A x with the engine's generator (device vectors; waits for the

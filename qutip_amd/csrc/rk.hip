@@ -527,6 +527,7 @@ struct DenseOp {
     int N, nc, prec;  // prec 0: FP32 MFMA, 1: bf16 x3 split, 2: bf16 x6 split
     int nsplit;       // waves per 16 x 16 tile (split K)
     int tiled;        // 1: 32 x 32 workgroup tiles staged through LDS (large N)
+    int herm;         // 1: Hermitian stage inputs, the upper tiles and their mirror
     const float2 *Hm, *Hp, *C, *Cd;  // N x N each; C / Cd: nc blocks
     float2 *T;                        // nc blocks of scratch
     float2 *Xf;                       // the stage input in complex64 (tiled path)
@@ -1028,6 +1029,301 @@ __global__ __launch_bounds__(256 * kGHalves) void rk_dense_stage_tiled_kernel(Ep
     if (P.epi == EPI_FINAL) {
         const double sum = block_sum(part, smem);
         if (threadIdx.x == 0) P.partials[blockIdx.y * gridDim.x + blockIdx.x] = sum;
+    }
+}
+
+// ---- Hermitian stage inputs: the upper tiles and their mirror -------------
+// For a Hermitian input X the RHS k = -i H_nh X + i X H_nh^dag + sum_c c X c^dag
+// is Hermitian (lindblad_matrix_form.pyx:175-189 forms A + A^dag for the same
+// reason, assume_hermitian_state), so only the tiles on and above the
+// diagonal are computed; each is also written conjugate-transposed below it.
+// Upper tile p = bj (bj + 1) / 2 + bi has block row bi <= block column bj.
+__device__ __forceinline__ void upper_tile(int p, int &bi, int &bj) {
+    int j = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+    while (j * (j + 1) / 2 > p) --j;
+    while ((j + 1) * (j + 2) / 2 <= p) ++j;
+    bj = j;
+    bi = p - j * (j + 1) / 2;
+}
+
+// Item q of the upper tile of edge E at (i0, j0): the element it writes
+// (column-major index) and the tile element it takes (src = row + E col,
+// conjugated or with its imaginary part dropped).  Items [0, E^2) are the
+// tile, [E^2, 2 E^2) the mirror tile at (j0, i0), whose element (r, c) is
+// conj(tile(c, r)).  A diagonal tile (i0 == j0) has only the first E^2
+// items: its strictly lower triangle mirrors the upper one and its diagonal
+// is real, so the block written is exactly Hermitian (as A + A^dag is).
+struct HermItem {
+    int64_t idx;
+    int src, conj, real;
+};
+
+template <int E>
+__device__ __forceinline__ HermItem herm_item(int N, int i0, int j0, int q) {
+    const int s = q % (E * E), r = s % E, c = s / E;
+    HermItem it;
+    if (q < E * E) {
+        it.idx = (int64_t)(j0 + c) * N + i0 + r;
+        const bool low = (i0 == j0) && r > c;
+        it.src = low ? c + E * r : r + E * c;
+        it.conj = low;
+        it.real = (i0 == j0) && r == c;
+    } else {
+        it.idx = (int64_t)(i0 + c) * N + j0 + r;
+        it.src = c + E * r;
+        it.conj = 1;
+        it.real = 0;
+    }
+    return it;
+}
+
+__device__ __forceinline__ double2 herm_value(float2 s, const HermItem &it) {
+    return make_double2((double)s.x, it.real ? 0.0 : (it.conj ? -(double)s.y : (double)s.y));
+}
+
+// A 32-wide K step of a 16 x 16 tile: lane (r, kq) holds k = kk + 8 kq + u,
+// u < 8 (a contiguous 64-byte piece of its B column), for the FP32 MFMAs
+// (eight 16x16x4 instructions per part) and the bf16 pieces alike.  B may be
+// the complex128 stage input (converted when the MFMAs use it, so the loads
+// stay in flight).
+template <typename TB>
+__device__ __forceinline__ void step_load(const float2 *__restrict__ A, const TB *__restrict__ B,
+                                          int N, int i0, int j0, int lane, int kk, float2 (&a)[8],
+                                          TB (&b)[8]) {
+    const int r = lane & 15, kq = lane >> 4;
+    const float2 *pa = A + (size_t)(kk + 8 * kq) * N + i0 + r;
+    const TB *pb = B + (size_t)(j0 + r) * N + kk + 8 * kq;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        a[u] = pa[(size_t)u * N];
+        b[u] = pb[u];
+    }
+}
+
+template <int NP, typename TB>
+__device__ __forceinline__ void step_mma(const float2 (&a)[8], const TB (&bv)[8], v4f &re,
+                                         v4f &im, v4f &re2, v4f &im2) {
+    float2 b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) b[u] = tof(bv[u]);
+    if constexpr (NP == 0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            re = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].x, re, 0, 0, 0);
+            re2 = __builtin_amdgcn_mfma_f32_16x16x4f32(-a[u].y, b[u].y, re2, 0, 0, 0);
+            im = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, b[u].y, im, 0, 0, 0);
+            im2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, b[u].x, im2, 0, 0, 0);
+        }
+    } else {
+        v8bf ar[NP], ai[NP], nai[NP], br[NP], bi[NP];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            split_bf16<NP>(a[u].x, u, ar);
+            split_bf16<NP>(a[u].y, u, ai);
+            split_bf16<NP>(b[u].x, u, br);
+            split_bf16<NP>(b[u].y, u, bi);
+        }
+        negate_bf16<NP>(ai, nai);
+        re = mma_pieces<NP>(ar, br, re);
+        re2 = mma_pieces<NP>(nai, bi, re2);
+        im = mma_pieces<NP>(ar, bi, im);
+        im2 = mma_pieces<NP>(ai, br, im2);
+    }
+}
+
+template <typename TB>
+__device__ __forceinline__ void step_mma_prec(int prec, const float2 (&a)[8], const TB (&b)[8],
+                                              v4f &re, v4f &im, v4f &re2, v4f &im2) {
+    if (prec == 1)
+        step_mma<2>(a, b, re, im, re2, im2);
+    else if (prec == 2)
+        step_mma<3>(a, b, re, im, re2, im2);
+    else
+        step_mma<0>(a, b, re, im, re2, im2);
+}
+
+// Step s of the stage products of a wave (nb steps per product): Hm Xf,
+// Xf Hp, then T_c Cd_c.
+__device__ __forceinline__ void herm_step_load(const DenseOp &D, int nb, int s, int i0, int j0,
+                                               int lane, int k0, float2 (&a)[8], float2 (&b)[8]) {
+    const float2 *A, *B;
+    gt_operands(D, s / nb, A, B);
+    step_load(A, B, D.N, i0, j0, lane, k0 + 32 * (s % nb), a, b);
+}
+
+// Waves per 16 x 16 tile of the Hermitian kernels: K split so that every
+// wave keeps whole 32-wide steps (N % 32 == 0).
+__host__ __device__ constexpr int herm_split(int N) {
+    return N % 256 == 0 ? 8 : N % 128 == 0 ? 4 : N % 64 == 0 ? 2 : 1;
+}
+
+// The wave partial tiles of a workgroup, summed in wave order (the same bits
+// for a tile element wherever it is read).
+__device__ __forceinline__ void herm_partial_store(float2 (*part)[256], int w, int lane,
+                                                   const v4f &re, const v4f &im) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        part[w][4 * (lane >> 4) + q + 16 * (lane & 15)] = make_float2(re[q], im[q]);
+}
+
+__device__ __forceinline__ float2 herm_partial_sum(const float2 (*part)[256], int ns, int e) {
+    float2 t = part[0][e];
+    for (int u = 1; u < ns; ++u) {
+        t.x += part[u][e].x;
+        t.y += part[u][e].y;
+    }
+    return t;
+}
+
+// T_c = C_c X and Xf = complex64(X) for Hermitian stages (grid N/16, N/16,
+// max(nc, 1); herm_split(N) waves per tile).
+__global__ __launch_bounds__(64 * kMaxSplit) void rk_dense_herm_t_kernel(EpiArgs P, DenseOp D) {
+    __shared__ float2 part[kMaxSplit][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, ns = blockDim.x >> 6;
+    const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16, c = blockIdx.z;
+    const bool prod = c < D.nc;  // uniform: nc == 0 launches one layer for Xf
+    const size_t NN = (size_t)D.N * D.N;
+    int k0, k1;
+    k_range(D.N, ns, w, k0, k1);
+    const int nb = (k1 - k0) / 32;
+    const float2 *Cc = D.C + (prod ? c : 0) * NN;
+    // the tile's element e: row e % 16, column e / 16
+    auto tile_idx = [&](int e) { return (size_t)(j0 + e / 16) * D.N + i0 + e % 16; };
+    if (P.ctl->phase != P.run_phase) return;
+    const StageVecs v = stage_vecs(P, P.ctl);
+    const double2 *X = v.xin;
+    if (c == 0) {
+        for (int e = threadIdx.x; e < 256; e += blockDim.x) D.Xf[tile_idx(e)] = tof(X[tile_idx(e)]);
+    }
+    if (!prod) return;
+    float2 a[8];
+    double2 b[8];
+    step_load(Cc, X, D.N, i0, j0, lane, k0, a, b);
+    v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
+    v4f re2 = {0.f, 0.f, 0.f, 0.f}, im2 = {0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < nb; ++s) {
+        float2 an[8];
+        double2 bn[8];
+        step_load(Cc, X, D.N, i0, j0, lane, k0 + 32 * min(s + 1, nb - 1), an, bn);
+        step_mma_prec(D.prec, a, b, re, im, re2, im2);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a[u] = an[u];
+            b[u] = bn[u];
+        }
+    }
+    re += re2;
+    im += im2;
+    herm_partial_store(part, w, lane, re, im);
+    __syncthreads();
+    for (int e = threadIdx.x; e < 256; e += blockDim.x)
+        D.T[c * NN + tile_idx(e)] = herm_partial_sum(part, ns, e);
+}
+
+// k on the upper 16 x 16 tiles (grid: one workgroup per upper tile, 8 / IPT
+// waves); IPT items per thread, so every thread takes part in the epilogue.
+// The first two product steps and the epilogue's loads are requested
+// together, then each later step two steps ahead.
+template <int IPT>
+__global__ __launch_bounds__(512 / IPT) void rk_dense_herm_stage_kernel(EpiArgs P, DenseOp D) {
+    __shared__ float2 part[kMaxSplit][256];
+    __shared__ double smem[kMaxSplit];
+    int bi, bj;
+    upper_tile(blockIdx.x, bi, bj);
+    const int i0 = 16 * bi, j0 = 16 * bj;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, ns = blockDim.x >> 6;
+    int k0, k1;
+    k_range(D.N, ns, w, k0, k1);
+    const int nb = (k1 - k0) / 32, total = (2 + D.nc) * nb;
+    RKCtl *ctl = P.ctl;
+    if (ctl->phase != P.run_phase) return;
+    float2 a0[8], b0[8], a1[8], b1[8];
+    herm_step_load(D, nb, 0, i0, j0, lane, k0, a0, b0);
+    herm_step_load(D, nb, min(1, total - 1), i0, j0, lane, k0, a1, b1);
+    const StageVecs v = stage_vecs(P, ctl);
+    const int nitems = bi == bj ? 256 : 512;
+    HermItem it[IPT];
+    double2 o[IPT], er[IPT];
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+        const int item = threadIdx.x + q * blockDim.x;
+        it[q] = herm_item<16>(D.N, i0, j0, item);
+        if (item < nitems) epi_pre(P, v, it[q].idx, o[q], er[q]);
+    }
+    v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
+    v4f re2 = {0.f, 0.f, 0.f, 0.f}, im2 = {0.f, 0.f, 0.f, 0.f};
+    for (int s = 0; s < total; ++s) {
+        float2 a2[8], b2[8];
+        herm_step_load(D, nb, min(s + 2, total - 1), i0, j0, lane, k0, a2, b2);
+        step_mma_prec(D.prec, a0, b0, re, im, re2, im2);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a0[u] = a1[u];
+            b0[u] = b1[u];
+            a1[u] = a2[u];
+            b1[u] = b2[u];
+        }
+    }
+    re += re2;
+    im += im2;
+    herm_partial_store(part, w, lane, re, im);
+    __syncthreads();
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+        const int item = threadIdx.x + q * blockDim.x;
+        if (item >= nitems) continue;
+        const float2 t = herm_partial_sum(part, ns, it[q].src);
+        sum += epi_post(P, ctl, v, it[q].idx, herm_value(t, it[q]), o[q], er[q]);
+    }
+    if (P.epi == EPI_FINAL) {
+        sum = block_sum(sum, smem);
+        if (threadIdx.x == 0) P.partials[blockIdx.x] = sum;
+    }
+}
+
+// The same on the upper 32 x 32 workgroup tiles of the LDS-staged path
+// (N >= kTiledN): 2 E^2 items over 512 threads.
+__global__ __launch_bounds__(256 * kGHalves) void rk_dense_herm_stage_tiled_kernel(EpiArgs P,
+                                                                                  DenseOp D) {
+    constexpr int IPT = 2 * kGT * kGT / (256 * kGHalves);
+    __shared__ GemmShared G;
+    __shared__ double smem[256 * kGHalves / QA_WAVE];
+    RKCtl *ctl = P.ctl;
+    if (ctl->phase != P.run_phase) return;
+    const StageVecs v = stage_vecs(P, ctl);
+    int bi, bj;
+    upper_tile(blockIdx.x, bi, bj);
+    const int i0 = kGT * bi, j0 = kGT * bj;
+    const int nitems = bi == bj ? kGT * kGT : 2 * kGT * kGT;
+    v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
+    gt_run<float2>(D, G, 2 + D.nc, nullptr, (const float2 *)nullptr, i0, j0, re, im);
+    // the tile is complete in half 0; it goes through LDS to the items
+    __syncthreads();
+    float2 *tile = (float2 *)&G.L[0];
+    const int h = threadIdx.x >> 8, lane = threadIdx.x & 63, w = (threadIdx.x & 255) >> 6;
+    if (h == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            tile[16 * (w & 1) + 4 * (lane >> 4) + q + kGT * (16 * (w >> 1) + (lane & 15))] =
+                make_float2(re[q], im[q]);
+    }
+    __syncthreads();
+    // the epilogue's loads follow the products here: issued before them,
+    // four items' K terms would not fit the registers beside the LDS pipeline
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+        const int item = threadIdx.x + q * blockDim.x;
+        if (item >= nitems) continue;
+        const HermItem it = herm_item<kGT>(D.N, i0, j0, item);
+        double2 o, er;
+        epi_pre(P, v, it.idx, o, er);
+        sum += epi_post(P, ctl, v, it.idx, herm_value(tile[it.src], it), o, er);
+    }
+    if (P.epi == EPI_FINAL) {
+        sum = block_sum(sum, smem);
+        if (threadIdx.x == 0) P.partials[blockIdx.x] = sum;
     }
 }
 
@@ -1538,9 +1834,49 @@ int launch_sparse_stage(qa_rk_solver *S, int i) {
     return QA_OK;
 }
 
+// Hermitian stage inputs: the T layer (which also writes Xf), then the
+// upper-tile stage kernel.
+int launch_dense_herm_stage(qa_rk_solver *S, const EpiArgs &P) {
+    const int N = S->dop.N, nz = std::max(S->dop.nc, 1);
+    if (S->dop.tiled) {
+        const int nt = N / kGT;
+        hipLaunchKernelGGL(rk_dense_t_tiled_kernel, dim3(nt, nt, nz), dim3(256 * kGHalves), 0,
+                           S->stream, P, S->dop);
+        QA_CHECK_LAUNCH();
+        hipLaunchKernelGGL(rk_dense_herm_stage_tiled_kernel, dim3(nt * (nt + 1) / 2),
+                           dim3(256 * kGHalves), 0, S->stream, P, S->dop);
+        QA_CHECK_LAUNCH();
+        return QA_OK;
+    }
+    const int nt = N / 16;
+    const int ns = herm_split(N);
+    hipLaunchKernelGGL(rk_dense_herm_t_kernel, dim3(nt, nt, nz), dim3(64 * ns), 0, S->stream, P,
+                       S->dop);
+    QA_CHECK_LAUNCH();
+    const dim3 grid(nt * (nt + 1) / 2), block(64 * ns);
+    switch (ns) {
+        case 8: hipLaunchKernelGGL(rk_dense_herm_stage_kernel<1>, grid, block, 0, S->stream, P, S->dop); break;
+        case 4: hipLaunchKernelGGL(rk_dense_herm_stage_kernel<2>, grid, block, 0, S->stream, P, S->dop); break;
+        case 2: hipLaunchKernelGGL(rk_dense_herm_stage_kernel<4>, grid, block, 0, S->stream, P, S->dop); break;
+        default: hipLaunchKernelGGL(rk_dense_herm_stage_kernel<8>, grid, block, 0, S->stream, P, S->dop); break;
+    }
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
+// Workgroups of the last stage kernel, i.e. the error partials per attempt.
+int stage_parts(const qa_rk_solver *S) {
+    if (S->kind == 1 && S->dop.herm) {
+        const int nt = S->dop.N / (S->dop.tiled ? kGT : 16);
+        return nt * (nt + 1) / 2;
+    }
+    return S->nparts;
+}
+
 int launch_dense_stage(qa_rk_solver *S, int i) {
     EpiArgs P;
     fill_epi(P, S, i);
+    if (S->dop.herm) return launch_dense_herm_stage(S, P);
     if (S->dop.tiled) {
         const int nt = S->dop.N / kGT;
         hipLaunchKernelGGL(rk_dense_t_tiled_kernel, dim3(nt, nt, std::max(S->dop.nc, 1)),
@@ -1663,7 +1999,7 @@ int enqueue_iteration(qa_rk_solver *S) {
         if (rc) return rc;
     }
     hipLaunchKernelGGL(rk_control_kernel, dim3(1), dim3(256), 0, S->stream, S->ctl, S->partials,
-                       S->nparts, S->n);
+                       stage_parts(S), S->n);
     QA_CHECK_LAUNCH();
     if (S->interpolate && S->rk_extra > S->rk_step) {
         if (!S->first_dense_from_front) {
@@ -2276,6 +2612,23 @@ int qa_functionals_check(int n, int nfun, const int *kinds, const int *fptr, con
                            "functional entry %d: row %d out of range", p, rows[p]);
         }
     }
+    return QA_OK;
+}
+
+int qa_rk_set_hermitian(qa_rk_solver *S, int on) {
+    QA_REQUIRE(S != nullptr, "qa_rk_set_hermitian: null solver");
+    // the upper-tile kernels split K in 32-wide steps
+    const int herm = (on && S->kind == 1 && S->dop.N % 32 == 0) ? 1 : 0;
+    if (herm != S->dop.herm) {
+        S->dop.herm = herm;
+        S->graph_dirty = true;
+    }
+    return QA_OK;
+}
+
+int qa_rk_get_hermitian(qa_rk_solver *S, int *on) {
+    QA_REQUIRE(S != nullptr && on != nullptr, "qa_rk_get_hermitian: null argument");
+    *on = S->kind == 1 ? S->dop.herm : 0;
     return QA_OK;
 }
 

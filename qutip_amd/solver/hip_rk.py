@@ -246,6 +246,17 @@ class RKEngine:
                   _ip(rows), _ip(cols), _dp(w))
         self.nfun = len(kinds)
 
+    def set_hermitian(self, on):
+        """Hermitian states on the dense engine (assume_hermitian_state):
+        the upper tiles and their mirror; no effect on the other engines."""
+        _lib.call("qa_rk_set_hermitian", self.handle, int(bool(on)))
+
+    @property
+    def hermitian(self):
+        h = C.c_int(0)
+        _lib.call("qa_rk_get_hermitian", self.handle, C.byref(h))
+        return bool(h.value)
+
     def set_state(self, t0, y0):
         y0 = y0 if isinstance(y0, _data.HipDense) else _data.HipDense.from_array(y0)
         if y0.size != self.n:
@@ -591,6 +602,8 @@ class IntegratorHipRK(Integrator):
         self._block = None  # (m, engine) for a block of m state columns
         owner = getattr(self.derivative, "__self__", None)
         dense = matrix_form_operators(owner) if o["fused"] else None
+        # the matrix form's assume_hermitian_state is read at every set_state
+        self._mf_owner = owner if dense is not None else None
         op = constant_operator(self.derivative) if (o["fused"] and dense is None) else None
         td = (time_dependent_operator(self.derivative)
               if (o["fused"] and dense is None and op is None
@@ -704,6 +717,11 @@ class IntegratorHipRK(Integrator):
             # step-wise call needs it
             self._engine_pending = True
         elif self._engine is not None:
+            if self._mf_owner is not None:
+                # MESolver._prepare_state sets it from state.isherm
+                # (mesolve.py:270-276); the reference's default is True
+                self._engine.set_hermitian(
+                    getattr(self._mf_owner, "assume_hermitian_state", True))
             self._engine.set_state(t, y.reshape((y.size, 1)))
         else:
             self._plain.set_initial_value(y.copy(), t)

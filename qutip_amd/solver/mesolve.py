@@ -53,6 +53,19 @@ class MESolver(Solver):
         super().__init__(rhs, options=options)
 
 
+    def _prepare_state(self, state):
+        # the matrix form's fast path assumes a Hermitian rho (A + A^dag);
+        # an operator state that is not Hermitian switches it off
+        # (mesolve.py:270-276).  A ket becomes ket2dm(psi), which is
+        # Hermitian, so it switches the fast path back on (the reference
+        # leaves the switch as the last operator state set it).
+        if self._matrix_form and state.isoper:
+            self.rhs.assume_hermitian_state = state.isherm
+        elif self._matrix_form and state.isket:
+            self.rhs.assume_hermitian_state = True
+        return super()._prepare_state(state)
+
+
 MESolver.add_integrator(IntegratorHipRK, "hip_rk")
 
 

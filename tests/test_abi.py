@@ -190,3 +190,14 @@ def test_coefficient_programs_pass_the_abi_checks():
     assert _coef_check(bad) == _lib.QA_ERR_ARG
     bad = [qc.DeviceProgram([qc.CO_INTERP], [complex(50, 2), 0j])]  # table past the end
     assert _coef_check(bad) == _lib.QA_ERR_ARG
+
+
+def test_hermitian_switch_rejects_null_solver():
+    """qa_rk_set_hermitian / qa_rk_get_hermitian check their arguments
+    before touching a solver (no device call)."""
+    L = _lib.load()
+    assert L.qa_rk_set_hermitian(None, 1) == _lib.QA_ERR_ARG
+    assert b"qa_rk_set_hermitian" in L.qa_last_error()
+    on = ctypes.c_int(7)
+    assert L.qa_rk_get_hermitian(None, ctypes.byref(on)) == _lib.QA_ERR_ARG
+    assert on.value == 7
