@@ -700,6 +700,26 @@ def main():
     out["fp32_operator"] = {"solves_per_s": round(1 / tf32, 4), "err_vs_fp64": float(ferr),
                             "rk_attempts": fsolver._integrator.stats()["attempts"]}
     log(f"fp32 operator: {1 / tf32:.2f} solves/s, err {ferr:.2e}")
+    # the same engine with every iteration's kernels launched directly
+    # instead of replaying the captured graph (the form the kernel traces
+    # are taken in): the same kernels, so the same result bits
+    os.environ["QUTIP_AMD_GRAPH"] = "0"
+    try:
+        dsolver = qa.MESolver(s["H"], s["c_ops"], options=opts)
+    finally:
+        del os.environ["QUTIP_AMD_GRAPH"]
+    dres = dsolver.run(s["psi0"], tlist, e_ops=s["e_ops"])
+    _lib.call("qa_device_sync")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dres = dsolver.run(s["psi0"], tlist, e_ops=s["e_ops"])
+    _lib.call("qa_device_sync")
+    tdir = (time.perf_counter() - t0) / args.steps
+    dsame = all(np.array_equal(np.asarray(dres.expect[k]), np.asarray(res.expect[k]))
+                for k in range(2))
+    out["direct_launch"] = {"solves_per_s": round(1 / tdir, 4), "same_bits_as_graph": dsame,
+                            "graph_launches": dsolver._integrator.stats()["graph_launches"]}
+    log(f"direct launches: {1 / tdir:.2f} solves/s, same bits as the graph replay: {dsame}")
     out["jc_mesolve"] = bench_jc(qa, systems, log, cpu=not args.no_cpu)
     if not args.no_mc:
         out["mcsolve"] = bench_mcsolve(qa, systems, log, cpu=not args.no_cpu)
