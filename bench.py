@@ -131,7 +131,7 @@ def bench_mcsolve(qa, systems, log, ntraj=2048, reps=3, cpu=True):
         # RHS per attempt (the vern7 step stages; dense-output stages add more)
         rhs = 10 * float(info[:, 3].sum())
         d = s["H"].shape[0]
-        alg = rhs * 2 * d * (vb) + rhs / ntraj * (s["H"].nnz * 20 + (d + 1) * 4)
+        alg = rhs * 2 * d * vb + rhs / ntraj * (solver.H_eff.nnz * 20 + (d + 1) * 4)
         achieved = alg / best / 1e9
         out[prec] = {"traj_per_s": round(ntraj / best, 1), "s_per_batch": round(best, 4),
                      "all_ok": ok, "jumps": int(info[:, 1].sum()),
@@ -153,6 +153,22 @@ def bench_mcsolve(qa, systems, log, ntraj=2048, reps=3, cpu=True):
                                              "launch time is the host-timed batch"}}
         log(f"mcsolve {prec}: {ntraj / best:.0f} traj/s ({best * 1e3:.1f} ms), worst "
             f"{worst_sigma:.2f} sigma, jumps {int(info[:, 1].sum())}")
+    # the plain drop-in form: qutip's trajectory loop through the hip_rk
+    # integrator and device data ops (MCSolver.run_plain), on the first
+    # trajectories of the same draws
+    solver = qa.MCSolver(s["H"], s["c_ops"], options={"precision": "fp64"})
+    nplain = 4
+    t0 = time.perf_counter()
+    runs, jumps = solver.run_plain(psi0, tlist, s["e_ops"], draws[:nplain])
+    tpl = time.perf_counter() - t0
+    ex, info, _, _ = solver.run_batch(psi0, tlist, s["e_ops"], draws[:nplain])
+    out["plain_form"] = {"traj_per_s": round(nplain / tpl, 3), "trajectories": nplain,
+                         "same_jumps_as_batched": bool((jumps == info[:, 1]).all()),
+                         "max_abs_diff_vs_batched": float(np.abs(runs - ex).max()),
+                         "batched_fp64_over_plain": round(out["fp64"]["traj_per_s"] * tpl / nplain,
+                                                          1)}
+    log(f"mcsolve plain form: {nplain / tpl:.2f} traj/s, same jumps {out['plain_form']['same_jumps_as_batched']}, "
+        f"max diff {out['plain_form']['max_abs_diff_vs_batched']:.1e}")
     if cpu:
         out["cpu_baseline"] = mc_cpu_baseline()
         log(f"mcsolve CPU baseline: {out['cpu_baseline']}")

@@ -13,7 +13,8 @@ overlaps on the device:
   products; dense collapse operators make the superoperator infeasible:
   d^4 non-zeros);
 * mcsolve: one persistent trajectory kernel for all trajectories of the
-  request.
+  request (complex64 states with FP64 norms unless the request asks for
+  ``precision='fp64'``).
 
 The host loop never waits on one request: it polls each request's progress
 (qa_rk_advance / an event after the trajectory kernel), tops up its queued
@@ -157,9 +158,11 @@ class _Running:
             _lib.call("qa_rk_begin", self.eng.handle, len(times),
                       times.ctypes.data_as(C.POINTER(C.c_double)), 0, None)
         else:
+            # the chosen precision form: complex64 states and products,
+            # FP64 norms, collapse search and expectation values
             self.solver = MCSolver(req.H, req.c_ops, options={
                 "rtol": o["rtol"], "atol": o["atol"], "nsteps": o["nsteps"],
-                "precision": o.get("precision", "fp64")})
+                "precision": o.get("precision", "fp32")})
             seeds = _read_seeds(req.seed, req.ntraj)
             self._draws = draws_for(seeds, 64)
             self.stream = C.c_void_p()

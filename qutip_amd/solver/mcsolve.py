@@ -383,6 +383,106 @@ class MCSolver:
         _lib.call("qa_device_sync")
         return self.collect_async(H, with_jumps=True)
 
+    def run_plain(self, psi0, tlist, e_ops, draws):
+        """The plain drop-in form of the trajectory loop: what qutip's
+        MCSolver does with method='hip_rk' (MCIntegrator, mcsolve.py:245-406,
+        over the Integrator protocol).  Trajectories run one after another;
+        every step is an IntegratorHipRK.mcstep on the fused engine of
+        H_eff, and the norms, the collapse-time search (log-ratio secant,
+        :321-369) and the jumps (:371-406) run on device data through the
+        data layer, with host round trips in between.  ``draws``: ntraj x
+        max_draws uniform numbers, consumed in the reference's order (the
+        numbers the batched kernel reads).  Returns (runs_expect [ntraj,
+        ntimes, n_e_ops], jumps [ntraj])."""
+        from .hip_rk import IntegratorHipRK
+        o = self.options
+        integ = IntegratorHipRK(QobjEvo(Qobj(self.H_eff)).matmul_data, {
+            k: o[k] for k in ("rtol", "atol", "nsteps", "first_step", "min_step", "max_step",
+                              "interpolate")} | {"tableau": o["method"]})
+        cs = [_data.HipCSR(c.to_csr()) for c in self.c_ops]
+        ns = [_data.HipCSR(n) for n in self._n_ops]
+        eo = [_data.HipCSR(O.to_csr()) for O in e_ops]
+        tlist = np.asarray(tlist, dtype=np.float64)
+        draws = np.asarray(draws, dtype=np.float64)
+        ntraj = draws.shape[0]
+        runs = np.zeros((ntraj, len(tlist), len(eo)), dtype=np.complex128)
+        jumps = np.zeros(ntraj, dtype=np.int64)
+        prob = lambda y: _data.frobenius(y) ** 2
+        p0 = _data.HipDense.from_array(np.asarray(psi0, dtype=np.complex128).reshape(-1, 1))
+        for k in range(ntraj):
+            it = iter(draws[k])
+            target = next(it)
+            integ.set_state(tlist[0], p0)
+            nrm0 = np.sqrt(prob(p0))
+            runs[k, 0] = [_data.expect(O, p0) / nrm0 ** 2 for O in eo]
+            for i, t in enumerate(tlist[1:], 1):
+                t_old, y_old = integ.get_state(copy=True)
+                norm_old = prob(y_old)
+                while t_old < t:
+                    t_step, state = integ.mcstep(t, copy=True)
+                    nrm = prob(state)
+                    if nrm <= target:
+                        t_col, state = self._plain_collapse_time(integ, prob, target, norm_old,
+                                                                 nrm, t_old, t_step)
+                        target = self._plain_collapse(integ, cs, ns, t_col, state, it, target,
+                                                      jumps, k)
+                        t_old, y_old = integ.get_state(copy=True)
+                        norm_old = 1.0
+                    else:
+                        t_old, y_old, norm_old = t_step, state, nrm
+                n2 = prob(y_old)
+                runs[k, i] = [_data.expect(O, y_old) / n2 for O in eo]
+        return runs, jumps
+
+    def _plain_collapse_time(self, integ, prob, target, norm_old, norm, t_prev, t_final):
+        """_find_collapse_time (mcsolve.py:321-369) through mcstep."""
+        o = self.options
+        tries = 0
+        cutoff = o["norm_min_step"]
+        while tries < o["norm_steps"]:
+            tries += 1
+            if (t_final - t_prev) < o["norm_t_tol"]:
+                t_guess = t_final
+                _, state = integ.get_state(copy=True)
+                break
+            ratio = np.log(norm_old / target) / np.log(norm_old / norm)
+            ratio = min(max(ratio, cutoff), 1 - cutoff)
+            t_guess = t_prev + (t_final - t_prev) * ratio
+            if (t_guess - t_prev) < o["norm_t_tol"]:
+                t_guess = t_prev + o["norm_t_tol"]
+            _, state = integ.mcstep(t_guess, copy=True)
+            n_guess = prob(state)
+            if abs(target - n_guess) < o["norm_tol"] * target:
+                break
+            if n_guess < target:
+                t_final, norm = t_guess, n_guess
+            else:
+                t_prev, norm_old = t_guess, n_guess
+        if tries >= o["norm_steps"]:
+            raise RuntimeError(MC_STATUS[-10])
+        return t_guess, state
+
+    def _plain_collapse(self, integ, cs, ns, t, state, draws, target, jumps, k):
+        """_do_collapse (mcsolve.py:371-406): the channel by <c^dag c>, the
+        jump, a new target norm; returns the target."""
+        which = 0
+        if len(ns) > 1:
+            probs = [_data.inner(state, _data.matmul(n, state)).real for n in ns]
+            left = sum(probs) * next(draws) - probs[0]
+            while left > 0 and which < len(ns) - 1:
+                which += 1
+                left -= probs[which]
+        new = _data.matmul(cs[which], state)
+        nn = _data.frobenius(new)
+        if nn < self.options["mc_corr_eps"]:
+            new = _data.mul(state, 1.0 / _data.frobenius(state))
+        else:
+            new = _data.mul(new, 1.0 / nn)
+            jumps[k] += 1
+            target = next(draws)
+        integ.set_state(t, new)
+        return target
+
     def run(self, state, tlist, ntraj=1, *, args=None, e_ops=None, timeout=None,
             target_tol=None, seeds=None):
         t0 = time.time()
