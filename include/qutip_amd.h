@@ -95,6 +95,15 @@ int qa_sell_matvec_z(int nrows, int lanes, int nslices, const int *slice_ptr, co
                      const void *vals, int vals_fp32, const void *x, void *out, double scale_re,
                      double scale_im, int accumulate, void *stream);
 
+/* SELL-C times nvec dense columns (column-major, leading dims ldx >= ncols,
+ * ldo >= nrows): every stored entry is read once for each group of 8
+ * columns.  The multi-column case of matmul_csr_dense_dense
+ * (matmul.pyx:255-265) over the layout of qa_sell_matvec_z. */
+int qa_sell_matmat_z(int nrows, int ncols, int lanes, int nslices, const int *slice_ptr,
+                     const int *col, const void *vals, int vals_fp32, int nvec, const void *x,
+                     int64_t ldx, void *out, int64_t ldo, double scale_re, double scale_im,
+                     int accumulate, void *stream);
+
 /* The same product with a coded SELL-C operator: every stored entry is a
  * 16-bit code, low `vbits` bits (6..10) = index into `vtab` (2^vbits
  * complex128 values), high 16 - vbits bits = index into `otab`

@@ -51,6 +51,8 @@ SIGNATURES = {
     "qa_event_query": [vp, P_i32],
     "qa_csr_matvec_z": [i32, i32, vp, vp, vp, i32, i32, vp, vp, f64, f64, i32, vp],
     "qa_csr_matmat_z": [i32, i32, vp, vp, vp, i32, i32, vp, i64, vp, i64, f64, f64, i32, vp],
+    "qa_sell_matmat_z": [i32, i32, i32, i32, vp, vp, vp, i32, i32, vp, i64, vp, i64, f64, f64, i32,
+                         vp],
     "qa_sell_matvec_z": [i32, i32, i32, vp, vp, vp, i32, vp, vp, f64, f64, i32, vp],
     "qa_sell_coded_matvec_z": [i32, i32, i32, i32, vp, vp, vp, vp, i32, vp, vp, f64, f64, i32,
                                vp],

@@ -44,9 +44,14 @@ __global__ __launch_bounds__(256) void csr_matvec_kernel(int nrows, const int *_
     }
 }
 
+// Columns of the dense block one thread or wavefront applies each loaded
+// operator entry to (the operator is read once per kMatCols columns).
+constexpr int kMatCols = 8;
+
 // CSR times a block of nvec dense columns (column-major, leading dimensions
-// ldx / ldo): one thread per (row, column).  Replaces the per-column loop of
-// matmul_csr_dense_dense (qutip/core/data/matmul.pyx:255-265).
+// ldx / ldo): one thread per row and kMatCols columns (grid y), so each
+// stored entry is read once for kMatCols columns.  Replaces the per-column
+// loop of matmul_csr_dense_dense (qutip/core/data/matmul.pyx:255-265).
 template <typename V>
 __global__ __launch_bounds__(256) void csr_matmat_kernel(int nrows, const int *__restrict__ rowptr,
                                                          const int *__restrict__ col,
@@ -55,15 +60,93 @@ __global__ __launch_bounds__(256) void csr_matmat_kernel(int nrows, const int *_
                                                          double2 *__restrict__ out, int64_t ldo,
                                                          double2 scale, int accumulate) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int v = blockIdx.y;
-    if (row >= nrows || v >= nvec) return;
-    const double2 *xv = x + (int64_t)v * ldx;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int p = rowptr[row]; p < rowptr[row + 1]; ++p) cfma(acc, ld(vals, p), xv[col[p]]);
-    double2 r = cmul(scale, acc);
-    double2 *o = out + (int64_t)v * ldo + row;
-    if (accumulate) r = cadd(r, *o);
-    *o = r;
+    const int v0 = blockIdx.y * kMatCols;
+    const int nv = min(kMatCols, nvec - v0);  // uniform over the grid layer
+    if (row >= nrows) return;
+    const double2 *xb = x + (int64_t)v0 * ldx;
+    double2 acc[kMatCols];
+#pragma unroll
+    for (int v = 0; v < kMatCols; ++v) acc[v] = make_double2(0.0, 0.0);
+    for (int p = rowptr[row]; p < rowptr[row + 1]; ++p) {
+        const int c = col[p];
+        const double2 a = ld(vals, p);
+#pragma unroll
+        for (int v = 0; v < kMatCols; ++v)
+            if (v < nv) cfma(acc[v], a, xb[(int64_t)v * ldx + c]);
+    }
+#pragma unroll
+    for (int v = 0; v < kMatCols; ++v) {
+        if (v >= nv) break;
+        double2 r = cmul(scale, acc[v]);
+        double2 *o = out + (int64_t)(v0 + v) * ldo + row;
+        if (accumulate) r = cadd(r, *o);
+        *o = r;
+    }
+}
+
+// SELL-C times a block of dense columns: one wavefront per slice (the lane
+// -> entry assignment of sell_row_dot) and kMatCols columns per grid layer.
+// Each lane loads its entries (column and value) once, four steps at a
+// time, and applies them to every column of the layer; the L partial sums
+// of a row are combined with lane shuffles.  Grid: (ceil(nslices 64 / 256),
+// ceil(nvec / kMatCols)).
+template <typename V, int L>
+__global__ __launch_bounds__(256) void sell_matmat_kernel(int nrows, int nslices,
+                                                          const int *__restrict__ slice_ptr,
+                                                          const int *__restrict__ col,
+                                                          const V *__restrict__ vals, int nvec,
+                                                          const double2 *__restrict__ x,
+                                                          int64_t ldx, double2 *__restrict__ out,
+                                                          int64_t ldo, double2 scale,
+                                                          int accumulate) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int slice = (int)(t / QA_WAVE), lane = (int)(t % QA_WAVE);
+    if (slice >= nslices) return;
+    const int v0 = blockIdx.y * kMatCols;
+    const int nv = min(kMatCols, nvec - v0);  // uniform over the grid layer
+    int base, steps;
+    sell_slice(slice_ptr, slice, base, steps);
+    const double2 *xb = x + (int64_t)v0 * ldx;
+    double2 acc[kMatCols];
+#pragma unroll
+    for (int v = 0; v < kMatCols; ++v) acc[v] = make_double2(0.0, 0.0);
+    const int p0 = base + lane;
+    for (int k = 0; k < steps; k += 4) {
+        const int m = min(4, steps - k);  // uniform over the slice
+        int c[4];
+        double2 a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = p0 + (k + min(u, m - 1)) * QA_WAVE;
+            c[u] = col[p];
+            a[u] = ld(vals, p);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u >= m) break;
+#pragma unroll
+            for (int v = 0; v < kMatCols; ++v)
+                if (v < nv) cfma(acc[v], a[u], xb[(int64_t)v * ldx + c[u]]);
+        }
+    }
+    const int64_t r = (int64_t)slice * (QA_WAVE / L) + lane / L;
+#pragma unroll
+    for (int v = 0; v < kMatCols; ++v) {
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) {
+            acc[v].x += __shfl_xor(acc[v].x, off, QA_WAVE);
+            acc[v].y += __shfl_xor(acc[v].y, off, QA_WAVE);
+        }
+    }
+    if (lane % L != 0 || r >= nrows) return;
+#pragma unroll
+    for (int v = 0; v < kMatCols; ++v) {
+        if (v >= nv) break;
+        double2 o = cmul(scale, acc[v]);
+        double2 *dst = out + (int64_t)(v0 + v) * ldo + r;
+        if (accumulate) o = cadd(o, *dst);
+        *dst = o;
+    }
 }
 
 template <typename V, int L>
@@ -386,6 +469,33 @@ static int launch_sell(int lanes, unsigned blocks, hipStream_t s, int nrows, int
     return QA_OK;
 }
 
+template <typename V>
+int launch_sell_matmat(int lanes, dim3 grid, hipStream_t s, int nrows, int nslices,
+                       const int *slice_ptr, const int *col, const V *vals, int nvec,
+                       const double2 *x, int64_t ldx, double2 *out, int64_t ldo, double2 sc,
+                       int accumulate) {
+    switch (lanes) {
+        case 1:
+            hipLaunchKernelGGL((sell_matmat_kernel<V, 1>), grid, dim3(256), 0, s, nrows, nslices,
+                               slice_ptr, col, vals, nvec, x, ldx, out, ldo, sc, accumulate);
+            break;
+        case 2:
+            hipLaunchKernelGGL((sell_matmat_kernel<V, 2>), grid, dim3(256), 0, s, nrows, nslices,
+                               slice_ptr, col, vals, nvec, x, ldx, out, ldo, sc, accumulate);
+            break;
+        case 4:
+            hipLaunchKernelGGL((sell_matmat_kernel<V, 4>), grid, dim3(256), 0, s, nrows, nslices,
+                               slice_ptr, col, vals, nvec, x, ldx, out, ldo, sc, accumulate);
+            break;
+        default:
+            hipLaunchKernelGGL((sell_matmat_kernel<V, 8>), grid, dim3(256), 0, s, nrows, nslices,
+                               slice_ptr, col, vals, nvec, x, ldx, out, ldo, sc, accumulate);
+            break;
+    }
+    QA_CHECK_LAUNCH();
+    return QA_OK;
+}
+
 extern "C" {
 
 int qa_csr_matvec_z(int nrows, int ncols, const int *rowptr, const int *col, const void *vals,
@@ -411,7 +521,7 @@ int qa_csr_matmat_z(int nrows, int ncols, const int *rowptr, const int *col, con
     QA_REQUIRE(ldx >= ncols && ldo >= nrows, "qa_csr_matmat_z: leading dimension too small");
     QA_REQUIRE(nvec <= 65535, "qa_csr_matmat_z: at most 65535 columns per call");
     if (nrows == 0 || nvec == 0) return QA_OK;
-    dim3 grid((unsigned)((nrows + 255) / 256), (unsigned)nvec);
+    dim3 grid((unsigned)((nrows + 255) / 256), (unsigned)((nvec + kMatCols - 1) / kMatCols));
     const double2 sc = make_double2(scale_re, scale_im);
     hipStream_t s = (hipStream_t)stream;
     if (vals_fp32)
@@ -486,6 +596,32 @@ int qa_sell_matvec_z(int nrows, int lanes, int nslices, const int *slice_ptr, co
                            (const double2 *)x, (double2 *)out, sc, accumulate);
     return launch_sell(lanes, blocks, s, nrows, nslices, slice_ptr, col, (const double2 *)vals,
                        (const double2 *)x, (double2 *)out, sc, accumulate);
+}
+
+int qa_sell_matmat_z(int nrows, int ncols, int lanes, int nslices, const int *slice_ptr,
+                     const int *col, const void *vals, int vals_fp32, int nvec, const void *x,
+                     int64_t ldx, void *out, int64_t ldo, double scale_re, double scale_im,
+                     int accumulate, void *stream) {
+    QA_REQUIRE(lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8,
+               "qa_sell_matmat_z: lanes must be 1, 2, 4 or 8");
+    const int C = QA_WAVE / lanes;
+    QA_REQUIRE(nrows >= 0 && ncols >= 0 && nvec >= 0, "qa_sell_matmat_z: negative shape");
+    QA_REQUIRE(nslices == (nrows + C - 1) / C,
+               "qa_sell_matmat_z: nslices must be ceil(nrows / (64 / lanes))");
+    QA_REQUIRE(ldx >= ncols && ldo >= nrows, "qa_sell_matmat_z: leading dimension too small");
+    QA_REQUIRE(nvec <= 65535, "qa_sell_matmat_z: at most 65535 columns per call");
+    if (nrows == 0 || nvec == 0) return QA_OK;
+    const dim3 grid((unsigned)(((int64_t)nslices * QA_WAVE + 255) / 256),
+                    (unsigned)((nvec + kMatCols - 1) / kMatCols));
+    const double2 sc = make_double2(scale_re, scale_im);
+    hipStream_t s = (hipStream_t)stream;
+    if (vals_fp32)
+        return launch_sell_matmat(lanes, grid, s, nrows, nslices, slice_ptr, col,
+                                  (const float2 *)vals, nvec, (const double2 *)x, ldx,
+                                  (double2 *)out, ldo, sc, accumulate);
+    return launch_sell_matmat(lanes, grid, s, nrows, nslices, slice_ptr, col,
+                              (const double2 *)vals, nvec, (const double2 *)x, ldx, (double2 *)out,
+                              ldo, sc, accumulate);
 }
 
 int qa_lincomb_z(int64_t n, const void *base, int nterms, const void *const *vecs,

@@ -609,8 +609,12 @@ def matmul(left, right, scale=1, out=None):
                       _vp(left._col.ptr), _vp(left._vals.ptr), int(left.fp32), left.nnz,
                       _vp(right.ptr), _vp(out.ptr), sre, sim, int(acc), None)
         else:
-            _lib.call("qa_csr_matmat_z", left.shape[0], left.shape[1], _vp(left._rowptr.ptr),
-                      _vp(left._col.ptr), _vp(left._vals.ptr), int(left.fp32), right.shape[1],
+            # several columns: the SELL-C copy (built once per operator), each
+            # stored entry read once per 8 columns
+            S = left.sell(1)
+            sptr, cols, vals = S.ptrs
+            _lib.call("qa_sell_matmat_z", left.shape[0], left.shape[1], S.lanes, S.nslices,
+                      _vp(sptr), _vp(cols), _vp(vals), int(left.fp32), right.shape[1],
                       _vp(right.ptr), right.shape[0], _vp(out.ptr), out.shape[0], sre, sim,
                       int(acc), None)
         return out

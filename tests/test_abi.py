@@ -201,3 +201,19 @@ def test_hermitian_switch_rejects_null_solver():
     on = ctypes.c_int(7)
     assert L.qa_rk_get_hermitian(None, ctypes.byref(on)) == _lib.QA_ERR_ARG
     assert on.value == 7
+
+
+def test_sell_matmat_rejects_bad_shapes():
+    """qa_sell_matmat_z checks lanes, slice count and leading dimensions
+    before any device call."""
+    L = _lib.load()
+    vp = ctypes.c_void_p
+    rc = L.qa_sell_matmat_z(64, 64, 3, 1, vp(16), vp(16), vp(16), 0, 2, vp(16), 64, vp(16), 64,
+                            1.0, 0.0, 0, None)
+    assert rc == _lib.QA_ERR_ARG and b"lanes" in L.qa_last_error()
+    rc = L.qa_sell_matmat_z(65, 64, 1, 1, vp(16), vp(16), vp(16), 0, 2, vp(16), 64, vp(16), 65,
+                            1.0, 0.0, 0, None)
+    assert rc == _lib.QA_ERR_ARG and b"nslices" in L.qa_last_error()
+    rc = L.qa_sell_matmat_z(64, 64, 1, 1, vp(16), vp(16), vp(16), 0, 2, vp(16), 63, vp(16), 64,
+                            1.0, 0.0, 0, None)
+    assert rc == _lib.QA_ERR_ARG and b"leading" in L.qa_last_error()

@@ -105,3 +105,41 @@ def test_zeros_and_copy_roundtrip():
     x = rand_vec(51, 8, cols=2)
     X = qd.HipDense.from_array(x)
     assert np.array_equal(X.copy().to_array(), x)
+
+
+@pytest.mark.parametrize("shape,density", [((1000, 1000), 0.01), ((300, 77), 0.2),
+                                           ((65, 65), 0.5), ((40, 40), 0.0)])
+@pytest.mark.parametrize("ncols", [2, 8, 13])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_sell_matmat_against_scipy(shape, density, ncols, fp32):
+    """data.matmul with several right-hand columns runs the SELL-C block
+    product (each stored entry read once per 8 columns): against scipy,
+    with a complex scale and accumulation into out."""
+    print(f"sell_matmat shape={shape} density={density} ncols={ncols} fp32={fp32}", flush=True)
+    A = rand_csr(*shape, density, seed=shape[0] + ncols)
+    X = rand_vec(shape[1], 3, ncols)
+    Ad = qd.HipCSR(A, fp32=fp32)
+    Aref = A.astype(np.complex64).astype(np.complex128) if fp32 else A
+    got = qd.matmul(Ad, qd.HipDense.from_array(X), scale=0.5 - 2j).to_array()
+    assert_allclose(got, (0.5 - 2j) * (Aref @ X), atol=ATOL, rtol=RTOL)
+    base = rand_vec(shape[0], 4, ncols)
+    out = qd.HipDense.from_array(base)
+    qd.matmul(Ad, qd.HipDense.from_array(X), scale=1.5, out=out)
+    assert_allclose(out.to_array(), base + 1.5 * (Aref @ X), atol=ATOL, rtol=RTOL)
+
+
+@pytest.mark.parametrize("nvec", [1, 8, 13])
+def test_csr_matmat_abi_column_tiles(nvec):
+    """qa_csr_matmat_z (one thread per row and 8 columns) with leading
+    dimensions larger than the matrix."""
+    n, m = 513, 300
+    A = rand_csr(n, m, 0.05, seed=nvec)
+    X = np.zeros((m + 5, nvec), dtype=np.complex128)
+    X[:m] = rand_vec(m, 5, nvec)
+    Ad = qd.HipCSR(A)
+    xd = qd.HipDense.from_array(np.asfortranarray(X))
+    out = qd.HipDense((n + 3, nvec))
+    _lib.call("qa_csr_matmat_z", n, m, C.c_void_p(Ad._rowptr.ptr), C.c_void_p(Ad._col.ptr),
+              C.c_void_p(Ad._vals.ptr), 0, nvec, C.c_void_p(xd.ptr), m + 5, C.c_void_p(out.ptr),
+              n + 3, 1.0, 0.0, 0, None)
+    assert_allclose(out.to_array()[:n], A @ X[:m], atol=ATOL, rtol=RTOL)
