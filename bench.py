@@ -349,7 +349,7 @@ def served_errors(done, log):
     return {"requests_checked": nref, **{k: float(f"{v:.3e}") for k, v in worst.items()}}
 
 
-def bench_serving(systems, log, nreq=2040, warm=40, policy="sjf"):
+def bench_serving(systems, log, nreq=2040, warm=40, policy="sjf", cpu=True, cpu_sample=40):
     """The serving mix: Poisson arrivals at 200 req/s (seed 7), d uniform in
     {16..512}, 40% sesolve / 40% mesolve / 20% mcsolve at rtol 1e-8 /
     atol 1e-10 (the tolerances of the <O> error bar), admitted by HBM
@@ -394,6 +394,20 @@ def bench_serving(systems, log, nreq=2040, warm=40, policy="sjf"):
            "served_error": served_errors(done, log)}
     log(f"serving: {out['solves_per_s']} solves/s p50 {out['p50_latency_s']} s "
         f"p99 {out['p99_latency_s']} s peak {srv.peak_in_flight}")
+    if cpu:
+        # the reference's CPU paths (restated) for the first requests of the
+        # trace on the host's cores
+        from oracle import baselines
+        sample = [(r.kind, r.H.to_csr(), [c.full() for c in r.c_ops],
+                   [O.to_csr() for O in r.e_ops], r.psi0.full().ravel(), r.tlist, r.ntraj,
+                   r.seed, r.options["rtol"], r.options["atol"]) for r in reqs[:cpu_sample]]
+        b = baselines.c5(sample)
+        out["cpu_baseline"] = {
+            "value": b["all_cores"], "unit": "solves/s", "cores": b["cores"], "kind": "port",
+            "path": "sesolve: C RK on -iH; mesolve: RK on the matrix form with numpy's dense "
+                    "products (zgemm, one BLAS thread per process); mcsolve: C trajectory loop",
+            "est_full_solve_s_by_kind": b["est_full_solve_s_by_kind"], "sample": b["sample"]}
+        log(f"serving CPU baseline: {out['cpu_baseline']}")
     return out
 
 
@@ -743,7 +757,8 @@ def main():
         out["dense_kerr"] = bench_dense(qa, systems, log, reps=args.dense_reps, tf=args.dense_tf,
                                         cpu=not args.no_cpu)
     if not args.no_serve:
-        out["serving"] = bench_serving(systems, log, nreq=args.serve_requests)
+        out["serving"] = bench_serving(systems, log, nreq=args.serve_requests,
+                                       cpu=not args.no_cpu)
     if not args.no_plain:
         psolver = qa.MESolver(s["H"], s["c_ops"], options={**opts, "fused": False})
         t0 = time.perf_counter()

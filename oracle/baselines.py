@@ -8,7 +8,13 @@ as the reference runs one solve per process (BLAS threads pinned to 1):
 * C3: mcsolve trajectories (qto_mc_traj) farmed over the workers, as
   map='parallel' farms them (qutip/solver/parallel.py:267-337);
 * C4: the banded matrix form (qto_rk_solve_dia), the reference's Dia
-  products, on a bounded window of the solve.
+  products, on a bounded window of the solve;
+* C5: the first requests of the serving trace farmed over the workers,
+  each solved on the first output interval of its time list: sesolve by
+  the C RK on -iH, mesolve by the restated RK on the matrix-form RHS with
+  numpy's (BLAS) dense products -- the reference's path for dense collapse
+  operators (lindblad_matrix_form.pyx:170-186 over zgemm) -- and mcsolve by
+  the C trajectory loop.
 """
 import multiprocessing as mp
 import os
@@ -89,6 +95,33 @@ def _c4_worker(tf):
     return time.perf_counter() - t0, st["n_rhs"]
 
 
+def _c5_worker(req):
+    """One request of the serving mix on [tlist[0], tlist[1]]: returns the
+    seconds it took."""
+    from oracle import cbind, models, rk, tableaus
+    from threadpoolctl import threadpool_limits
+    kind, H, cs, e_ops, psi0, tl, ntraj, seed, rtol, atol = req
+    tab = tableaus.TABLEAUS["vern7"]
+    d = H.shape[0]
+    # one BLAS thread per worker process, whatever the environment says
+    threadpool_limits(1)
+    t0 = time.perf_counter()
+    if kind == "se":
+        fun = (np.array([0, d], np.int32), np.arange(d, dtype=np.int32),
+               np.ones(d, np.complex128).view(np.float64))
+        cbind.rk_solve(-1j * H, psi0, tl, tab, fun, rtol=rtol, atol=atol, nsteps=10 ** 6)
+    elif kind == "me":
+        f = models.matrix_form_rhs(H, cs)
+        rk.solve(f, models.vec(models.ket2dm(psi0)), tl, tab, rtol=rtol, atol=atol,
+                 nsteps=10 ** 6)
+    else:
+        cm = cbind.McTrajectories(H, cs, e_ops, tab, rtol=rtol, atol=atol)
+        ss = np.random.SeedSequence(seed).spawn(ntraj)
+        for s_ in ss:
+            cm.run(psi0, tl, np.random.default_rng(s_).random(64))
+    return time.perf_counter() - t0
+
+
 # ---- baselines -----------------------------------------------------------------
 def c1(seconds=3.0):
     """JC mesolve solves/s: one core, and every worker solving at once."""
@@ -142,3 +175,27 @@ def c4(rhs_full, tf=0.6):
     return {"one_core": round(one, 5), "all_cores": round(allc, 4), "cores": nproc,
             "sample": f"t in [0, {tf}] of the window ({r1} RHS, {t1:.1f} s on one core), "
                       f"scaled by {rhs_full} RHS of the full solve"}
+
+
+def c5(requests):
+    """Serving-mix solves/s on the host's cores: ``requests`` (tuples of
+    kind, H, c_ops, e_ops, psi0, tlist, ntraj, seed, rtol, atol) farmed over
+    the workers, each solved on its first output interval; the full solves
+    are estimated as that work times the number of intervals (the step
+    sizes of these constant generators change little over a solve)."""
+    nproc = workers()
+    nint = len(requests[0][5]) - 1
+    reqs = [r[:5] + (np.asarray(r[5][:2]),) + r[6:] for r in requests]
+    t0 = time.perf_counter()
+    with _pool(nproc) as p:
+        ts = p.map(_c5_worker, reqs, chunksize=1)
+    wall = time.perf_counter() - t0
+    kinds = {}
+    for r, t in zip(requests, ts):
+        kinds.setdefault(r[0], []).append(t * nint)
+    return {"all_cores": round(len(requests) / (wall * nint), 4), "cores": nproc,
+            "wall_s": round(wall, 2),
+            "est_full_solve_s_by_kind": {k: round(float(np.mean(v)), 2) for k, v in kinds.items()},
+            "sample": f"the first {len(requests)} requests of the trace, each solved over its "
+                      f"first output interval (1 of {nint}) on {nproc} workers; full solves "
+                      f"estimated as {nint} x that work"}
