@@ -265,7 +265,7 @@ def bench_dense(qa, systems, log, reps=2, tf=None, cpu=True):
         t_step, t_dense = C.c_double(), C.c_double()
         _lib.call("qa_rk_profile_sequence", eng.handle, 10, C.byref(t_step), C.byref(t_dense))
         rhs_ms = t_step.value
-        tf = flop_rhs / (rhs_ms * 1e-3) / 1e12
+        tflops = flop_rhs / (rhs_ms * 1e-3) / 1e12
         best = min(ts)
         out[prec] = {"solves_per_s": round(1 / best, 4), "s_per_solve": round(best, 3),
                      "err_vs_fp64": float(err), "rk_attempts": st["attempts"],
@@ -279,12 +279,12 @@ def bench_dense(qa, systems, log, reps=2, tf=None, cpu=True):
             out[prec]["path"] = "FP64 products of the operators' diagonals"
             log(f"dense {prec}: {1 / best:.3f} solves/s err {err:.2e} stage {rhs_ms * 1e3:.1f} us")
             continue
-        out[prec]["mfma_tflops"] = round(tf, 2)
+        out[prec]["mfma_tflops"] = round(tflops, 2)
         out[prec]["mfma_peak_tflops"] = 157.3 if prec == "fp32" else 2500.0
         if prec.startswith("bf16"):
             out[prec]["hardware_products_per_real_product"] = 3 if prec == "bf16x3" else 6
         log(f"dense {prec}: {1 / best:.3f} solves/s err {err:.2e} stage {rhs_ms * 1e3:.1f} us "
-            f"({tf:.1f} TFLOP/s)")
+            f"({tflops:.1f} TFLOP/s)")
     # the same solve through the sparse Liouvillian (the banded structure
     # the reference's Dia products exploit): 65536^2 superoperator, SELL-C
     solver = qa.MESolver(s["H"], s["c_ops"], options={"rtol": 1e-8, "atol": 1e-10})
@@ -313,15 +313,15 @@ def served_errors(done, log):
     """The served <O>(t) of the first requests against FP64 references, as
     tests/test_gpu_serve.py checks them: sesolve against the matrix
     exponential (scipy), mesolve against the committed FP64 fixture,
-    mcsolve in standard errors of that master equation, the standard error
-    bounded below by 2 ||O|| sqrt(p (1 - p) / ntraj) with p the no-jump
-    probability (most requests of this mix see no jump, so their sample
-    standard error is 0)."""
+    mcsolve as |mean - FP64| / (4 se + 1e-6), at most 1 within the bar,
+    the standard error bounded below by 2 ||O|| sqrt(p (1 - p) / ntraj) with
+    p the no-jump probability (most requests of this mix see no jump, so
+    their sample standard error is 0)."""
     import scipy.sparse.linalg as spla
     g = np.load(ROOT / "tests" / "golden" / "serving_seed7.npz")
     nref = int(g["nreq"])
     ref = {int(i): e for i, e in zip(g["ids"], g["expect"])}
-    worst = {"se": 0.0, "me": 0.0, "mc_over_sigma": 0.0}
+    worst = {"se": 0.0, "me": 0.0, "mc_dev_over_4se_plus_1e-6": 0.0}
     for r in done:
         if r.id >= nref or r.error:
             continue
@@ -343,8 +343,9 @@ def served_errors(done, log):
             p = np.clip(1 - np.sum(np.abs(psis) ** 2, axis=1), 0.0, 1.0)
             onorm = np.array([1.0, np.linalg.norm(H, 2)])[:, None]
             se = np.maximum(np.asarray(r.stderr), 2 * onorm * np.sqrt(p * (1 - p) / r.ntraj))
-            dev = np.abs(got - ref[r.id]) / (se + 1e-12)
-            worst["mc_over_sigma"] = max(worst["mc_over_sigma"], float(dev.max()))
+            dev = np.abs(got - ref[r.id]) / (4 * se + 1e-6)
+            worst["mc_dev_over_4se_plus_1e-6"] = max(worst["mc_dev_over_4se_plus_1e-6"],
+                                                     float(dev.max()))
     log(f"served error (first {nref} requests): {worst}")
     return {"requests_checked": nref, **{k: float(f"{v:.3e}") for k, v in worst.items()}}
 

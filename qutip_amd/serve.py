@@ -13,8 +13,7 @@ overlaps on the device:
   products; dense collapse operators make the superoperator infeasible:
   d^4 non-zeros);
 * mcsolve: one persistent trajectory kernel for all trajectories of the
-  request (complex64 states with FP64 norms unless the request asks for
-  ``precision='fp64'``).
+  request (complex128; complex64 with FP64 norms on request).
 
 The host loop never waits on one request: it polls each request's progress
 (qa_rk_advance / an event after the trajectory kernel), tops up its queued
@@ -133,6 +132,9 @@ class _Running:
                 Hnh = req.H.full() - 0.5j * sum(c.conj().T @ c for c in cs)
                 self.eng = RKEngine(None, "vern7", o["rtol"], o["atol"], o["nsteps"],
                                     dense=(Hnh, cs), dense_precision=o.get("dense_precision", "fp32"))
+                # rho0 = |psi0><psi0| is Hermitian: the upper tiles and their
+                # mirror (mesolve's assume_hermitian_state)
+                self.eng.set_hermitian(True)
                 psi = req.psi0.full().ravel()
                 rho = np.outer(psi, psi.conj())
                 d = rho.shape[0]
@@ -158,11 +160,12 @@ class _Running:
             _lib.call("qa_rk_begin", self.eng.handle, len(times),
                       times.ctypes.data_as(C.POINTER(C.c_double)), 0, None)
         else:
-            # the chosen precision form: complex64 states and products,
-            # FP64 norms, collapse search and expectation values
+            # complex128 unless the request asks for complex64: at the
+            # mix's sizes (d <= 512, dense collapse operators) the complex64
+            # trajectory kernel runs 2-3x slower (measured per request)
             self.solver = MCSolver(req.H, req.c_ops, options={
                 "rtol": o["rtol"], "atol": o["atol"], "nsteps": o["nsteps"],
-                "precision": o.get("precision", "fp32")})
+                "precision": o.get("precision", "fp64")})
             seeds = _read_seeds(req.seed, req.ntraj)
             self._draws = draws_for(seeds, 64)
             self.stream = C.c_void_p()
