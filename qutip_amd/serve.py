@@ -161,8 +161,10 @@ class _Running:
                       times.ctypes.data_as(C.POINTER(C.c_double)), 0, None)
         else:
             # complex128 unless the request asks for complex64: at the
-            # mix's sizes (d <= 512, dense collapse operators) the complex64
-            # trajectory kernel runs 2-3x slower (measured per request)
+            # mix's tolerances (rtol 1e-8) the complex64 trajectory kernel
+            # ran 2-3x slower per request (its error estimate carries FP32
+            # rounding of about 1e-7, which a 1e-8 tolerance turns into
+            # smaller steps)
             self.solver = MCSolver(req.H, req.c_ops, options={
                 "rtol": o["rtol"], "atol": o["atol"], "nsteps": o["nsteps"],
                 "precision": o.get("precision", "fp64")})
