@@ -857,15 +857,15 @@ __device__ __forceinline__ void gt_mma_bf16(const GemmTile &L, int buf, int wi, 
     }
 }
 
-// The wave's 16 x 16 tile += A_chunk B_chunk from LDS buffer `buf`.
-__device__ __forceinline__ void gt_mma(const GemmTile &L, int buf, int prec, int wi, int wj,
-                                       int lane, v4f &re, v4f &im) {
+// The wave's 16 x 16 tile += A_chunk B_chunk from LDS buffer `buf`; NP = 0:
+// FP32 MFMA, 2 / 3: the bf16 x3 / x6 splits.
+template <int NP>
+__device__ __forceinline__ void gt_mma(const GemmTile &L, int buf, int wi, int wj, int lane,
+                                       v4f &re, v4f &im) {
     const int r = lane & 15, kq = lane >> 4;
     v4f re2 = {0.f, 0.f, 0.f, 0.f}, im2 = {0.f, 0.f, 0.f, 0.f};
-    if (prec == 1) {
-        gt_mma_bf16<2>(L, buf, wi, wj, lane, re, im, re2, im2);
-    } else if (prec == 2) {
-        gt_mma_bf16<3>(L, buf, wi, wj, lane, re, im, re2, im2);
+    if constexpr (NP != 0) {
+        gt_mma_bf16<NP>(L, buf, wi, wj, lane, re, im, re2, im2);
     } else {
 #pragma unroll
         for (int st = 0; st < kGK / 4; ++st) {
@@ -905,12 +905,13 @@ __device__ __forceinline__ void gt_operands(const DenseOp &D, int p, const float
 // and discards it), so every barrier is reached by all threads.
 constexpr int kGHalves = 2;
 
+
 struct GemmShared {
     GemmTile L[kGHalves];
 };
 
 // Product p's operands: p < 0 selects the T kernel's single product C_c X.
-template <typename TB>
+template <typename TB, int NP>
 __device__ __forceinline__ void gt_run(const DenseOp &D, GemmShared &G, int np,
                                        const float2 *A0, const TB *B0, int i0, int j0,
                                        v4f &re, v4f &im) {
@@ -944,7 +945,7 @@ __device__ __forceinline__ void gt_run(const DenseOp &D, GemmShared &G, int np,
         const int nx = min(cur + kGHalves, total - 1);
         operands(nx, A, B, k0);
         gt_fetch(A, B, D.N, i0, j0, k0, tid, ra, rb);
-        if (cur < total) gt_mma(L, it & 1, D.prec, wi, wj, lane, re, im);
+        if (cur < total) gt_mma<NP>(L, it & 1, wi, wj, lane, re, im);
         gt_store(L, (it + 1) & 1, tid, ra, rb);
         __syncthreads();
     }
@@ -969,6 +970,7 @@ __device__ __forceinline__ void gt_run(const DenseOp &D, GemmShared &G, int np,
 
 // T_c = C_c X for the workgroup's tile (grid N/32, N/32, max(nc, 1)); the
 // blockIdx.z == 0 workgroups also write Xf = complex64(X) for their tile.
+template <int NP>
 __device__ __forceinline__ void dense_t_tiled(const DenseOp &D, const double2 *X) {
     __shared__ GemmShared G;
     const int i0 = blockIdx.x * kGT, j0 = blockIdx.y * kGT, c = blockIdx.z;
@@ -981,7 +983,7 @@ __device__ __forceinline__ void dense_t_tiled(const DenseOp &D, const double2 *X
     }
     if (c >= D.nc) return;  // uniform over the workgroup
     v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
-    gt_run<double2>(D, G, 1, D.C + (size_t)c * NN, X, i0, j0, re, im);
+    gt_run<double2, NP>(D, G, 1, D.C + (size_t)c * NN, X, i0, j0, re, im);
     if ((threadIdx.x >> 8) != 0) return;
     const int lane = threadIdx.x & 63, w = (threadIdx.x & 255) >> 6;
 #pragma unroll
@@ -991,18 +993,21 @@ __device__ __forceinline__ void dense_t_tiled(const DenseOp &D, const double2 *X
 }
 
 // k tile = Hm Xf + Xf Hp + sum_c T_c Cd_c; complete in half 0.
+template <int NP>
 __device__ __forceinline__ void dense_k_tiled(const DenseOp &D, GemmShared &G, v4f &re,
                                               v4f &im) {
-    gt_run<float2>(D, G, 2 + D.nc, nullptr, (const float2 *)nullptr, blockIdx.x * kGT,
+    gt_run<float2, NP>(D, G, 2 + D.nc, nullptr, (const float2 *)nullptr, blockIdx.x * kGT,
                    blockIdx.y * kGT, re, im);
 }
 
+template <int NP>
 __global__ __launch_bounds__(256 * kGHalves) void rk_dense_t_tiled_kernel(EpiArgs P, DenseOp D) {
     if (P.ctl->phase != P.run_phase) return;
     const StageVecs v = stage_vecs(P, P.ctl);
-    dense_t_tiled(D, v.xin);
+    dense_t_tiled<NP>(D, v.xin);
 }
 
+template <int NP>
 __global__ __launch_bounds__(256 * kGHalves) void rk_dense_stage_tiled_kernel(EpiArgs P,
                                                                              DenseOp D) {
     __shared__ GemmShared G;
@@ -1018,7 +1023,7 @@ __global__ __launch_bounds__(256 * kGHalves) void rk_dense_stage_tiled_kernel(Ep
         for (int q = 0; q < 4; ++q) epi_pre(P, v, tile_index(D.N, i0, j0, lane, q), o[q], er[q]);
     }
     v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
-    dense_k_tiled(D, G, re, im);
+    dense_k_tiled<NP>(D, G, re, im);
     double part = 0.0;
     if (h == 0) {
 #pragma unroll
@@ -1284,6 +1289,7 @@ __global__ __launch_bounds__(512 / IPT) void rk_dense_herm_stage_kernel(EpiArgs 
 
 // The same on the upper 32 x 32 workgroup tiles of the LDS-staged path
 // (N >= kTiledN): 2 E^2 items over 512 threads.
+template <int NP>
 __global__ __launch_bounds__(256 * kGHalves) void rk_dense_herm_stage_tiled_kernel(EpiArgs P,
                                                                                   DenseOp D) {
     constexpr int IPT = 2 * kGT * kGT / (256 * kGHalves);
@@ -1297,7 +1303,7 @@ __global__ __launch_bounds__(256 * kGHalves) void rk_dense_herm_stage_tiled_kern
     const int i0 = kGT * bi, j0 = kGT * bj;
     const int nitems = bi == bj ? kGT * kGT : 2 * kGT * kGT;
     v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
-    gt_run<float2>(D, G, 2 + D.nc, nullptr, (const float2 *)nullptr, i0, j0, re, im);
+    gt_run<float2, NP>(D, G, 2 + D.nc, nullptr, (const float2 *)nullptr, i0, j0, re, im);
     // the tile is complete in half 0; it goes through LDS to the items
     __syncthreads();
     float2 *tile = (float2 *)&G.L[0];
@@ -1328,16 +1334,18 @@ __global__ __launch_bounds__(256 * kGHalves) void rk_dense_herm_stage_tiled_kern
 }
 
 // Ungated tiled versions (qa_rk_apply and the first-step estimate).
+template <int NP>
 __global__ __launch_bounds__(256 * kGHalves) void dense_t_tiled_kernel(DenseOp D,
                                                                       const double2 *X) {
-    dense_t_tiled(D, X);
+    dense_t_tiled<NP>(D, X);
 }
 
+template <int NP>
 __global__ __launch_bounds__(256 * kGHalves) void dense_rhs_tiled_kernel(DenseOp D,
                                                                         double2 *out) {
     __shared__ GemmShared G;
     v4f re = {0.f, 0.f, 0.f, 0.f}, im = {0.f, 0.f, 0.f, 0.f};
-    dense_k_tiled(D, G, re, im);
+    dense_k_tiled<NP>(D, G, re, im);
     if ((threadIdx.x >> 8) != 0) return;
     const int lane = threadIdx.x & 63, w = (threadIdx.x & 255) >> 6;
     const int i0 = blockIdx.x * kGT + 16 * (w & 1), j0 = blockIdx.y * kGT + 16 * (w >> 1);
@@ -1834,17 +1842,28 @@ int launch_sparse_stage(qa_rk_solver *S, int i) {
     return QA_OK;
 }
 
+// Launches KERNEL<NP> for the dense engine's precision (prec 0: FP32 MFMA,
+// NP 0; 1: bf16 x3, NP 2; 2: bf16 x6, NP 3).
+#define QA_DENSE_LAUNCH(PREC, KERNEL, GRID, BLOCK, STREAM, ...)                             \
+    do {                                                                                 \
+        switch (PREC) {                                                                  \
+            case 1: hipLaunchKernelGGL(KERNEL<2>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+            case 2: hipLaunchKernelGGL(KERNEL<3>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL(KERNEL<0>, GRID, BLOCK, 0, STREAM, __VA_ARGS__); break; \
+        }                                                                                \
+    } while (0)
+
 // Hermitian stage inputs: the T layer (which also writes Xf), then the
 // upper-tile stage kernel.
 int launch_dense_herm_stage(qa_rk_solver *S, const EpiArgs &P) {
     const int N = S->dop.N, nz = std::max(S->dop.nc, 1);
     if (S->dop.tiled) {
         const int nt = N / kGT;
-        hipLaunchKernelGGL(rk_dense_t_tiled_kernel, dim3(nt, nt, nz), dim3(256 * kGHalves), 0,
-                           S->stream, P, S->dop);
+        QA_DENSE_LAUNCH(S->dop.prec, rk_dense_t_tiled_kernel, dim3(nt, nt, nz),
+                        dim3(256 * kGHalves), S->stream, P, S->dop);
         QA_CHECK_LAUNCH();
-        hipLaunchKernelGGL(rk_dense_herm_stage_tiled_kernel, dim3(nt * (nt + 1) / 2),
-                           dim3(256 * kGHalves), 0, S->stream, P, S->dop);
+        QA_DENSE_LAUNCH(S->dop.prec, rk_dense_herm_stage_tiled_kernel, dim3(nt * (nt + 1) / 2),
+                        dim3(256 * kGHalves), S->stream, P, S->dop);
         QA_CHECK_LAUNCH();
         return QA_OK;
     }
@@ -1879,11 +1898,11 @@ int launch_dense_stage(qa_rk_solver *S, int i) {
     if (S->dop.herm) return launch_dense_herm_stage(S, P);
     if (S->dop.tiled) {
         const int nt = S->dop.N / kGT;
-        hipLaunchKernelGGL(rk_dense_t_tiled_kernel, dim3(nt, nt, std::max(S->dop.nc, 1)),
-                           dim3(256 * kGHalves), 0, S->stream, P, S->dop);
+        QA_DENSE_LAUNCH(S->dop.prec, rk_dense_t_tiled_kernel, dim3(nt, nt, std::max(S->dop.nc, 1)),
+                        dim3(256 * kGHalves), S->stream, P, S->dop);
         QA_CHECK_LAUNCH();
-        hipLaunchKernelGGL(rk_dense_stage_tiled_kernel, dim3(nt, nt), dim3(256 * kGHalves), 0,
-                           S->stream, P, S->dop);
+        QA_DENSE_LAUNCH(S->dop.prec, rk_dense_stage_tiled_kernel, dim3(nt, nt),
+                        dim3(256 * kGHalves), S->stream, P, S->dop);
         QA_CHECK_LAUNCH();
         return QA_OK;
     }
@@ -1930,11 +1949,11 @@ int engine_rhs(qa_rk_solver *S, const double2 *x, double2 *out, double t = 0.0) 
     }
     if (S->kind == 1 && S->dop.tiled) {
         const int nt = S->dop.N / kGT;
-        hipLaunchKernelGGL(dense_t_tiled_kernel, dim3(nt, nt, std::max(S->dop.nc, 1)),
-                           dim3(256 * kGHalves), 0, S->stream, S->dop, x);
+        QA_DENSE_LAUNCH(S->dop.prec, dense_t_tiled_kernel, dim3(nt, nt, std::max(S->dop.nc, 1)),
+                        dim3(256 * kGHalves), S->stream, S->dop, x);
         QA_CHECK_LAUNCH();
-        hipLaunchKernelGGL(dense_rhs_tiled_kernel, dim3(nt, nt), dim3(256 * kGHalves), 0,
-                           S->stream, S->dop, out);
+        QA_DENSE_LAUNCH(S->dop.prec, dense_rhs_tiled_kernel, dim3(nt, nt),
+                        dim3(256 * kGHalves), S->stream, S->dop, out);
         QA_CHECK_LAUNCH();
         return QA_OK;
     }
