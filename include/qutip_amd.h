@@ -247,6 +247,15 @@ int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, co
  * No effect on the other engines; get reports what the engine runs. */
 int qa_rk_set_hermitian(qa_rk_solver *s, int on);
 int qa_rk_get_hermitian(qa_rk_solver *s, int *on);
+/* Tile shape of the dense engine's MFMA kernels (the autotuner measures
+ * and picks it; no counterpart in the reference, whose zgemm picks its own
+ * blocking): tiled 1 = 32 x 32 workgroup tiles staged through LDS
+ * (N % 64 == 0), 0 = one 16 x 16 tile per workgroup; nsplit / hsplit =
+ * waves per 16 x 16 tile (1, 2, 4, 8) of the full-RHS and of the Hermitian
+ * kernels, with N a multiple of 16 nsplit (32 nsplit for the bf16 forms)
+ * and of 32 hsplit.  The defaults come from N alone. */
+int qa_rk_set_dense_shape(qa_rk_solver *s, int tiled, int nsplit, int hsplit);
+int qa_rk_get_dense_shape(qa_rk_solver *s, int *tiled, int *nsplit, int *hsplit);
 int qa_rk_destroy(qa_rk_solver *s);
 
 /* out = A x with the engine's generator (device vectors; waits for the

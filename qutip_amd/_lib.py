@@ -75,6 +75,8 @@ SIGNATURES = {
                            P_f64, P_f64, P_f64, P_f64, P_f64, f64, f64, i32, f64, f64, f64, i32],
     "qa_rk_set_hermitian": [vp, i32],
     "qa_rk_get_hermitian": [vp, P_i32],
+    "qa_rk_set_dense_shape": [vp, i32, i32, i32],
+    "qa_rk_get_dense_shape": [vp, P_i32, P_i32, P_i32],
     "qa_rk_apply": [vp, vp, vp],
     "qa_rk_apply_at": [vp, f64, vp, vp],
     "qa_rk_set_coefficients": [vp, i32, P_i32, P_i32, i32, P_i32, i32, P_f64],

@@ -1,4 +1,5 @@
-"""Autotuner for the SELL-C spmv of the RK stage kernels.
+"""Autotuners: the SELL-C spmv of the RK stage kernels and the tile shape of
+the matrix-form MFMA kernels.
 
 The tile shape is the number of lanes per row L (1, 2, 4, 8; C = 64 / L
 rows per wavefront).  Few lanes per row keep the padding low, more lanes per
@@ -94,3 +95,66 @@ def lanes_for(A, fp32=False, coded=False):
 
 def cache():
     return dict(_CACHE)
+
+
+# ---- matrix-form MFMA kernels -------------------------------------------------
+# The shape (tiled, nsplit, hsplit) of qa_rk_set_dense_shape: LDS-staged
+# 32 x 32 workgroup tiles or one 16 x 16 tile per workgroup, and the waves
+# that split each tile's K range.  Measured like the lanes: every candidate
+# on the engine itself (before its state is set), interleaved rounds of the
+# stage launches in solve order, the fastest round counts.
+MIN_DENSE_TUNE_N = 128
+
+
+def dense_candidates(N, fp32_mfma, herm, default):
+    """Valid shapes for an N x N state: the default first."""
+    kstep = 16 if fp32_mfma else 32
+    tiled0, ns0, hs0 = default
+    out = [tuple(default)]
+    if N % 64 == 0:
+        out.append((1, ns0, hs0))
+    for s in (1, 2, 4, 8):
+        if herm and N % (32 * s) == 0:
+            out.append((0, ns0, s))
+        if not herm and N % (kstep * s) == 0:
+            out.append((0, s, hs0))
+    seen, uniq = set(), []
+    for c in out:
+        if c not in seen:
+            seen.add(c)
+            uniq.append(c)
+    return uniq
+
+
+def dense_shape_for(eng, reps=4, rounds=3):
+    """The measured best shape for the dense engine ``eng`` in its current
+    Hermitian mode (None below MIN_DENSE_TUNE_N or with QUTIP_AMD_NO_TUNE);
+    cached per (N, collapse operators, precision, mode)."""
+    import ctypes as C
+
+    from . import _lib
+    N = int(round(eng.n ** 0.5))
+    if N < MIN_DENSE_TUNE_N or os.environ.get("QUTIP_AMD_NO_TUNE"):
+        return None
+    herm = eng.hermitian
+    nc = len(eng._dense_host[1]) // (N * N) if eng._dense_host[1].size > 1 else 0
+    k = f"dense-n{N}-c{nc}-{eng.dense_precision}-{'herm' if herm else 'full'}"
+    if not _CACHE:
+        _load_file()
+    if k not in _CACHE:
+        default = eng.dense_shape
+        cands = dense_candidates(N, eng.dense_precision == "fp32", herm, default)
+        best = {c: float("inf") for c in cands}
+        t_step, t_dense = C.c_double(), C.c_double()
+        for rnd in range(rounds + 1):
+            for c in cands:
+                eng.dense_shape = c
+                _lib.call("qa_rk_profile_sequence", eng.handle, reps, C.byref(t_step),
+                          C.byref(t_dense))
+                if rnd > 0:
+                    best[c] = min(best[c], t_step.value)
+        pick = min(best, key=best.get)
+        _CACHE[k] = {"shape": list(pick),
+                     "ms": {",".join(map(str, c)): v for c, v in best.items()}}
+        _save_file()
+    return tuple(_CACHE[k]["shape"])

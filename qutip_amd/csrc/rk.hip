@@ -525,7 +525,8 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 struct DenseOp {
     int N, nc, prec;  // prec 0: FP32 MFMA, 1: bf16 x3 split, 2: bf16 x6 split
-    int nsplit;       // waves per 16 x 16 tile (split K)
+    int nsplit;       // waves per 16 x 16 tile (split K) of the full-RHS kernels
+    int hsplit;       // the same for the Hermitian kernels (32-wide K steps)
     int tiled;        // 1: 32 x 32 workgroup tiles staged through LDS (large N)
     int herm;         // 1: Hermitian stage inputs, the upper tiles and their mirror
     const float2 *Hm, *Hp, *C, *Cd;  // N x N each; C / Cd: nc blocks
@@ -1868,7 +1869,7 @@ int launch_dense_herm_stage(qa_rk_solver *S, const EpiArgs &P) {
         return QA_OK;
     }
     const int nt = N / 16;
-    const int ns = herm_split(N);
+    const int ns = S->dop.hsplit;
     hipLaunchKernelGGL(rk_dense_herm_t_kernel, dim3(nt, nt, nz), dim3(64 * ns), 0, S->stream, P,
                        S->dop);
     QA_CHECK_LAUNCH();
@@ -1885,9 +1886,9 @@ int launch_dense_herm_stage(qa_rk_solver *S, const EpiArgs &P) {
 
 // Workgroups of the last stage kernel, i.e. the error partials per attempt.
 int stage_parts(const qa_rk_solver *S) {
-    if (S->kind == 1 && S->dop.herm) {
+    if (S->kind == 1) {
         const int nt = S->dop.N / (S->dop.tiled ? kGT : 16);
-        return nt * (nt + 1) / 2;
+        return S->dop.herm ? nt * (nt + 1) / 2 : nt * nt;
     }
     return S->nparts;
 }
@@ -2477,8 +2478,10 @@ int qa_rk_create_dense(qa_rk_solver **out, int N, int nc, const double *H_nh, co
     S->dop.Cd = d + (2 + (size_t)nc) * NN;
     S->dop.T = d + (2 + 2 * (size_t)nc) * NN;
     S->dop.Xf = d + (2 + 3 * (size_t)nc) * NN;
+    S->dop.hsplit = N % 32 == 0 ? herm_split(N) : 1;
     S->dop.tiled = (N >= kTiledN && N % kGK == 0 && N % kGT == 0) ? 1 : 0;
-    if (S->dop.tiled) S->nparts = (N / kGT) * (N / kGT);
+    // S->nparts stays (N / 16)^2, the most any tile shape needs (the shape
+    // can be changed later, qa_rk_set_dense_shape)
     rc = rk_finish_create(S, order, rk_step, rk_extra, dorder, a, b, c, e, bi, rtol, atol, nsteps,
                           first_step, min_step, max_step, interpolate);
     if (rc) return rc;
@@ -2642,6 +2645,38 @@ int qa_rk_set_hermitian(qa_rk_solver *S, int on) {
         S->dop.herm = herm;
         S->graph_dirty = true;
     }
+    return QA_OK;
+}
+
+int qa_rk_set_dense_shape(qa_rk_solver *S, int tiled, int nsplit, int hsplit) {
+    QA_REQUIRE(S != nullptr, "qa_rk_set_dense_shape: null solver");
+    QA_REQUIRE(S->kind == 1, "qa_rk_set_dense_shape: not a dense (MFMA) engine");
+    const int N = S->dop.N;
+    const bool pow2 = [](int v) { return v == 1 || v == 2 || v == 4 || v == 8; }(nsplit) &&
+                      [](int v) { return v == 1 || v == 2 || v == 4 || v == 8; }(hsplit);
+    QA_REQUIRE(pow2, "qa_rk_set_dense_shape: nsplit and hsplit must be 1, 2, 4 or 8");
+    QA_REQUIRE(tiled == 0 || (tiled == 1 && N % kGK == 0 && N % kGT == 0),
+               "qa_rk_set_dense_shape: tiled needs N %% %d == 0", kGK);
+    const int kstep = S->dop.prec >= 1 ? 32 : 16;
+    QA_REQUIRE(N % (nsplit * kstep) == 0,
+               "qa_rk_set_dense_shape: N must be a multiple of %d x nsplit", kstep);
+    QA_REQUIRE(N % (hsplit * 32) == 0 || N % 32 != 0,
+               "qa_rk_set_dense_shape: N must be a multiple of 32 x hsplit");
+    if (tiled != S->dop.tiled || nsplit != S->dop.nsplit || hsplit != S->dop.hsplit) {
+        S->dop.tiled = tiled;
+        S->dop.nsplit = nsplit;
+        S->dop.hsplit = N % 32 == 0 ? hsplit : 1;
+        S->graph_dirty = true;
+    }
+    return QA_OK;
+}
+
+int qa_rk_get_dense_shape(qa_rk_solver *S, int *tiled, int *nsplit, int *hsplit) {
+    QA_REQUIRE(S != nullptr && tiled && nsplit && hsplit, "qa_rk_get_dense_shape: null argument");
+    QA_REQUIRE(S->kind == 1, "qa_rk_get_dense_shape: not a dense (MFMA) engine");
+    *tiled = S->dop.tiled;
+    *nsplit = S->dop.nsplit;
+    *hsplit = S->dop.hsplit;
     return QA_OK;
 }
 

@@ -246,10 +246,28 @@ class RKEngine:
                   _ip(rows), _ip(cols), _dp(w))
         self.nfun = len(kinds)
 
-    def set_hermitian(self, on):
+    def set_hermitian(self, on, tune=True):
         """Hermitian states on the dense engine (assume_hermitian_state):
-        the upper tiles and their mirror; no effect on the other engines."""
+        the upper tiles and their mirror; no effect on the other engines.
+        On an MFMA engine the tile shape for that mode is then taken from
+        the autotuner (measured once per size, precision and mode)."""
         _lib.call("qa_rk_set_hermitian", self.handle, int(bool(on)))
+        if tune and getattr(self, "dense_precision", "fp64_banded") in ("fp32", "bf16x3",
+                                                                         "bf16x6"):
+            shape = autotune.dense_shape_for(self)
+            if shape is not None:
+                self.dense_shape = shape
+
+    @property
+    def dense_shape(self):
+        """(tiled, nsplit, hsplit) of an MFMA engine."""
+        t, n, h = C.c_int(), C.c_int(), C.c_int()
+        _lib.call("qa_rk_get_dense_shape", self.handle, C.byref(t), C.byref(n), C.byref(h))
+        return (t.value, n.value, h.value)
+
+    @dense_shape.setter
+    def dense_shape(self, shape):
+        _lib.call("qa_rk_set_dense_shape", self.handle, *map(int, shape))
 
     @property
     def hermitian(self):

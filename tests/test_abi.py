@@ -217,3 +217,24 @@ def test_sell_matmat_rejects_bad_shapes():
     rc = L.qa_sell_matmat_z(64, 64, 1, 1, vp(16), vp(16), vp(16), 0, 2, vp(16), 63, vp(16), 64,
                             1.0, 0.0, 0, None)
     assert rc == _lib.QA_ERR_ARG and b"leading" in L.qa_last_error()
+
+
+def test_dense_shape_rejects_null_solver():
+    L = _lib.load()
+    assert L.qa_rk_set_dense_shape(None, 0, 1, 1) == _lib.QA_ERR_ARG
+    t, n, h = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert L.qa_rk_get_dense_shape(None, ctypes.byref(t), ctypes.byref(n),
+                                   ctypes.byref(h)) == _lib.QA_ERR_ARG
+
+
+def test_dense_shape_candidates():
+    """The autotuner's candidate shapes are valid for N (no device)."""
+    from qutip_amd import autotune
+    c = autotune.dense_candidates(512, True, True, (1, 8, 8))
+    assert c[0] == (1, 8, 8)
+    assert (0, 8, 1) in c and (0, 8, 8) in c and len(set(c)) == len(c)
+    c = autotune.dense_candidates(96, True, False, (0, 2, 1))
+    assert all(t == 0 for t, _, _ in c)                 # 96 % 64 != 0: no LDS tiles
+    assert {ns for _, ns, _ in c} == {1, 2}             # 96 % 32 == 0, 96 % 64 != 0
+    c = autotune.dense_candidates(256, False, False, (0, 8, 8))
+    assert {ns for t, ns, _ in c if t == 0} == {1, 2, 4, 8}   # bf16: 32-wide steps

@@ -107,3 +107,35 @@ def test_non_hermitian_state_takes_full_rhs():
     solver.run(qa.basis(N, 0), tl, e_ops=e)
     assert solver.rhs.assume_hermitian_state is True
     assert solver._integrator._engine.hermitian is True
+
+
+@pytest.mark.parametrize("N", [128, 256, 512])
+def test_dense_shapes_agree_and_autotuner_picks_one(N):
+    """Every tile shape the autotuner may pick gives the same solve to the
+    FP32 products' rounding; invalid shapes are refused; the tuned shape is
+    one of the candidates."""
+    from qutip_amd import autotune
+    print(f"dense shapes N={N}", flush=True)
+    H, cs, Hnh, rho0 = _system(N, 3 + N, nc=1)
+    tl = np.linspace(0, 0.3, 4)
+    ops = [H]
+    ref = None
+    for herm in (True, False):
+        eng = RKEngine(None, "vern7", 1e-8, 1e-10, dense=(Hnh, cs), dense_precision="fp32")
+        eng.set_hermitian(herm, tune=False)
+        default = eng.dense_shape
+        for shape in autotune.dense_candidates(N, True, herm, default):
+            eng.dense_shape = shape
+            assert eng.dense_shape == shape
+            eng.set_functionals(*_trace_functionals(ops, N))
+            eng.set_state(tl[0], rho0.ravel(order="F"))
+            vals, _, _ = eng.run(tl[1:])
+            if ref is None:
+                ref = vals
+            diff = np.abs(vals - ref).max() / max(1.0, np.abs(ref).max())
+            print(f"  herm={herm} shape={shape}: rel diff {diff:.1e}", flush=True)
+            assert diff < 2e-6
+        with pytest.raises(ValueError):  # an argument error of the C ABI
+            eng.dense_shape = (0, 3, 1)
+        tuned = autotune.dense_shape_for(eng)
+        assert tuned in autotune.dense_candidates(N, True, herm, default)
