@@ -52,7 +52,8 @@ enum Status {
 
 constexpr int kMaxStages = 16;
 constexpr int kMaxPending = 4;     // output times handled per output launch
-constexpr int kOutChunk = 1024;    // functional entries per output workgroup
+constexpr int kOutChunk = 16;      // functional entries per output workgroup
+constexpr int kOutThreads = 64;    // one wavefront per output workgroup
 constexpr int kStepBlocks = 256;   // threads per stage workgroup
 
 struct RKCtl {
@@ -1377,10 +1378,15 @@ __global__ __launch_bounds__(64 * kMaxSplit) void dense_rhs_kernel(DenseOp D, co
 // per workgroup costs more than this launch.
 __global__ __launch_bounds__(256) void rk_control_kernel(RKCtl *ctl, const double *partials,
                                                          int nparts, int nrows) {
-    if (ctl->phase != PH_STEP) return;
     __shared__ double smem[256 / QA_WAVE];
-    const double tot = sum_partials(partials, nparts, smem);
-    if (threadIdx.x == 0) after_attempt(ctl, ctl->adaptive ? sqrt(tot / (double)nrows) : 0.0);
+    // the partials and the control block are read in one round trip; the
+    // sum is sum_partials' (strided per thread, then the block tree)
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) v += partials[i];
+    const int phase = ctl->phase, adaptive = ctl->adaptive;
+    if (phase != PH_STEP) return;
+    const double tot = block_sum(v, smem);
+    if (threadIdx.x == 0) after_attempt(ctl, adaptive ? sqrt(tot / (double)nrows) : 0.0);
 }
 
 struct LinCoef {
@@ -1424,33 +1430,53 @@ struct OutArgs {
     // array: a per-thread index into a kernel-argument array becomes one
     // dependent kernarg load per element)
     const double *bi;
-    // nchunks == 1: every workgroup of the output grid arrives here once and
-    // the last one completes the output (rk_output_finish_kernel's work), so
-    // no second launch follows; zero between launches
+    // every workgroup of the output grid arrives here once and the last one
+    // completes the output; zero between launches
     unsigned int *arrivals;
 };
+
+// The fields of the control block the output kernels read, loaded at once
+// at the start of a kernel (one round trip).
+struct OutCtl {
+    int phase, cur, out_idx, ntimes;
+    double t_front, t_prev, dt_int;
+    const double *times;
+};
+
+__device__ __forceinline__ OutCtl out_ctl(const RKCtl *c) {
+    OutCtl o;
+    o.phase = c->phase;
+    o.cur = c->cur;
+    o.out_idx = c->out_idx;
+    o.ntimes = c->ntimes;
+    o.t_front = c->t_front;
+    o.t_prev = c->t_prev;
+    o.dt_int = c->dt_int;
+    o.times = c->times;
+    return o;
+}
 
 // Number of pending outputs (output times inside [t_prev, t_front]) and
 // their interpolation weights w[q][i] = dt_int * b_factor_i
 // (explicit_rk.pyx:412-430), computed by the whole workgroup into LDS;
 // front[q] = 1 when the output time equals t_front (the state is y_front).
-__device__ int pending_outputs(const RKCtl *c, const OutArgs &A, double (*w)[kMaxStages],
-                               int *front, int *nq_s) {
+__device__ int pending_outputs(const OutCtl &c, const OutArgs &A, double (*w)[kMaxStages],
+                               int *front) {
     // Thread (q, i) reads output time q itself, so all the reads are one
     // round trip; the count is the number of leading times inside the step.
     __shared__ int okq[kMaxPending];
     const int t = threadIdx.x;
     if (t < kMaxPending * kMaxStages) {
         const int q = t / kMaxStages, i = t % kMaxStages;
-        const int idx = c->out_idx + q;
-        const double t_front = c->t_front, t_prev = c->t_prev, dt_int = c->dt_int;
-        const bool in_range = idx < c->ntimes;
+        const int idx = c.out_idx + q;
+        const double t_front = c.t_front, t_prev = c.t_prev, dt_int = c.dt_int;
+        const bool in_range = idx < c.ntimes;
         // this thread's row of the dense-output table, every load issued at
         // once (they do not depend on the control block)
         double bv[kMaxDenseOrder];
 #pragma unroll
         for (int j = 0; j < kMaxDenseOrder; ++j) bv[j] = A.bi[i * kMaxDenseOrder + j];
-        const double tq = in_range ? c->times[idx] : 0.0;
+        const double tq = in_range ? c.times[idx] : 0.0;
         const bool ok = in_range && tq <= t_front;
         double v = 0.0;
         if (ok && tq != t_front && A.has_bi && i < A.nk) {
@@ -1470,30 +1496,30 @@ __device__ int pending_outputs(const RKCtl *c, const OutArgs &A, double (*w)[kMa
         }
     }
     __syncthreads();
-    if (t == 0) {
-        int nq = 0;
-        while (nq < kMaxPending && okq[nq]) ++nq;
-        *nq_s = nq;
-    }
-    __syncthreads();
-    return *nq_s;
+    int nq = 0;
+    while (nq < kMaxPending && okq[nq]) ++nq;
+    return nq;
 }
 
-// y(t_q)[idx]: y_front, or y_prev + sum_i w_i k_i[idx].  Every stage load is
-// unconditional (the stage index is clamped into [0, nk), so no load sits
-// behind a branch and all of them are in flight at once); stages without a
-// weight are dropped by a select, so what an unused slot holds never enters.
-__device__ __forceinline__ double2 state_at(const OutArgs &A, const double2 *yprev,
-                                            const double2 *yfront, const double *w, int front,
-                                            int64_t idx) {
-    if (front) return yfront[idx];
-    double2 o = yprev[idx];
-    double2 kv[kMaxStages];
+// The stage vectors at idx, every load unconditional (the stage index is
+// clamped into [0, nk), so no load sits behind a branch and all of them are
+// in flight at once).
+__device__ __forceinline__ void gather_stages(const OutArgs &A, int64_t idx,
+                                              double2 (&kv)[kMaxStages]) {
 #pragma unroll
     for (int i = 0; i < kMaxStages; ++i) {
         const int ii = i < A.nk ? i : A.nk - 1;
         kv[i] = A.K[(int64_t)ii * A.npad + idx];
     }
+}
+
+// y(t_q) from gathered values: y_front (yf), or y_prev (yp) + sum_i w_i k_i.
+// Stages without a weight are dropped by a select, so what an unused slot
+// holds never enters.
+__device__ __forceinline__ double2 interp(const OutArgs &A, const double2 (&kv)[kMaxStages],
+                                          double2 yp, double2 yf, const double *w, int front) {
+    if (front) return yf;
+    double2 o = yp;
 #pragma unroll
     for (int i = 0; i < kMaxStages; ++i) {
         const double wi = i < A.nk ? w[i] : 0.0;
@@ -1501,6 +1527,16 @@ __device__ __forceinline__ double2 state_at(const OutArgs &A, const double2 *ypr
         raxpy(o, wi, kk);
     }
     return o;
+}
+
+// y(t_q)[idx]: y_front, or y_prev + sum_i w_i k_i[idx].
+__device__ __forceinline__ double2 state_at(const OutArgs &A, const double2 *yprev,
+                                            const double2 *yfront, const double *w, int front,
+                                            int64_t idx) {
+    if (front) return yfront[idx];
+    double2 kv[kMaxStages];
+    gather_stages(A, idx, kv);
+    return interp(A, kv, yprev[idx], make_double2(0.0, 0.0), w, 0);
 }
 
 // Writes the states of the pending outputs (for store_states) and the state
@@ -1516,10 +1552,10 @@ __global__ __launch_bounds__(256) void rk_store_kernel(OutArgs A) {
     }
     __shared__ double w[kMaxPending][kMaxStages];
     __shared__ int front[kMaxPending];
-    __shared__ int nq_s;
-    const int nq = pending_outputs(c, A, w, front, &nq_s);
-    const double2 *yprev = c->cur ? A.y1 : A.y0;
-    const double2 *yfront = c->cur ? A.y0 : A.y1;
+    const OutCtl oc = out_ctl(c);
+    const int nq = pending_outputs(oc, A, w, front);
+    const double2 *yprev = oc.cur ? A.y1 : A.y0;
+    const double2 *yfront = oc.cur ? A.y0 : A.y1;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < A.npad;
          r += (int64_t)gridDim.x * blockDim.x) {
         for (int q = 0; q < nq; ++q) {
@@ -1555,94 +1591,162 @@ __device__ void finish_outputs(RKCtl *c, int ph, int nq, const int *front) {
 // Functionals of the pending output states: linear f = sum w y[col] (a
 // vectorised density matrix: tr(O rho), tr(rho)) or quadratic
 // f = sum w conj(y[row]) y[col] (a ket: <psi|O|psi>, ||psi||^2).  Grid
-// (nchunks, nfun, kMaxPending): one workgroup per chunk, functional and
-// pending output time (layers past the pending count skip the sum).  With
-// several chunks each workgroup writes the partial sum of its chunk, which
-// rk_output_finish_kernel adds up; with one chunk the sums are the values
-// and the last workgroup to finish completes the output.
-__global__ __launch_bounds__(256) void rk_output_kernel(OutArgs A) {
+// (nchunks, nfun) of one wavefront each: a workgroup takes kOutChunk
+// entries of one functional, one per lane, and sums them for every pending
+// output time (the stage gathers of an entry serve all of them).  Many small
+// workgroups, because the gathers (18 scattered lines per entry) are bound
+// by what one CU can request, not by HBM.  With one chunk per functional
+// the sums are the values; otherwise each workgroup writes its partial sum.
+// The last workgroup to arrive adds the partial sums in a fixed order and
+// completes the output (counters, next phase).
+//
+// Round trips: 1 the control block and the functional's range, 2 this
+// lane's entry beside the output times, 3 the entry's gathers; all are
+// issued before the weights are known.
+__global__ __launch_bounds__(kOutThreads) void rk_output_kernel(OutArgs A) {
     RKCtl *c = A.ctl;
-    // the functional's range does not depend on the control block: it is
-    // read in the same round trip
     const int f = blockIdx.y;
+    const int lane = threadIdx.x;
+    const OutCtl oc = out_ctl(c);
     const int fbeg = A.fptr[f], fend = A.fptr[f + 1];
     const int quad = A.fkind[f];
-    const int ph = c->phase;
-    // one chunk per functional: the values go straight to fun_out and the
-    // last workgroup to arrive completes the output (uniform over the grid)
-    const bool one_pass = A.nchunks == 1;
-    const bool active = ph == PH_DENSE || ph == PH_OUTPUT;
-    if (!active && !one_pass) return;
-    __shared__ double w[kMaxPending][kMaxStages];
-    __shared__ int front[kMaxPending];
-    __shared__ int nq_s;
-    __shared__ double smem[256 / QA_WAVE];
-    int nq = 0;
-    if (active) {
-        nq = pending_outputs(c, A, w, front, &nq_s);
-        const int q = blockIdx.z;  // one output time per workgroup layer
-        if (q < nq) {              // uniform over the workgroup
-            const double2 *yprev = c->cur ? A.y1 : A.y0;
-            const double2 *yfront = c->cur ? A.y0 : A.y1;
-            const int beg = fbeg + blockIdx.x * kOutChunk;
-            const int end = min(fend, beg + kOutChunk);
-            double2 acc = make_double2(0.0, 0.0);
-            for (int e = beg + threadIdx.x; e < end; e += blockDim.x) {
-                double2 v = cmul(A.fw[e], state_at(A, yprev, yfront, w[q], front[q], A.fcol[e]));
-                if (quad) {
-                    const double2 l = state_at(A, yprev, yfront, w[q], front[q], A.frow[e]);
-                    v = cmul(make_double2(l.x, -l.y), v);
-                }
-                acc = cadd(acc, v);
-            }
-            const double2 sq = block_sum2(acc, smem);
-            if (threadIdx.x == 0) {
-                if (!one_pass)
-                    A.partials[((int64_t)q * A.nfun + f) * A.nchunks + blockIdx.x] = sq;
-                else if (f < A.nfun)
-                    c->fun_out[(int64_t)(c->out_idx + q) * A.nfun + f] = sq;
-            }
-        }
-    }
-    if (!one_pass) return;
-    // Every workgroup arrives once, after all its threads' reads of the
-    // control block, so the last one to arrive may change it.  It resets the
-    // counter for the next launch.
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned total = gridDim.x * gridDim.y * gridDim.z;
-        if (atomicAdd(A.arrivals, 1u) == total - 1) {
-            __threadfence();
-            *A.arrivals = 0u;
-            if (active) finish_outputs(c, ph, nq, front);
-        }
-    }
-}
-
-// Completes an output launch: sums the partial sums of every (output time,
-// functional) pair in a fixed order, stores them and advances the control
-// block.  A separate one-workgroup kernel (the launch boundary orders it
-// after the partial sums) instead of a last-block handshake, whose
-// device-scope fence and atomic cost more than a launch.
-__global__ __launch_bounds__(256) void rk_output_finish_kernel(OutArgs A) {
-    RKCtl *c = A.ctl;
-    const int ph = c->phase;
+    const int ph = oc.phase;
+    // an inactive launch changes nothing, so every workgroup of it reads the
+    // same phase and leaves (the arrival counter stays at zero)
     if (ph != PH_DENSE && ph != PH_OUTPUT) return;
     __shared__ double w[kMaxPending][kMaxStages];
     __shared__ int front[kMaxPending];
-    __shared__ int nq_s;
-    const int nq = pending_outputs(c, A, w, front, &nq_s);
-    for (int p = threadIdx.x; p < nq * A.nfun; p += blockDim.x) {
-        const int q = p / A.nfun, ff = p % A.nfun;
-        double2 t = make_double2(0.0, 0.0);
-        for (int k = 0; k < A.nchunks; ++k)
-            t = cadd(t, A.partials[((int64_t)q * A.nfun + ff) * A.nchunks + k]);
-        c->fun_out[(int64_t)(c->out_idx + q) * A.nfun + ff] = t;
+    const double2 *yprev = oc.cur ? A.y1 : A.y0;
+    const double2 *yfront = oc.cur ? A.y0 : A.y1;
+    const int e = fbeg + blockIdx.x * kOutChunk + lane;
+    const bool has = lane < kOutChunk && e < fend;
+    const int ec = has ? e : 0;  // the entry arrays hold at least one element
+    int col = A.fcol[ec], row = A.frow[ec];
+    const double2 fw = A.fw[ec];
+    if (!has) col = row = 0;
+    double2 kc[kMaxStages], kr[kMaxStages];
+    gather_stages(A, col, kc);
+    const double2 pc = yprev[col], fc = yfront[col];
+    double2 pr = pc, fr = fc;
+    if (quad) {  // uniform over the workgroup
+        gather_stages(A, row, kr);
+        pr = yprev[row];
+        fr = yfront[row];
+    }
+    const int nq = pending_outputs(oc, A, w, front);
+    const bool one_pass = A.nchunks == 1;
+#pragma unroll
+    for (int q = 0; q < kMaxPending; ++q) {
+        if (q >= nq) break;  // uniform
+        double2 v = make_double2(0.0, 0.0);
+        if (has) {
+            v = cmul(fw, interp(A, kc, pc, fc, w[q], front[q]));
+            if (quad) {
+                const double2 l = interp(A, kr, pr, fr, w[q], front[q]);
+                v = cmul(make_double2(l.x, -l.y), v);
+            }
+        }
+        const double re = wave_sum(v.x), im = wave_sum(v.y);
+        if (lane == 0 && f < A.nfun) {
+            const double2 sq = make_double2(re, im);
+            if (one_pass)
+                c->fun_out[(int64_t)(oc.out_idx + q) * A.nfun + f] = sq;
+            else
+                A.partials[((int64_t)q * A.nfun + f) * A.nchunks + blockIdx.x] = sq;
+        }
+    }
+    // Every workgroup arrives once, after its reads of the control block
+    // have returned (their values are used above), so the last one to arrive
+    // may change it.  With one chunk nothing this kernel writes is read by
+    // another of its workgroups (the values go to the host, the control
+    // block to the next kernel across the launch boundary) and the arrival
+    // needs no fence (a device-scope fence is a write-back and invalidation
+    // of the L2); with several, the partial sums are released before the
+    // arrival and acquired by the last workgroup.  It resets the counter for
+    // the next launch.
+    __shared__ int last_s;
+    if (lane == 0) {
+        if (!one_pass) __threadfence();
+        const unsigned total = gridDim.x * gridDim.y;
+        const bool last = __hip_atomic_fetch_add(A.arrivals, 1u, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) == total - 1;
+        if (last) {
+            __hip_atomic_exchange(A.arrivals, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!one_pass) __threadfence();
+        }
+        last_s = last;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    finish_outputs(c, ph, nq, front);
+    if (!last_s) return;
+    if (!one_pass) {
+        // each value: the partial sums strided over the lanes, then the wave
+        // tree; the values go to LDS first, so no store sits between the
+        // loads of one value and the next (four of them are in flight)
+        __shared__ double2 vals[kOutThreads];
+        const int npairs = nq * A.nfun;
+        for (int p0 = 0; p0 < npairs; p0 += kOutThreads) {
+#pragma unroll 4
+            for (int p = p0; p < min(npairs, p0 + kOutThreads); ++p) {
+                const double2 *part = A.partials + (int64_t)p * A.nchunks;  // p = q nfun + f
+                double re = 0.0, im = 0.0;
+                for (int k = lane; k < A.nchunks; k += kOutThreads) {
+                    re += part[k].x;
+                    im += part[k].y;
+                }
+                re = wave_sum(re);
+                im = wave_sum(im);
+                if (lane == 0) vals[p - p0] = make_double2(re, im);
+            }
+            __syncthreads();
+            const int p = p0 + lane;
+            if (p < npairs) {
+                const int q = p / A.nfun, ff = p % A.nfun;
+                c->fun_out[(int64_t)(oc.out_idx + q) * A.nfun + ff] = vals[lane];
+            }
+            __syncthreads();
+        }
+    }
+    if (lane == 0) finish_outputs(c, ph, nq, front);
+}
+
+// The state at the last output time of a finished run into y_last, for a
+// run whose iterations carry no rk_store_kernel: after the last output
+// nothing moves the control block, the stage vectors or y_prev / y_front,
+// so this is the state rk_store_kernel would have written (the same
+// weights and the same sums).
+__global__ __launch_bounds__(256) void rk_final_state_kernel(OutArgs A) {
+    RKCtl *c = A.ctl;
+    __shared__ double w[kMaxStages];
+    __shared__ int front_s;
+    const int tid = threadIdx.x;
+    const RKCtl &c0 = *c;
+    if (c0.phase != PH_DONE || c0.out_idx != c0.ntimes || c0.ntimes < 1) return;
+    if (tid < kMaxStages) {
+        double bv[kMaxDenseOrder];
+#pragma unroll
+        for (int j = 0; j < kMaxDenseOrder; ++j) bv[j] = A.bi[tid * kMaxDenseOrder + j];
+        const double tq = c0.times[c0.ntimes - 1];
+        double v = 0.0;
+        if (tq != c0.t_front && A.has_bi && tid < A.nk) {
+            const double tau = (tq - c0.t_prev) / c0.dt_int;
+#pragma unroll
+            for (int j = kMaxDenseOrder - 1; j >= 0; --j)
+                if (j < A.dorder) {
+                    v += bv[j];
+                    v *= tau;
+                }
+            v *= c0.dt_int;
+        }
+        w[tid] = v;
+        if (tid == 0) front_s = tq == c0.t_front;
+    }
+    __syncthreads();
+    const double2 *yprev = c0.cur ? A.y1 : A.y0;
+    const double2 *yfront = c0.cur ? A.y0 : A.y1;
+    const int front = front_s;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + tid; r < A.npad;
+         r += (int64_t)gridDim.x * blockDim.x)
+        A.y_last[r] = state_at(A, yprev, yfront, w, front, r);
 }
 
 // Start of a run over new output times.
@@ -1728,6 +1832,10 @@ struct qa_rk_solver {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_dirty = true;
+    // whether the iteration carries rk_store_kernel: a run that stores
+    // states does; otherwise rk_final_state_kernel writes y_last once at
+    // the end of the run.  graph_store is the form the graph was captured in.
+    bool run_store = true, graph_store = true;
     bool use_graph = true;  // QUTIP_AMD_GRAPH=0: launch the iteration directly
     RKCtl *ctl_host = nullptr;  // pinned mirror
     long long graph_launches = 0;
@@ -2011,8 +2119,9 @@ OutArgs out_args(qa_rk_solver *S) {
     return A;
 }
 
-// Enqueue one loop iteration (used both for graph capture and direct launch).
-int enqueue_iteration(qa_rk_solver *S) {
+// Enqueue one loop iteration (used both for graph capture and direct launch);
+// with_store: include rk_store_kernel (stored states, or y_last per output).
+int enqueue_iteration(qa_rk_solver *S, bool with_store) {
     int rc;
     for (int i = 0; i < S->rk_step; ++i) {
         rc = launch_stage(S, i);
@@ -2039,16 +2148,14 @@ int enqueue_iteration(qa_rk_solver *S) {
         }
     }
     OutArgs A = out_args(S);
-    unsigned sblocks = (unsigned)std::min<int64_t>((S->npad + 255) / 256, 2048);
-    hipLaunchKernelGGL(rk_store_kernel, dim3(sblocks), dim3(256), 0, S->stream, A);
-    QA_CHECK_LAUNCH();
-    hipLaunchKernelGGL(rk_output_kernel, dim3(S->nchunks, std::max(S->nfun, 1), kMaxPending),
-                       dim3(256), 0, S->stream, A);
-    QA_CHECK_LAUNCH();
-    if (S->nchunks > 1) {  // one chunk: rk_output_kernel completes the output itself
-        hipLaunchKernelGGL(rk_output_finish_kernel, dim3(1), dim3(256), 0, S->stream, A);
+    if (with_store) {
+        unsigned sblocks = (unsigned)std::min<int64_t>((S->npad + 255) / 256, 2048);
+        hipLaunchKernelGGL(rk_store_kernel, dim3(sblocks), dim3(256), 0, S->stream, A);
         QA_CHECK_LAUNCH();
     }
+    hipLaunchKernelGGL(rk_output_kernel, dim3(S->nchunks, std::max(S->nfun, 1)),
+                       dim3(kOutThreads), 0, S->stream, A);
+    QA_CHECK_LAUNCH();
     return QA_OK;
 }
 
@@ -2062,7 +2169,7 @@ int rebuild_graph(qa_rk_solver *S) {
         S->graph = nullptr;
     }
     QA_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_iteration(S);
+    int rc = enqueue_iteration(S, S->run_store);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(S->stream, &g);
     if (rc) {
@@ -2073,6 +2180,7 @@ int rebuild_graph(qa_rk_solver *S) {
     S->graph = g;
     QA_HIP(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
     S->graph_dirty = false;
+    S->graph_store = S->run_store;
     return QA_OK;
 }
 
@@ -2828,7 +2936,8 @@ int qa_rk_begin(qa_rk_solver *S, int ntimes, const double *times, int single_ste
                           sizeof(int), hipMemcpyHostToDevice, s));
     // the sources above are pageable host memory: complete the copies here
     QA_HIP(hipStreamSynchronize(s));
-    if (S->use_graph && S->graph_dirty) {
+    S->run_store = states_out != nullptr;
+    if (S->use_graph && (S->graph_dirty || S->graph_store != S->run_store)) {
         int rc = rebuild_graph(S);
         if (rc) return rc;
     }
@@ -2873,6 +2982,12 @@ int qa_rk_advance(qa_rk_solver *S, int *done) {
         S->in_flight = false;
         const RKCtl &h = *S->ctl_host;
         if (h.phase == PH_DONE || h.phase == PH_IDLE) {
+            if (!S->run_store) {  // y_last, which rk_store_kernel did not write
+                OutArgs A = out_args(S);
+                const unsigned blocks = (unsigned)std::min<int64_t>((S->npad + 255) / 256, 2048);
+                hipLaunchKernelGGL(rk_final_state_kernel, dim3(blocks), dim3(256), 0, s, A);
+                QA_CHECK_LAUNCH();
+            }
             S->run_state = 2;
             *done = 1;
             return QA_OK;
@@ -2898,7 +3013,7 @@ int qa_rk_advance(qa_rk_solver *S, int *done) {
         if (S->use_graph) {
             QA_HIP(hipGraphLaunch(S->exec, s));
         } else {
-            int rc = enqueue_iteration(S);
+            int rc = enqueue_iteration(S, S->run_store);
             if (rc) return rc;
         }
     }
@@ -3106,7 +3221,7 @@ int qa_rk_profile_sequence(qa_rk_solver *S, int reps, double *ms_step, double *m
 int qa_rk_iterate(qa_rk_solver *S, int count) {
     QA_REQUIRE(S != nullptr && count >= 0, "qa_rk_iterate: bad arguments");
     for (int k = 0; k < count; ++k) {
-        int rc = enqueue_iteration(S);
+        int rc = enqueue_iteration(S, true);
         if (rc) return rc;
     }
     QA_HIP(hipStreamSynchronize(S->stream));
