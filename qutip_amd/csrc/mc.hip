@@ -99,6 +99,7 @@ struct McArgs {
     double2 *fun_out;  // [nsys][ntimes][nfun]
     double2 *y_final;  // [nsys][d] or null
     int fun_lds;       // LDS bytes reserved for the functionals and times (0: none)
+    int x2;            // a second LDS copy of the product's input vector is reserved
 };
 
 template <typename R>
@@ -152,6 +153,7 @@ constexpr int kRegStagesMax = 10;
 template <typename R>
 struct McLds {
     C2<R> *x;      // [npad]
+    C2<R> *xalt;   // [npad] the other input buffer, or null (one buffer)
     C2<R> *kd;     // [kMcStages - RS][npad]
     C2<R> *kfsal;  // [npad]
     double *red;   // [2 * threads / 64]
@@ -260,9 +262,21 @@ __device__ __forceinline__ double2 bsum2(Ctx<R, NR, RS> &c, double a, double b) 
     return make_double2(ta, tb);
 }
 
+// The input of a product (or of the functionals) into LDS, for every thread
+// to gather from.  With two buffers the writes go to the one not read last,
+// so only the barrier after them is needed: a thread still gathering from
+// the other buffer has not reached that barrier, and every read of a
+// buffer lies between the barrier after its writes and the barrier after
+// the next input's, before the buffer is written again.
 template <typename R, int NR, int RS>
 __device__ __forceinline__ void stage_in(Ctx<R, NR, RS> &c, const V<R, NR> &x) {
-    __syncthreads();
+    if (c.S.xalt) {  // uniform
+        C2<R> *t = c.S.x;
+        c.S.x = c.S.xalt;
+        c.S.xalt = t;
+    } else {
+        __syncthreads();
+    }
 #pragma unroll
     for (int q = 0; q < NR; ++q) c.S.x[c.row[q]] = x.v[q];
     __syncthreads();
@@ -622,6 +636,7 @@ __device__ __forceinline__ void kernel_prologue(const McArgs &A, unsigned char *
         S.red = (double *)p;
         p += sizeof(double) * 2 * (nt / QA_WAVE);
         S.probs = (double *)p;
+        S.xalt = nullptr;
     }
     {
         const double *src = (const double *)A.tab;
@@ -685,9 +700,14 @@ __device__ __forceinline__ void integrate_to(Ctx<R, NR, RS> &c, Traj<R, NR, RS> 
     }
 }
 
+// p rounded up to 16 bytes as an offset from p itself, so the result keeps
+// p's provenance: a pointer into the LDS stays one the compiler knows to be
+// in the LDS (DS instructions, not FLAT ones, which wait on both the memory
+// and the LDS counters).
 template <typename T>
 __device__ __forceinline__ unsigned char *align16(T *p) {
-    return (unsigned char *)(((uintptr_t)p + 15) & ~(uintptr_t)15);
+    unsigned char *b = (unsigned char *)p;
+    return b + ((16 - ((uintptr_t)b & 15)) & 15);
 }
 
 // The functionals as the whole-solve kernel reads them: kind, CSR-style
@@ -748,6 +768,7 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj
         S.red = (double *)p;
         p += sizeof(double) * 2 * (nt / QA_WAVE);
         S.probs = (double *)p;
+        S.xalt = nullptr;
     }
     {
         const double *src = (const double *)A.tab;
@@ -928,6 +949,10 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void rk_smal
     const int *lcol = nullptr;
     const C2<R> *lval = nullptr;
     unsigned char *tail = align16(S.probs + kMcMaxChan);
+    if (A.x2) {  // the launcher reserved the second input buffer here
+        S.xalt = (C2<R> *)tail;
+        tail += sizeof(C2<R>) * (size_t)S.npad;
+    }
     if constexpr (HL) {
         const int nent = A.h_width * A.d;
         C2<R> *v = (C2<R> *)tail;
@@ -1034,6 +1059,7 @@ struct McPlan {
     size_t lds;   // dynamic LDS bytes per workgroup
     bool hl;      // the shared ELL operator is staged in LDS
     int fun_lds;  // bytes for the functionals and times (0: none)
+    bool x2;      // a second input buffer (one barrier per product instead of two)
 };
 
 // RS == 0 runs two workgroups per CU, so each gets half of the CU's LDS,
@@ -1048,6 +1074,7 @@ McPlan plan_mc(const McArgs &A, int threads, int NR, int RS, bool small) {
     P.lds = mc_lds_bytes<R>(threads * NR, threads, RS);
     P.hl = false;
     P.fun_lds = 0;
+    P.x2 = false;
     const size_t cap = mc_lds_cap(RS);
     if (small) {
         if (A.h_stride == 0 && A.h_width > 0) {
@@ -1057,6 +1084,14 @@ McPlan plan_mc(const McArgs &A, int threads, int NR, int RS, bool small) {
                 P.lds += ell;
                 P.hl = true;
             }
+        }
+        // the second input buffer (with 16 bytes for its alignment) when it
+        // fits and keeps the number of workgroups an LDS-limited CU holds
+        const size_t xb = 16 + sizeof(C2<R>) * (size_t)threads * NR;
+        if (P.lds + xb <= cap &&
+            kLdsBudget / (P.lds + xb + kLdsStatic) == kLdsBudget / (P.lds + kLdsStatic)) {
+            P.lds += xb;
+            P.x2 = true;
         }
         if (A.nfun > 0 || A.ntimes > 0) {
             // room for the functionals and times, up to kFunLdsMax, that
@@ -1084,6 +1119,7 @@ int launch_mc_cfg(const McArgs &A0, int threads, int blocks, hipStream_t s) {
         return QA_ERR_ARG;
     }
     A.fun_lds = P.fun_lds;
+    A.x2 = P.x2 ? 1 : 0;
     const size_t lds = P.lds;
     const bool hl = P.hl;
     const void *fn = !SMALL ? (const void *)mc_traj_kernel<R, NR, RS>
