@@ -494,6 +494,22 @@ def mc_probe(reps=2):
         print(f"mc probe {prec}: {reps} launches", flush=True)
 
 
+def jc_probe(reps=6):
+    """Only the C1 single solves (Jaynes-Cummings mesolve through
+    MESolver.run, one whole-solve launch each): the command the whole-solve
+    kernel's counters are read on."""
+    import qutip_amd as qa
+    from qutip_amd import _lib
+    from tests import systems
+    s = systems.jaynes_cummings()
+    tl = np.load(ROOT / "tests" / "golden" / "jc_mesolve.npz")["tlist"]
+    solver = qa.MESolver(s["H"], s["c_ops"], options={"rtol": 1e-8, "atol": 1e-10})
+    for _ in range(reps):
+        solver.run(s["psi0"], tl, e_ops=s["e_ops"])
+    _lib.call("qa_device_sync")
+    print(f"jc probe: {reps} solves, {solver._integrator.stats()}", flush=True)
+
+
 def dense_rhs_probe(N, nc, prec="fp32"):
     """Time the matrix-form Lindblad RHS alone (stage kernels in solve order,
     qa_rk_profile_sequence) for a random N x N system with nc dense collapse
@@ -627,9 +643,14 @@ def main():
                     help="only time the matrix-form RHS at size N with NC collapse operators")
     ap.add_argument("--mc-probe", action="store_true",
                     help="only run the batched mcsolve launches the trajectory counters are read on")
+    ap.add_argument("--jc-probe", action="store_true",
+                    help="only run the C1 single solves the whole-solve kernel's counters are read on")
     args = ap.parse_args()
     if args.mc_probe:
         mc_probe()
+        return
+    if args.jc_probe:
+        jc_probe()
         return
     if args.pmc:
         pmc_workload()
