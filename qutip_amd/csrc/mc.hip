@@ -156,15 +156,19 @@ struct McLds {
     C2<R> *xalt;   // [npad] the other input buffer, or null (one buffer)
     C2<R> *kd;     // [kMcStages - RS][npad]
     C2<R> *kfsal;  // [npad]
-    double *red;   // [2 * threads / 64]
+    double *red;   // [2 * kFunGroup * threads / 64]
     double *probs; // [kMcMaxChan]
     int npad;
 };
 
+// Functionals summed per block reduction (one pair of barriers for each
+// group) by the whole-solve kernel.
+constexpr int kFunGroup = 4;
+
 template <typename R>
 inline size_t mc_lds_bytes(int npad, int threads, int RS) {
     return sizeof(C2<R>) * (size_t)npad * (2 + kMcStages - RS) +
-           sizeof(double) * (2 * (threads / 64) + kMcMaxChan) + 64;
+           sizeof(double) * (2 * kFunGroup * (threads / 64) + kMcMaxChan) + 64;
 }
 
 // A per-row value: each thread owns NR rows (r = threadIdx.x + q * blockDim.x).
@@ -634,7 +638,7 @@ __device__ __forceinline__ void kernel_prologue(const McArgs &A, unsigned char *
         S.kfsal = (C2<R> *)p;
         p += sizeof(C2<R>) * npad;
         S.red = (double *)p;
-        p += sizeof(double) * 2 * (nt / QA_WAVE);
+        p += sizeof(double) * 2 * kFunGroup * (nt / QA_WAVE);
         S.probs = (double *)p;
         S.xalt = nullptr;
     }
@@ -725,27 +729,54 @@ template <typename R, int NR, int RS>
 __device__ __forceinline__ void write_functionals(Ctx<R, NR, RS> &c, const V<R, NR> &y,
                                                   const FunView &fv, double2 *out) {
     stage_in(c, y);
-    for (int f = 0; f < c.A.nfun; ++f) {
-        const int quad = fv.kind[f];
-        const int e1 = fv.ptr[f + 1];
-        double re = 0.0, im = 0.0;
-        for (int e = fv.ptr[f] + threadIdx.x; e < e1; e += blockDim.x) {
-            const C2<R> xc = c.S.x[fv.col[e]];
-            const double2 w = fv.w[e];
-            double vr = w.x * (double)xc.x - w.y * (double)xc.y;
-            double vi = w.x * (double)xc.y + w.y * (double)xc.x;
-            if (quad) {
-                const C2<R> xr = c.S.x[fv.row[e]];
-                const double lr = (double)xr.x, li = -(double)xr.y;
-                const double tr = lr * vr - li * vi;
-                vi = lr * vi + li * vr;
-                vr = tr;
+    const int lane = threadIdx.x & (QA_WAVE - 1), wid = threadIdx.x / QA_WAVE;
+    const int nw = blockDim.x / QA_WAVE;
+    // kFunGroup functionals per block reduction: each wave's sums go to LDS
+    // as they are formed (no array of them in registers), then one barrier,
+    // and thread t < kFunGroup adds functional f0 + t over the waves in
+    // order, which is bsum2's order, so the bits are bsum2's.  The barrier
+    // after stage_in's writes (first group) or at the top of the loop (the
+    // previous group's sums read) comes before a wave writes LDS sums.
+    for (int f0 = 0; f0 < c.A.nfun; f0 += kFunGroup) {
+        if (f0 > 0) __syncthreads();
+#pragma unroll 1
+        for (int g = 0; g < kFunGroup; ++g) {
+            const int f = f0 + g;
+            if (f >= c.A.nfun) break;  // uniform
+            const int quad = fv.kind[f];
+            const int e1 = fv.ptr[f + 1];
+            double re = 0.0, im = 0.0;
+            for (int e = fv.ptr[f] + threadIdx.x; e < e1; e += blockDim.x) {
+                const C2<R> xc = c.S.x[fv.col[e]];
+                const double2 w = fv.w[e];
+                double vr = w.x * (double)xc.x - w.y * (double)xc.y;
+                double vi = w.x * (double)xc.y + w.y * (double)xc.x;
+                if (quad) {
+                    const C2<R> xr = c.S.x[fv.row[e]];
+                    const double lr = (double)xr.x, li = -(double)xr.y;
+                    const double tr = lr * vr - li * vi;
+                    vi = lr * vi + li * vr;
+                    vr = tr;
+                }
+                re += vr;
+                im += vi;
             }
-            re += vr;
-            im += vi;
+            re = wave_sum(re);
+            im = wave_sum(im);
+            if (lane == 0) {
+                c.S.red[(2 * g) * nw + wid] = re;
+                c.S.red[(2 * g + 1) * nw + wid] = im;
+            }
         }
-        const double2 r = bsum2(c, re, im);
-        if (threadIdx.x == 0) out[f] = r;
+        __syncthreads();
+        if (threadIdx.x < kFunGroup && f0 + (int)threadIdx.x < c.A.nfun) {
+            double ta = 0.0, tb = 0.0;
+            for (int w = 0; w < nw; ++w) {
+                ta += c.S.red[2 * threadIdx.x * nw + w];
+                tb += c.S.red[(2 * threadIdx.x + 1) * nw + w];
+            }
+            out[f0 + threadIdx.x] = make_double2(ta, tb);
+        }
     }
 }
 
@@ -766,7 +797,7 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj
         S.kfsal = (C2<R> *)p;
         p += sizeof(C2<R>) * npad;
         S.red = (double *)p;
-        p += sizeof(double) * 2 * (nt / QA_WAVE);
+        p += sizeof(double) * 2 * kFunGroup * (nt / QA_WAVE);
         S.probs = (double *)p;
         S.xalt = nullptr;
     }
