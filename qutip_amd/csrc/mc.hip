@@ -158,6 +158,8 @@ struct McLds {
     C2<R> *kfsal;  // [npad]
     double *red;   // [2 * kFunGroup * threads / 64]
     double *probs; // [kMcMaxChan]
+    double *bfs;   // [2][kMcStages] dense-output weights (two buffers, alternating)
+    int bsel;      // the buffer the next interpolation writes
     int npad;
 };
 
@@ -168,7 +170,7 @@ constexpr int kFunGroup = 4;
 template <typename R>
 inline size_t mc_lds_bytes(int npad, int threads, int RS) {
     return sizeof(C2<R>) * (size_t)npad * (2 + kMcStages - RS) +
-           sizeof(double) * (2 * kFunGroup * (threads / 64) + kMcMaxChan) + 64;
+           sizeof(double) * (2 * kFunGroup * (threads / 64) + kMcMaxChan + 2 * kMcStages) + 64;
 }
 
 // A per-row value: each thread owns NR rows (r = threadIdx.x + q * blockDim.x).
@@ -417,22 +419,39 @@ __device__ __forceinline__ void prep_dense_out(Ctx<R, NR, RS> &c, Traj<R, NR, RS
     }
 }
 
+// The weights b_i(tau) (explicit_rk.pyx:412-430) are formed by threads
+// i < kMcStages, each with its row of the table loaded at once (a loop to
+// the run-time order would wait on every load before the next step), and
+// shared through LDS.  Two weight buffers alternate, so one barrier per
+// interpolation suffices: a buffer is written again only after the barrier
+// of the interpolation in between, which every thread passes after its
+// reads of it.
 template <typename R, int NR, int RS>
 __device__ __forceinline__ V<R, NR> interpolate(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, double t) {
     const McTableau &T = c.T;
-    const double tau = (t - s.t_prev) / s.dt_int;
-    double bf[kMcStages];
+    double *bw = c.S.bfs + kMcStages * c.S.bsel;
+    c.S.bsel ^= 1;
+    const int i = threadIdx.x;
+    if (i < kMcStages) {
+        const double tau = (t - s.t_prev) / s.dt_int;
+        double bv[kMcOrder];
 #pragma unroll
-    for (int i = 0; i < kMcStages; ++i) {
+        for (int j = 0; j < kMcOrder; ++j) bv[j] = T.bi[i][j];
         double v = 0.0;
         if (i < T.rk_extra) {
-            for (int j = T.dorder - 1; j >= 0; --j) {
-                v += T.bi[i][j];
-                v *= tau;
-            }
+#pragma unroll
+            for (int j = kMcOrder - 1; j >= 0; --j)
+                if (j < T.dorder) {  // Horner from j = dorder - 1 down
+                    v += bv[j];
+                    v *= tau;
+                }
         }
-        bf[i] = v;
+        bw[i] = v;
     }
+    __syncthreads();
+    double bf[kMcStages];
+#pragma unroll
+    for (int k = 0; k < kMcStages; ++k) bf[k] = bw[k];
     return combine(c, s, s.yprev, bf, T.rk_extra, s.dt_int);
 }
 
@@ -640,6 +659,9 @@ __device__ __forceinline__ void kernel_prologue(const McArgs &A, unsigned char *
         S.red = (double *)p;
         p += sizeof(double) * 2 * kFunGroup * (nt / QA_WAVE);
         S.probs = (double *)p;
+        p += sizeof(double) * kMcMaxChan;
+        S.bfs = (double *)p;
+        S.bsel = 0;
         S.xalt = nullptr;
     }
     {
@@ -799,6 +821,9 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj
         S.red = (double *)p;
         p += sizeof(double) * 2 * kFunGroup * (nt / QA_WAVE);
         S.probs = (double *)p;
+        p += sizeof(double) * kMcMaxChan;
+        S.bfs = (double *)p;
+        S.bsel = 0;
         S.xalt = nullptr;
     }
     {
@@ -979,7 +1004,7 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void rk_smal
     Ctx<R, NR, RS> c{A, S, T};
     const int *lcol = nullptr;
     const C2<R> *lval = nullptr;
-    unsigned char *tail = align16(S.probs + kMcMaxChan);
+    unsigned char *tail = align16(S.bfs + 2 * kMcStages);
     if (A.x2) {  // the launcher reserved the second input buffer here
         S.xalt = (C2<R> *)tail;
         tail += sizeof(C2<R>) * (size_t)S.npad;
