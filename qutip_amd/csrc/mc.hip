@@ -824,7 +824,8 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj
         p += sizeof(double) * kMcMaxChan;
         S.bfs = (double *)p;
         S.bsel = 0;
-        S.xalt = nullptr;
+        // the second input buffer, when the launcher reserved it
+        S.xalt = A.x2 ? (C2<R> *)align16(S.bfs + 2 * kMcStages) : nullptr;
     }
     {
         const double *src = (const double *)A.tab;
@@ -1132,15 +1133,15 @@ McPlan plan_mc(const McArgs &A, int threads, int NR, int RS, bool small) {
     P.fun_lds = 0;
     P.x2 = false;
     const size_t cap = mc_lds_cap(RS);
-    if (small) {
-        if (A.h_stride == 0 && A.h_width > 0) {
-            // the shared operator in LDS when it fits the budget of this shape
-            const size_t ell = (size_t)A.h_width * A.d * (sizeof(C2<R>) + sizeof(int));
-            if (P.lds + ell <= cap) {
-                P.lds += ell;
-                P.hl = true;
-            }
+    if (small && A.h_stride == 0 && A.h_width > 0) {
+        // the shared operator in LDS when it fits the budget of this shape
+        const size_t ell = (size_t)A.h_width * A.d * (sizeof(C2<R>) + sizeof(int));
+        if (P.lds + ell <= cap) {
+            P.lds += ell;
+            P.hl = true;
         }
+    }
+    {
         // the second input buffer (with 16 bytes for its alignment) when it
         // fits and keeps the number of workgroups an LDS-limited CU holds
         const size_t xb = 16 + sizeof(C2<R>) * (size_t)threads * NR;
@@ -1149,6 +1150,8 @@ McPlan plan_mc(const McArgs &A, int threads, int NR, int RS, bool small) {
             P.lds += xb;
             P.x2 = true;
         }
+    }
+    if (small) {
         if (A.nfun > 0 || A.ntimes > 0) {
             // room for the functionals and times, up to kFunLdsMax, that
             // keeps the number of workgroups an LDS-limited CU holds
