@@ -233,39 +233,11 @@ __device__ __forceinline__ void kset(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, int 
     }
 }
 
-// A double moved across lanes by the DPP pattern C (both 32-bit halves).
-template <int C>
-__device__ __forceinline__ double dpp_move(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), C, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), C, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ double lane_value(double v, int lane) {
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
-                            __builtin_amdgcn_readlane(__double2loint(v), lane));
-}
-
-// Sum over the 64 lanes of a wavefront (every lane active), the result in
-// every lane.  Within each row of 16 lanes, DPP moves (lane xor 1, xor 2,
-// the half-row mirror, the row mirror) leave the row's sum in all of its
-// lanes; the four row sums are then read from lanes 0, 16, 32 and 48 and
-// added, (r0 + r1) + (r2 + r3).  A fixed order, so deterministic; these
-// register moves replace the six LDS-crossbar exchanges per component of a
-// butterfly built on shuffles.
-__device__ __forceinline__ double wave_sum_rows(double v) {
-    v += dpp_move<0xB1>(v);   // quad_perm [1, 0, 3, 2]
-    v += dpp_move<0x4E>(v);   // quad_perm [2, 3, 0, 1]
-    v += dpp_move<0x141>(v);  // row_half_mirror
-    v += dpp_move<0x140>(v);  // row_mirror
-    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
-}
-
 template <typename R, int NR, int RS>
 __device__ __forceinline__ double bsum(Ctx<R, NR, RS> &c, double v) {
     const int lane = threadIdx.x & (QA_WAVE - 1), wid = threadIdx.x / QA_WAVE;
     const int nw = blockDim.x / QA_WAVE;
-    v = wave_sum_rows(v);
+    v = wave_sum(v);
     __syncthreads();
     if (lane == 0) c.S.red[wid] = v;
     __syncthreads();
@@ -280,8 +252,8 @@ template <typename R, int NR, int RS>
 __device__ __forceinline__ double2 bsum2(Ctx<R, NR, RS> &c, double a, double b) {
     const int lane = threadIdx.x & (QA_WAVE - 1), wid = threadIdx.x / QA_WAVE;
     const int nw = blockDim.x / QA_WAVE;
-    a = wave_sum_rows(a);
-    b = wave_sum_rows(b);
+    a = wave_sum(a);
+    b = wave_sum(b);
     __syncthreads();
     if (lane == 0) {
         c.S.red[wid] = a;
@@ -811,8 +783,8 @@ __device__ __forceinline__ void write_functionals(Ctx<R, NR, RS> &c, const V<R, 
                 re += vr;
                 im += vi;
             }
-            re = wave_sum_rows(re);
-            im = wave_sum_rows(im);
+            re = wave_sum(re);
+            im = wave_sum(im);
             if (lane == 0) {
                 c.S.red[(2 * g) * nw + wid] = re;
                 c.S.red[(2 * g + 1) * nw + wid] = im;

@@ -45,10 +45,33 @@ __device__ __forceinline__ double2 ld(const float2 *p, int64_t i) {
     return make_double2((double)v.x, (double)v.y);
 }
 
+// A double moved across lanes by the DPP pattern C (both 32-bit halves).
+template <int C>
+__device__ __forceinline__ double dpp_move(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), C, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), C, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_value(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+// Sum over the 64 lanes of a wavefront (every lane active), the result in
+// every lane.  Within each row of 16 lanes, DPP moves (lane xor 1, xor 2,
+// the half-row mirror, the row mirror) leave the row's sum in all of its
+// lanes (each addition is commutative, so every lane holds the same bits);
+// the four row sums are then read from lanes 0, 16, 32 and 48 and added,
+// (r0 + r1) + (r2 + r3).  A fixed order, so deterministic; register moves
+// instead of the six LDS-crossbar exchanges per 32-bit half of a shuffle
+// butterfly.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = QA_WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, QA_WAVE);
-    return v;
+    v += dpp_move<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_move<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_move<0x141>(v);  // row_half_mirror
+    v += dpp_move<0x140>(v);  // row_mirror
+    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 
 // Sum over the workgroup; every thread gets the result.  `smem` must hold
