@@ -160,6 +160,8 @@ struct McLds {
     double *probs; // [kMcMaxChan]
     double *bfs;   // [2][kMcStages] dense-output weights (two buffers, alternating)
     int bsel;      // the buffer the next interpolation writes
+    double *rsum;  // [2][2 * threads / 64] block-sum partials (two buffers, alternating)
+    int rsel;      // the buffer the next block sum writes
     int npad;
 };
 
@@ -170,7 +172,8 @@ constexpr int kFunGroup = 4;
 template <typename R>
 inline size_t mc_lds_bytes(int npad, int threads, int RS) {
     return sizeof(C2<R>) * (size_t)npad * (2 + kMcStages - RS) +
-           sizeof(double) * (2 * kFunGroup * (threads / 64) + kMcMaxChan + 2 * kMcStages) + 64;
+           sizeof(double) * ((2 * kFunGroup + 4) * (threads / 64) + kMcMaxChan + 2 * kMcStages) +
+           64;
 }
 
 // A per-row value: each thread owns NR rows (r = threadIdx.x + q * blockDim.x).
@@ -233,16 +236,21 @@ __device__ __forceinline__ void kset(Ctx<R, NR, RS> &c, Traj<R, NR, RS> &s, int 
     }
 }
 
+// Block sums write their wave partials to one of two LDS buffers in turn,
+// so one barrier per sum suffices: a buffer is written again only after the
+// barrier of the sum in between, which every thread passes after its reads
+// of it (the buffers serve nothing else).
 template <typename R, int NR, int RS>
 __device__ __forceinline__ double bsum(Ctx<R, NR, RS> &c, double v) {
     const int lane = threadIdx.x & (QA_WAVE - 1), wid = threadIdx.x / QA_WAVE;
     const int nw = blockDim.x / QA_WAVE;
+    double *rb = c.S.rsum + 2 * nw * c.S.rsel;
+    c.S.rsel ^= 1;
     v = wave_sum(v);
-    __syncthreads();
-    if (lane == 0) c.S.red[wid] = v;
+    if (lane == 0) rb[wid] = v;
     __syncthreads();
     double t = 0.0;
-    for (int w = 0; w < nw; ++w) t += c.S.red[w];
+    for (int w = 0; w < nw; ++w) t += rb[w];
     return t;
 }
 
@@ -252,18 +260,19 @@ template <typename R, int NR, int RS>
 __device__ __forceinline__ double2 bsum2(Ctx<R, NR, RS> &c, double a, double b) {
     const int lane = threadIdx.x & (QA_WAVE - 1), wid = threadIdx.x / QA_WAVE;
     const int nw = blockDim.x / QA_WAVE;
+    double *rb = c.S.rsum + 2 * nw * c.S.rsel;
+    c.S.rsel ^= 1;
     a = wave_sum(a);
     b = wave_sum(b);
-    __syncthreads();
     if (lane == 0) {
-        c.S.red[wid] = a;
-        c.S.red[nw + wid] = b;
+        rb[wid] = a;
+        rb[nw + wid] = b;
     }
     __syncthreads();
     double ta = 0.0, tb = 0.0;
     for (int w = 0; w < nw; ++w) {
-        ta += c.S.red[w];
-        tb += c.S.red[nw + w];
+        ta += rb[w];
+        tb += rb[nw + w];
     }
     return make_double2(ta, tb);
 }
@@ -662,6 +671,9 @@ __device__ __forceinline__ void kernel_prologue(const McArgs &A, unsigned char *
         p += sizeof(double) * kMcMaxChan;
         S.bfs = (double *)p;
         S.bsel = 0;
+        p += sizeof(double) * 2 * kMcStages;
+        S.rsum = (double *)p;
+        S.rsel = 0;
         S.xalt = nullptr;
     }
     {
@@ -824,8 +836,11 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void mc_traj
         p += sizeof(double) * kMcMaxChan;
         S.bfs = (double *)p;
         S.bsel = 0;
+        p += sizeof(double) * 2 * kMcStages;
+        S.rsum = (double *)p;
+        S.rsel = 0;
         // the second input buffer, when the launcher reserved it
-        S.xalt = A.x2 ? (C2<R> *)align16(S.bfs + 2 * kMcStages) : nullptr;
+        S.xalt = A.x2 ? (C2<R> *)align16(S.rsum + 4 * (nt / QA_WAVE)) : nullptr;
     }
     {
         const double *src = (const double *)A.tab;
@@ -1005,7 +1020,7 @@ __global__ __launch_bounds__(RS == 0 ? 2 * kMcThreads : kMcThreads) void rk_smal
     Ctx<R, NR, RS> c{A, S, T};
     const int *lcol = nullptr;
     const C2<R> *lval = nullptr;
-    unsigned char *tail = align16(S.bfs + 2 * kMcStages);
+    unsigned char *tail = align16(S.rsum + 4 * (blockDim.x / QA_WAVE));
     if (A.x2) {  // the launcher reserved the second input buffer here
         S.xalt = (C2<R> *)tail;
         tail += sizeof(C2<R>) * (size_t)S.npad;
