@@ -1837,6 +1837,11 @@ struct qa_rk_solver {
     // the end of the run.  graph_store is the form the graph was captured in.
     bool run_store = true, graph_store = true;
     bool use_graph = true;  // QUTIP_AMD_GRAPH=0: launch the iteration directly
+    // The step-only iteration: the step stages and rk_control_kernel, none of
+    // the launches that only an output needs (QUTIP_AMD_STEP_ONLY=0: off).
+    hipGraph_t graph_step = nullptr;
+    hipGraphExec_t exec_step = nullptr;
+    bool step_only = true;
     RKCtl *ctl_host = nullptr;  // pinned mirror
     long long graph_launches = 0;
     long long setup_rhs = 0;  // derivative evaluations of the first-step estimate
@@ -1847,6 +1852,13 @@ struct qa_rk_solver {
     int run_ntimes = 0, chunk = 4;
     long long run_iters = 0, last_attempts = -1, last_outputs = -1;
     bool in_flight = false;
+    // step-only chunks: the run's output times (host copy), whether the run
+    // takes single steps, the longest step-only head allowed next, and the
+    // previous chunk's head length and starting time
+    std::vector<double> times_host;
+    bool run_single = false;
+    int ramp = 8, last_head = 0;
+    double last_front = 0.0;
 };
 
 namespace {
@@ -2121,7 +2133,12 @@ OutArgs out_args(qa_rk_solver *S) {
 
 // Enqueue one loop iteration (used both for graph capture and direct launch);
 // with_store: include rk_store_kernel (stored states, or y_last per output).
-int enqueue_iteration(qa_rk_solver *S, bool with_store) {
+// step_only: end after rk_control_kernel.  Every later launch of the full
+// iteration runs only in the dense-output and output phases, so a step-only
+// iteration whose attempt reaches an output leaves the loop in that phase:
+// the step-only iterations after it return at once, and the next full
+// iteration carries on from there exactly as the full iteration would have.
+int enqueue_iteration(qa_rk_solver *S, bool with_store, bool step_only = false) {
     int rc;
     for (int i = 0; i < S->rk_step; ++i) {
         rc = launch_stage(S, i);
@@ -2130,6 +2147,7 @@ int enqueue_iteration(qa_rk_solver *S, bool with_store) {
     hipLaunchKernelGGL(rk_control_kernel, dim3(1), dim3(256), 0, S->stream, S->ctl, S->partials,
                        stage_parts(S), S->n);
     QA_CHECK_LAUNCH();
+    if (step_only) return QA_OK;
     if (S->interpolate && S->rk_extra > S->rk_step) {
         if (!S->first_dense_from_front) {
             LinCoef lc;
@@ -2159,17 +2177,18 @@ int enqueue_iteration(qa_rk_solver *S, bool with_store) {
     return QA_OK;
 }
 
-int rebuild_graph(qa_rk_solver *S) {
-    if (S->exec) {
-        QA_HIP(hipGraphExecDestroy(S->exec));
-        S->exec = nullptr;
+// Capture one iteration (full or step-only) into *graph / *exec.
+int capture_iteration(qa_rk_solver *S, bool step_only, hipGraph_t *graph, hipGraphExec_t *exec) {
+    if (*exec) {
+        QA_HIP(hipGraphExecDestroy(*exec));
+        *exec = nullptr;
     }
-    if (S->graph) {
-        QA_HIP(hipGraphDestroy(S->graph));
-        S->graph = nullptr;
+    if (*graph) {
+        QA_HIP(hipGraphDestroy(*graph));
+        *graph = nullptr;
     }
     QA_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_iteration(S, S->run_store);
+    int rc = enqueue_iteration(S, S->run_store, step_only);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(S->stream, &g);
     if (rc) {
@@ -2177,11 +2196,30 @@ int rebuild_graph(qa_rk_solver *S) {
         return rc;
     }
     QA_HIP(e2);
-    S->graph = g;
-    QA_HIP(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
+    *graph = g;
+    QA_HIP(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return QA_OK;
+}
+
+int rebuild_graph(qa_rk_solver *S) {
+    int rc = capture_iteration(S, false, &S->graph, &S->exec);
+    if (rc) return rc;
+    if (S->step_only) {
+        rc = capture_iteration(S, true, &S->graph_step, &S->exec_step);
+        if (rc) return rc;
+    }
     S->graph_dirty = false;
     S->graph_store = S->run_store;
     return QA_OK;
+}
+
+// Launch one iteration: graph replay or direct launches.
+int launch_iteration(qa_rk_solver *S, bool step_only) {
+    if (S->use_graph) {
+        QA_HIP(hipGraphLaunch(step_only ? S->exec_step : S->exec, S->stream));
+        return QA_OK;
+    }
+    return enqueue_iteration(S, S->run_store, step_only);
 }
 
 // Retired buffers are freed in batches of kRetiredBatch (one device-wide
@@ -2197,6 +2235,8 @@ int release_retired(qa_rk_solver *S) {
 void free_all(qa_rk_solver *S) {
     if (S->exec) (void)hipGraphExecDestroy(S->exec);
     if (S->graph) (void)hipGraphDestroy(S->graph);
+    if (S->exec_step) (void)hipGraphExecDestroy(S->exec_step);
+    if (S->graph_step) (void)hipGraphDestroy(S->graph_step);
     if (S->event) (void)hipEventDestroy(S->event);
     void *ptrs[] = {S->block, S->dense_block, S->ctl, S->bi_dev, S->times_dev, S->fun_out_dev, S->fptr_dev,
                     S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev, S->td_block,
@@ -2233,6 +2273,8 @@ int rk_finish_create(qa_rk_solver *S, int order, int rk_step, int rk_extra, int 
     {
         const char *g = getenv("QUTIP_AMD_GRAPH");
         S->use_graph = !(g && g[0] == '0');
+        const char *so = getenv("QUTIP_AMD_STEP_ONLY");
+        S->step_only = !(so && so[0] == '0');
     }
     if (bi) S->bi.assign(bi, bi + (size_t)rk_extra * dorder);
     S->rtol = rtol;
@@ -2951,9 +2993,53 @@ int qa_rk_begin(qa_rk_solver *S, int ntimes, const double *times, int single_ste
     S->last_attempts = -1;
     S->last_outputs = -1;
     S->in_flight = false;
+    S->times_host.assign(times, times + ntimes);
+    S->run_single = single_step != 0;
+    S->ramp = 8;
+    S->last_head = 0;
     S->run_state = 1;
     return QA_OK;
 }
+
+namespace {
+
+// The next chunk of loop iterations after the control block mirror `h`:
+// *head step-only iterations, then full ones; returns the chunk length
+// (`chunk` full iterations when step-only iterations do not pay).  While the
+// loop steps towards an output that is, at the larger of the trial step and
+// the last chunk's mean advance per attempt, p >= 4 / 0.75 attempts away,
+// the head is 0.75 p iterations (at most `ramp`, which doubles after a head
+// whose every iteration attempted a step and halves, down to 4, after one
+// that reached its output early) and two full iterations follow; closer to
+// the output the chunk is ceil(p) + 2 full iterations.  Runs whose outputs
+// are a few attempts apart keep full chunks.  A head that reaches its output
+// early costs only launches that return at once: the results do not depend
+// on the plan.
+int plan_chunk(qa_rk_solver *S, const RKCtl &h, long long attempts_before, double front_before,
+               int chunk, int *head) {
+    *head = 0;
+    if (S->last_head > 0) {
+        const bool full = h.n_attempts - attempts_before >= S->last_head;
+        S->ramp = full ? std::min(64, 2 * S->ramp) : std::max(4, S->ramp / 2);
+    }
+    if (!S->step_only || S->run_single || h.phase != PH_STEP || h.out_idx >= S->run_ntimes)
+        return chunk;
+    if (h.n_attempts < 8 * std::max(1LL, h.n_outputs)) return chunk;
+    const double dist = S->times_host[h.out_idx] - h.t_front;
+    double dt = std::max(h.dt_try, h.dt_safe);
+    if (attempts_before >= 0 && h.n_attempts > attempts_before)
+        dt = std::max(dt, (h.t_front - front_before) / (double)(h.n_attempts - attempts_before));
+    if (!(dt > 0.0) || !(dist > 0.0)) return chunk;
+    const double p = dist / dt;
+    const int n = (int)std::min<double>(0.75 * p, (double)S->ramp);
+    if (n >= 4) {
+        *head = n;
+        return n + 2;
+    }
+    return std::min(chunk, (int)std::ceil(p) + 2);
+}
+
+}  // namespace
 
 // Non-blocking progress: when the previous chunk has finished (event
 // query), read the device phase from the pinned mirror and either report
@@ -2975,6 +3061,7 @@ int qa_rk_advance(qa_rk_solver *S, int *done) {
         return QA_OK;
     }
     hipStream_t s = S->stream;
+    int head = 0;  // step-only iterations at the start of the next chunk
     if (S->in_flight) {
         const hipError_t q = hipEventQuery(S->event);
         if (q == hipErrorNotReady) return QA_OK;
@@ -3002,20 +3089,20 @@ int qa_rk_advance(qa_rk_solver *S, int *done) {
             qa::set_error("device loop made no progress (phase %d, output %d)", h.phase, h.out_idx);
             return QA_ERR_STATE;
         }
+        const long long attempts_before = S->last_attempts;
         S->last_attempts = h.n_attempts;
         S->last_outputs = h.n_outputs;
         const int ndone = std::max(h.out_idx, 1);
         const double per_out = (double)S->run_iters / ndone;
         const long long est = (long long)(per_out * (S->run_ntimes - h.out_idx)) + 1;
         S->chunk = (int)std::max(1LL, std::min(est, 64LL));
+        S->chunk = plan_chunk(S, h, attempts_before, S->last_front, S->chunk, &head);
+        S->last_front = h.t_front;
     }
+    S->last_head = head;
     for (int k = 0; k < S->chunk; ++k) {
-        if (S->use_graph) {
-            QA_HIP(hipGraphLaunch(S->exec, s));
-        } else {
-            int rc = enqueue_iteration(S, S->run_store);
-            if (rc) return rc;
-        }
+        int rc = launch_iteration(S, k < head);
+        if (rc) return rc;
     }
     S->run_iters += S->chunk;
     S->graph_launches += S->chunk;
