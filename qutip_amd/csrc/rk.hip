@@ -55,6 +55,7 @@ constexpr int kMaxPending = 4;     // output times handled per output launch
 constexpr int kOutChunk = 16;      // functional entries per output workgroup
 constexpr int kOutThreads = 64;    // one wavefront per output workgroup
 constexpr int kStepBlocks = 256;   // threads per stage workgroup
+constexpr int kMaxChunk = 64;      // loop iterations per host chunk
 
 struct RKCtl {
     double t_prev, t_front, dt_safe, dt_int, dt_try, err, t_last;
@@ -1852,9 +1853,9 @@ struct qa_rk_solver {
     int run_ntimes = 0, chunk = 4;
     long long run_iters = 0, last_attempts = -1, last_outputs = -1;
     bool in_flight = false;
-    // step-only chunks: the run's output times (host copy), whether the run
-    // takes single steps, the longest step-only head allowed next, and the
-    // previous chunk's head length and starting time
+    // step-only iterations: the run's output times (host copy), whether the
+    // run takes single steps, the longest step-only head allowed next, the
+    // previous chunk's step-only head and starting time
     std::vector<double> times_host;
     bool run_single = false;
     int ramp = 8, last_head = 0;
@@ -3003,34 +3004,37 @@ int qa_rk_begin(qa_rk_solver *S, int ntimes, const double *times, int single_ste
 
 namespace {
 
-// The next chunk of loop iterations after the control block mirror `h`:
-// *head step-only iterations, then full ones; returns the chunk length
-// (`chunk` full iterations when step-only iterations do not pay).  While the
-// loop steps towards an output that is, at the larger of the trial step and
-// the last chunk's mean advance per attempt, p >= 4 / 0.75 attempts away,
-// the head is 0.75 p iterations (at most `ramp`, which doubles after a head
-// whose every iteration attempted a step and halves, down to 4, after one
-// that reached its output early) and two full iterations follow; closer to
-// the output the chunk is ceil(p) + 2 full iterations.  Runs whose outputs
-// are a few attempts apart keep full chunks.  A head that reaches its output
-// early costs only launches that return at once: the results do not depend
-// on the plan.
+// The next chunk after the control block mirror `h`: its first *head
+// iterations step-only, the rest full; returns the chunk length (`chunk`,
+// the full iterations the outputs left need at the recent rate, when
+// step-only iterations do not pay).  They pay on output intervals of 32 or
+// more attempts, at the larger of the trial step and the last chunk's mean
+// advance per attempt: with p the attempts still to the next output, the
+// chunk is min(0.75 p, ramp) step-only iterations and two full ones, or,
+// once 0.75 p < 4, ceil(p) + 2 full ones; every chunk of such an interval
+// ends there and the next one plans from where it ends up.  `ramp` doubles
+// (to 64) after a chunk that made at least as many attempts as it had
+// step-only iterations and halves (to 4) after one that did not (the output
+// came earlier than planned).  A step-only iteration after its output was
+// reached costs only launches that return at once: the results do not
+// depend on the plan.
 int plan_chunk(qa_rk_solver *S, const RKCtl &h, long long attempts_before, double front_before,
                int chunk, int *head) {
     *head = 0;
     if (S->last_head > 0) {
         const bool full = h.n_attempts - attempts_before >= S->last_head;
-        S->ramp = full ? std::min(64, 2 * S->ramp) : std::max(4, S->ramp / 2);
+        S->ramp = full ? std::min(kMaxChunk, 2 * S->ramp) : std::max(4, S->ramp / 2);
     }
     if (!S->step_only || S->run_single || h.phase != PH_STEP || h.out_idx >= S->run_ntimes)
         return chunk;
-    if (h.n_attempts < 8 * std::max(1LL, h.n_outputs)) return chunk;
-    const double dist = S->times_host[h.out_idx] - h.t_front;
     double dt = std::max(h.dt_try, h.dt_safe);
     if (attempts_before >= 0 && h.n_attempts > attempts_before)
         dt = std::max(dt, (h.t_front - front_before) / (double)(h.n_attempts - attempts_before));
-    if (!(dt > 0.0) || !(dist > 0.0)) return chunk;
-    const double p = dist / dt;
+    const int j = h.out_idx;
+    const double start = j > 0 ? S->times_host[j - 1] : h.t_prev;
+    if (!(dt > 0.0) || !((S->times_host[j] - start) / dt >= 32.0)) return chunk;
+    const double p = (S->times_host[j] - h.t_front) / dt;
+    if (!(p >= 0.0)) return chunk;
     const int n = (int)std::min<double>(0.75 * p, (double)S->ramp);
     if (n >= 4) {
         *head = n;
@@ -3095,7 +3099,7 @@ int qa_rk_advance(qa_rk_solver *S, int *done) {
         const int ndone = std::max(h.out_idx, 1);
         const double per_out = (double)S->run_iters / ndone;
         const long long est = (long long)(per_out * (S->run_ntimes - h.out_idx)) + 1;
-        S->chunk = (int)std::max(1LL, std::min(est, 64LL));
+        S->chunk = (int)std::max(1LL, std::min(est, (long long)kMaxChunk));
         S->chunk = plan_chunk(S, h, attempts_before, S->last_front, S->chunk, &head);
         S->last_front = h.t_front;
     }
