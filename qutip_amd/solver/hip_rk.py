@@ -280,6 +280,7 @@ class RKEngine:
         if y0.size != self.n:
             raise ValueError(f"state has {y0.size} entries, the generator {self.n}")
         _lib.call("qa_rk_set_state", self.handle, float(t0), C.c_void_p(y0.ptr))
+        self.t_set = float(t0)  # the state is the one set at t0 until a run moves it
 
     def run(self, times, single_step=False, states_out=None):
         """Returns (functionals [ntimes, nfun] complex, status, t_reached);
@@ -288,6 +289,7 @@ class RKEngine:
         out = np.zeros((len(times), max(self.nfun, 1)), dtype=np.complex128)
         st = C.c_int(0)
         tr = C.c_double(0.0)
+        self.t_set = None
         rc = _lib.lib().qa_rk_run(self.handle, len(times), _dp(times), int(single_step),
                                   _dp(out.view(np.float64)),
                                   C.c_void_p(states_out) if states_out else None,
@@ -827,8 +829,8 @@ class IntegratorHipRK(Integrator):
         """Fused Solver.run: functionals of the state at every time of
         ``tlist[1:]`` evaluated on the device.  Returns (values, started):
         values is [len(tlist)-1, nfun], or, when ``at_start`` is asked and
-        the whole-solve kernel runs (started True), [len(tlist), nfun] with
-        the functionals of the initial state at tlist[0] as row 0."""
+        the state was set at tlist[0] (started True), [len(tlist), nfun]
+        with the functionals of the initial state at tlist[0] as row 0."""
         if self._engine is None:
             raise IntegratorException("run_functionals needs the fused engine")
         times = np.asarray(tlist[1:], dtype=np.float64)
@@ -851,9 +853,16 @@ class IntegratorHipRK(Integrator):
             return vals[0], started
         self._small_stats = None
         self._leave_small()
-        self._engine.set_functionals(kinds, fptr, rows, cols, w)
-        vals, _, _ = self._engine.run(times)
-        return vals, False
+        eng = self._engine
+        # an output at the set-state time is one the device loop emits first,
+        # from the state itself, with the same functionals as the rest
+        started = (bool(at_start) and len(tlist) > 0
+                   and getattr(eng, "t_set", None) == float(tlist[0]))
+        if started:
+            times = np.asarray(tlist, dtype=np.float64)
+        eng.set_functionals(kinds, fptr, rows, cols, w)
+        vals, _, _ = eng.run(times)
+        return vals, started
 
     def stats(self):
         if self._small_stats is not None:
