@@ -25,7 +25,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from .. import data as _data
-from ..core import Qobj, QobjEvo, ket2dm
+from ..core import Qobj, QobjEvo
 from .integrator import Integrator
 from .result import Result
 
@@ -118,7 +118,20 @@ class Solver:
     # -- state preparation (solver_base.py:81-157) ------------------------------
     def _prepare_state(self, state):
         if (self.rhs.issuper or getattr(self, "_matrix_form", False)) and state.isket:
-            state = ket2dm(state)
+            # ket2dm(psi) = |psi><psi| as one outer product: the same
+            # entries, without the operator product of Qobj
+            psi = np.asarray(state.full()).ravel()
+            arr = np.outer(psi, psi.conj())
+            self._state_is_ket = False
+            self._state_dims = [list(state.dims[0]), list(state.dims[0])]
+            self._state_shape = arr.shape
+            self._normalize_output = bool(
+                self._options.get("normalize_output", False)
+                and abs(float(np.vdot(psi, psi).real) - 1) <= NORM_ATOL)
+            self._vectorized = bool(self._vectorize_state and self.rhs.issuper)
+            if self._vectorized:
+                return _data.HipDense.from_array(arr.ravel(order="F").reshape(-1, 1))
+            return _data.HipDense.from_array(arr)
         if state.isket:
             norm = state.norm()
         elif state.shape[0] == state.shape[1]:
