@@ -312,6 +312,7 @@ class RKEngine:
         return dict(zip(keys, [int(x) for x in s]))
 
     def iterate(self, count):
+        self.t_set = None
         _lib.call("qa_rk_iterate", self.handle, int(count))
 
     def apply(self, x, out=None, t=0.0):
