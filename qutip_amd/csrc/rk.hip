@@ -56,6 +56,8 @@ constexpr int kOutChunk = 16;      // functional entries per output workgroup
 constexpr int kOutThreads = 64;    // one wavefront per output workgroup
 constexpr int kStepBlocks = 256;   // threads per stage workgroup
 constexpr int kMaxChunk = 64;      // loop iterations per host chunk
+constexpr int kGraphBatch = 8;     // loop iterations per batch graph
+constexpr int kBatchAfter = 128;   // iterations an engine runs before it batches
 
 struct RKCtl {
     double t_prev, t_front, dt_safe, dt_int, dt_try, err, t_last;
@@ -1842,6 +1844,10 @@ struct qa_rk_solver {
     // the launches that only an output needs (QUTIP_AMD_STEP_ONLY=0: off).
     hipGraph_t graph_step = nullptr;
     hipGraphExec_t exec_step = nullptr;
+    // kGraphBatch iterations in one graph, full and step-only: one graph
+    // launch instead of kGraphBatch inside a chunk
+    hipGraph_t graph_batch = nullptr, graph_step_batch = nullptr;
+    hipGraphExec_t exec_batch = nullptr, exec_step_batch = nullptr;
     bool step_only = true;
     RKCtl *ctl_host = nullptr;  // pinned mirror
     long long graph_launches = 0;
@@ -2178,8 +2184,9 @@ int enqueue_iteration(qa_rk_solver *S, bool with_store, bool step_only = false) 
     return QA_OK;
 }
 
-// Capture one iteration (full or step-only) into *graph / *exec.
-int capture_iteration(qa_rk_solver *S, bool step_only, hipGraph_t *graph, hipGraphExec_t *exec) {
+// Capture `count` iterations (full or step-only) into *graph / *exec.
+int capture_iteration(qa_rk_solver *S, bool step_only, int count, hipGraph_t *graph,
+                      hipGraphExec_t *exec) {
     if (*exec) {
         QA_HIP(hipGraphExecDestroy(*exec));
         *exec = nullptr;
@@ -2189,7 +2196,8 @@ int capture_iteration(qa_rk_solver *S, bool step_only, hipGraph_t *graph, hipGra
         *graph = nullptr;
     }
     QA_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_iteration(S, S->run_store, step_only);
+    int rc = QA_OK;
+    for (int k = 0; k < count && rc == QA_OK; ++k) rc = enqueue_iteration(S, S->run_store, step_only);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(S->stream, &g);
     if (rc) {
@@ -2203,24 +2211,47 @@ int capture_iteration(qa_rk_solver *S, bool step_only, hipGraph_t *graph, hipGra
 }
 
 int rebuild_graph(qa_rk_solver *S) {
-    int rc = capture_iteration(S, false, &S->graph, &S->exec);
+    int rc = capture_iteration(S, false, 1, &S->graph, &S->exec);
     if (rc) return rc;
     if (S->step_only) {
-        rc = capture_iteration(S, true, &S->graph_step, &S->exec_step);
+        rc = capture_iteration(S, true, 1, &S->graph_step, &S->exec_step);
         if (rc) return rc;
     }
+    // the batch graphs are captured again when a chunk first needs them
+    for (hipGraphExec_t *e : {&S->exec_batch, &S->exec_step_batch})
+        if (*e) {
+            QA_HIP(hipGraphExecDestroy(*e));
+            *e = nullptr;
+        }
+    for (hipGraph_t *g : {&S->graph_batch, &S->graph_step_batch})
+        if (*g) {
+            QA_HIP(hipGraphDestroy(*g));
+            *g = nullptr;
+        }
     S->graph_dirty = false;
     S->graph_store = S->run_store;
     return QA_OK;
 }
 
-// Launch one iteration: graph replay or direct launches.
-int launch_iteration(qa_rk_solver *S, bool step_only) {
-    if (S->use_graph) {
-        QA_HIP(hipGraphLaunch(step_only ? S->exec_step : S->exec, S->stream));
-        return QA_OK;
+// Launch `count` iterations of one kind: graph replays (kGraphBatch at a
+// time while that many are left and the batch graph is there) or direct
+// launches.
+int launch_iterations(qa_rk_solver *S, bool step_only, int count) {
+    const bool batch = (step_only ? S->exec_step_batch : S->exec_batch) != nullptr;
+    while (count > 0) {
+        if (!S->use_graph) {
+            int rc = enqueue_iteration(S, S->run_store, step_only);
+            if (rc) return rc;
+            --count;
+        } else if (batch && count >= kGraphBatch) {
+            QA_HIP(hipGraphLaunch(step_only ? S->exec_step_batch : S->exec_batch, S->stream));
+            count -= kGraphBatch;
+        } else {
+            QA_HIP(hipGraphLaunch(step_only ? S->exec_step : S->exec, S->stream));
+            --count;
+        }
     }
-    return enqueue_iteration(S, S->run_store, step_only);
+    return QA_OK;
 }
 
 // Retired buffers are freed in batches of kRetiredBatch (one device-wide
@@ -2238,6 +2269,10 @@ void free_all(qa_rk_solver *S) {
     if (S->graph) (void)hipGraphDestroy(S->graph);
     if (S->exec_step) (void)hipGraphExecDestroy(S->exec_step);
     if (S->graph_step) (void)hipGraphDestroy(S->graph_step);
+    for (hipGraphExec_t e : {S->exec_batch, S->exec_step_batch})
+        if (e) (void)hipGraphExecDestroy(e);
+    for (hipGraph_t g : {S->graph_batch, S->graph_step_batch})
+        if (g) (void)hipGraphDestroy(g);
     if (S->event) (void)hipEventDestroy(S->event);
     void *ptrs[] = {S->block, S->dense_block, S->ctl, S->bi_dev, S->times_dev, S->fun_out_dev, S->fptr_dev,
                     S->fkind_dev, S->frow_dev, S->fcol_dev, S->fw_dev, S->fpart_dev, S->td_block,
@@ -3065,7 +3100,8 @@ int qa_rk_advance(qa_rk_solver *S, int *done) {
         return QA_OK;
     }
     hipStream_t s = S->stream;
-    int head = 0;  // step-only iterations at the start of the next chunk
+    int head = 0;        // step-only iterations at the start of the next chunk
+    bool idle = false;   // the stream has no work left (the last chunk completed)
     if (S->in_flight) {
         const hipError_t q = hipEventQuery(S->event);
         if (q == hipErrorNotReady) return QA_OK;
@@ -3102,12 +3138,29 @@ int qa_rk_advance(qa_rk_solver *S, int *done) {
         S->chunk = (int)std::max(1LL, std::min(est, (long long)kMaxChunk));
         S->chunk = plan_chunk(S, h, attempts_before, S->last_front, S->chunk, &head);
         S->last_front = h.t_front;
+        idle = true;
     }
     S->last_head = head;
-    for (int k = 0; k < S->chunk; ++k) {
-        int rc = launch_iteration(S, k < head);
-        if (rc) return rc;
+    // An engine that has run kBatchAfter iterations captures the batch graph
+    // of a kind when a chunk first holds kGraphBatch of it, at a point where
+    // its stream is idle (the last chunk has completed); engines that live
+    // short lives (a server's requests) seldom pay for the capture.
+    if (idle && S->use_graph && S->graph_launches >= kBatchAfter) {
+        const int nso = std::min(head, S->chunk);
+        if (nso >= kGraphBatch && !S->exec_step_batch) {
+            int rc = capture_iteration(S, true, kGraphBatch, &S->graph_step_batch,
+                                       &S->exec_step_batch);
+            if (rc) return rc;
+        }
+        if (S->chunk - nso >= kGraphBatch && !S->exec_batch) {
+            int rc = capture_iteration(S, false, kGraphBatch, &S->graph_batch, &S->exec_batch);
+            if (rc) return rc;
+        }
     }
+    int rc = launch_iterations(S, true, std::min(head, S->chunk));
+    if (rc) return rc;
+    rc = launch_iterations(S, false, S->chunk - std::min(head, S->chunk));
+    if (rc) return rc;
     S->run_iters += S->chunk;
     S->graph_launches += S->chunk;
     QA_HIP(hipMemcpyAsync(S->ctl_host, S->ctl, sizeof(RKCtl), hipMemcpyDeviceToHost, s));
